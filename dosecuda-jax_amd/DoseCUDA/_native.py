@@ -1,0 +1,118 @@
+"""ctypes binding of libdosehip.so (the C ABI declared in include/dosehip.h).
+
+The library is loaded after ``torch`` so that it binds to the HIP runtime torch
+already mapped (both use the soname libamdhip64.so.7): device pointers and
+streams handed over from torch tensors are then valid inside the library.
+There is no fallback: if the library is missing the import fails.
+"""
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must be loaded before libdosehip.so, see above)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("DOSEHIP_LIB", os.path.join(_HERE, "libdosehip.so"))
+
+DH_OK = 0
+DH_ERR_ARG = -1
+DH_ERR_HIP = -2
+
+
+class dh_grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("spacing", ctypes.c_float)]
+
+
+class dh_beam_geom(ctypes.Structure):
+    _fields_ = [("iso", ctypes.c_float * 3), ("src", ctypes.c_float * 3),
+                ("sin_g", ctypes.c_float), ("cos_g", ctypes.c_float),
+                ("sin_c", ctypes.c_float), ("cos_c", ctypes.c_float),
+                ("vsadx", ctypes.c_float), ("vsady", ctypes.c_float)]
+
+
+class dh_spot(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("mu", ctypes.c_float),
+                ("energy_id", ctypes.c_int32)]
+
+
+class dh_layer(ctypes.Structure):
+    _fields_ = [("spot_start", ctypes.c_int32), ("n_spots", ctypes.c_int32),
+                ("energy_id", ctypes.c_int32), ("r80", ctypes.c_float), ("energy", ctypes.c_float)]
+
+
+class dh_beam_model(ctypes.Structure):
+    _fields_ = [("idd_sigma", ctypes.c_void_p), ("depths", ctypes.c_void_p), ("dvp", ctypes.c_void_p),
+                ("n_energies", ctypes.c_int32), ("lut_len", ctypes.c_int32), ("dvp_len", ctypes.c_int32),
+                ("depth_uniform", ctypes.c_int32), ("depth0", ctypes.c_float), ("ddepth", ctypes.c_float)]
+
+
+class dh_spot_set(ctypes.Structure):
+    _fields_ = [("spots", ctypes.c_void_p), ("layers", ctypes.c_void_p),
+                ("n_spots", ctypes.c_int32), ("n_layers", ctypes.c_int32)]
+
+
+# exported symbol -> (restype, argtypes); every declaration of include/dosehip.h
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "dh_version": (ctypes.c_int, []),
+    "dh_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "dh_padded_len": (ctypes.c_int64, [dh_grid]),
+    "dh_rlsp_from_hu": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.c_int32, _P]),
+    "dh_rlsp_pad": (ctypes.c_int, [_P, _P, dh_grid, _P]),
+    "dh_wet_raymarch": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom), _P]),
+    "dh_wet_smooth": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom), _P]),
+    "dh_pb_workspace_size": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, dh_grid]),
+    "dh_pb_forward": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom),
+                                     ctypes.POINTER(dh_beam_model), ctypes.POINTER(dh_spot_set),
+                                     ctypes.c_float, ctypes.c_int32, _P, ctypes.c_int64, _P]),
+    "dh_pb_adjoint": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom),
+                                     ctypes.POINTER(dh_beam_model), ctypes.POINTER(dh_spot_set),
+                                     ctypes.c_float, _P, _P, _P, ctypes.c_int64, _P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libdosehip.so once and declare its signatures."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libdosehip.so not found at %s; build it with "
+                              "`make -C dosecuda-jax_amd/csrc` (or __graft_entry__.build())" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().dh_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what):
+    """Map a library status to the reference extension's exceptions
+    (dosemodule.cu:189-197): bad arguments -> ValueError, HIP failures ->
+    RuntimeError with the message prefixed "CUDA error: " as the reference does."""
+    if rc == DH_OK:
+        return
+    msg = "%s: %s" % (what, last_error())
+    if rc == DH_ERR_ARG:
+        raise ValueError(msg)
+    raise RuntimeError("CUDA error: " + msg)
+
+
+def ptr(t):
+    """Device (or host) address of a tensor as a ctypes void pointer."""
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,113 @@
+"""Drop-in for the reference's ``dose_kernels`` extension module (proton part).
+
+``proton_raytrace_cuda(model, vol, beam, gpu_id)`` and
+``proton_spot_cuda(model, vol, wet, beam, gpu_id)`` keep the names, argument
+meaning, attribute protocol, return values and exceptions of
+DoseCUDA/dose_kernels/dosemodule.cu:104-374:
+
+* attributes are read by name; a missing one raises AttributeError, a
+  non-numeric float TypeError, and an array that is not a numpy float32 array
+  of the required rank raises
+  ``ValueError("'<attr>' must be <n>-dimensional and of type float.")``
+  (dosemodule.cu:50-66);
+* outputs are new C-contiguous float32 numpy arrays shaped like the input
+  volume; inputs are not modified;
+* device failures raise RuntimeError("CUDA error: ...") (dosemodule.cu:193-195).
+
+The work runs on the HIP device ``gpu_id`` through libdosehip.so, on torch's
+current stream of that device; the call returns once the result is on the host,
+as the reference's blocking copy does.  The photon entry point
+(``photon_dose_cuda``) is not provided.
+"""
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import device as D
+
+_native = N.load()  # fail at import if the HIP library is missing
+
+
+def _getfloat(obj, attr):
+    # PyFloat_AsDouble semantics: numbers (and objects with __float__) only
+    val = getattr(obj, attr)
+    if isinstance(val, (str, bytes, bytearray)):
+        raise TypeError("must be real number, not %s" % type(val).__name__)
+    return float(val)
+
+
+def _getarray(obj, attr, ndim):
+    arr = getattr(obj, attr)
+    if not (isinstance(arr, np.ndarray) and arr.ndim == ndim and arr.dtype == np.float32):
+        raise ValueError("'%s' must be %d-dimensional and of type float." % (attr, ndim))
+    return arr
+
+
+def _device(gpu_id):
+    if not torch.cuda.is_available():
+        raise RuntimeError("CUDA error: no HIP device available")
+    gpu_id = int(gpu_id)
+    if gpu_id < 0 or gpu_id >= torch.cuda.device_count():
+        raise RuntimeError("CUDA error: invalid device ordinal %d" % gpu_id)
+    return torch.device("cuda", gpu_id)
+
+
+def proton_raytrace_cuda(model, vol, beam, gpu_id):
+    """Smoothed water-equivalent thickness (cm) of every voxel for one beam
+    (proton_raytrace, dosemodule.cu:104-201)."""
+    vsadx = _getfloat(model, "VSADX")
+    vsady = _getfloat(model, "VSADY")
+    rlsp = _getarray(vol, "voxel_data", 3)
+    spacing = _getarray(vol, "spacing", 1)
+    origin = _getarray(vol, "origin", 1)
+    iso = _getarray(beam, "iso", 1)
+    ga = _getfloat(beam, "gantry_angle")
+    ta = _getfloat(beam, "couch_angle")
+
+    dev = _device(gpu_id)
+    geo = D.make_geom(D.beam_geometry(iso, origin, ga, ta, vsadx, vsady))
+    grid = D.make_grid(rlsp.shape, spacing[0])
+    with torch.cuda.device(dev):
+        d_rlsp = torch.from_numpy(np.ascontiguousarray(rlsp)).to(dev)
+        padded = D.rlsp_pad(d_rlsp, grid, dev)
+        _, wet = D.wet_from_padded(padded, grid, geo, dev)
+        out = wet.cpu().numpy()
+    return np.ascontiguousarray(out)
+
+
+def proton_spot_cuda(model, vol, wet, beam, gpu_id, cutoff_sigma=D.DEFAULT_CUTOFF_SIGMA):
+    """Pencil-beam dose of one beam from its WET volume (proton_spot,
+    dosemodule.cu:243-374).  Not multiplied by the number of fractions.
+
+    ``cutoff_sigma`` (keyword, not in the reference) bounds the beam's-eye-view
+    reach of a spot; 0 evaluates every spot for every voxel."""
+    vsadx = _getfloat(model, "VSADX")
+    vsady = _getfloat(model, "VSADY")
+    dvp = _getarray(model, "divergence_params", 2)
+    _getarray(model, "lut_depths", 2)
+    _getarray(model, "lut_sigmas", 2)
+    _getarray(model, "lut_idds", 2)
+    _getarray(vol, "voxel_data", 3)
+    spacing = _getarray(vol, "spacing", 1)
+    origin = _getarray(vol, "origin", 1)
+    wet_arr = _getarray(wet, "voxel_data", 3)
+    iso = _getarray(beam, "iso", 1)
+    spots = _getarray(beam, "spot_list", 2)
+    ga = _getfloat(beam, "gantry_angle")
+    ta = _getfloat(beam, "couch_angle")
+    if spots.shape[1] != 4:
+        raise ValueError("'spot_list' must be 2-dimensional and of type float.")
+
+    dev = _device(gpu_id)
+    geo = D.make_geom(D.beam_geometry(iso, origin, ga, ta, vsadx, vsady))
+    grid = D.make_grid(wet_arr.shape, spacing[0])
+    with torch.cuda.device(dev):
+        dmodel = D.DeviceBeamModel(model, dev)
+        dspots = D.DeviceSpots(spots, dvp, dev)
+        d_wet = torch.from_numpy(np.ascontiguousarray(wet_arr)).to(dev)
+        dose = torch.empty_like(d_wet)
+        ws, nbytes = D.alloc_workspace(dspots.n_spots, dspots.n_layers, grid, dev)
+        D.pb_forward(d_wet, dose, grid, geo, dmodel, dspots, ws, nbytes, cutoff_sigma=cutoff_sigma)
+        out = dose.cpu().numpy()
+    return np.ascontiguousarray(out)

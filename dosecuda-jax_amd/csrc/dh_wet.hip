@@ -1,0 +1,275 @@
+// HU -> RLSP and water-equivalent-thickness kernels (gfx950).
+//
+//  k_rlsp_from_hu  fused np.interp HU->RLSP into the padded volume layout
+//  k_rlsp_pad      unpadded -> padded copy (drop-in path: RLSP comes from the host)
+//  k_wet_raymarch  per-voxel 1 mm march toward the source, manual trilinear
+//  k_wet_smooth    6-direction lateral mean of the raw WET
+//
+// The padded layout (one zero plane past the high end of every axis) lets the
+// ray-march read all 8 trilinear corners without per-corner bounds tests: a
+// sample with continuous index c in [0, n) has floor(c) + 1 <= n.
+#include "dh_common.h"
+
+namespace dh {
+
+// ---------------------------------------------------------------------------
+// HU -> RLSP (IMPTDoseGrid.RLSPFromHU, plan_impt.py:158-168)
+constexpr int MAX_KNOTS = 32;
+struct Knots {
+    double hu[MAX_KNOTS];
+    double rlsp[MAX_KNOTS];
+    int n;
+};
+
+__device__ __forceinline__ double interp_knots(const Knots &kn, double x) {
+    // np.interp: clamp outside [hu[0], hu[n-1]]; otherwise linear on the
+    // bracketing interval, slope * (x - hu[j]) + rlsp[j].
+    if (!(x == x)) return x;  // NaN stays NaN
+    if (x <= kn.hu[0]) return kn.rlsp[0];
+    if (x >= kn.hu[kn.n - 1]) return kn.rlsp[kn.n - 1];
+    int j = 0;
+    for (int t = 1; t < kn.n - 1; ++t) j = (x >= kn.hu[t]) ? t : j;
+    double slope = __ddiv_rn(kn.rlsp[j + 1] - kn.rlsp[j], kn.hu[j + 1] - kn.hu[j]);
+    return __dadd_rn(__dmul_rn(slope, x - kn.hu[j]), kn.rlsp[j]);
+}
+
+__global__ __launch_bounds__(256) void k_rlsp_from_hu(const float *__restrict__ hu, float *__restrict__ out,
+                                                     Grid g, Knots kn, long long npad) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npad) return;
+    const int px = g.nx + 1, py = g.ny + 1;
+    int i = (int)(t % px);
+    long long r = t / px;
+    int j = (int)(r % py);
+    int k = (int)(r / py);
+    float v = 0.0f;
+    if (i < g.nx && j < g.ny && k < g.nz)
+        v = (float)interp_knots(kn, (double)hu[i + (long long)g.nx * (j + (long long)g.ny * k)]);
+    out[t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_rlsp_pad(const float *__restrict__ in, float *__restrict__ out, Grid g,
+                                                 long long npad) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npad) return;
+    const int px = g.nx + 1, py = g.ny + 1;
+    int i = (int)(t % px);
+    long long r = t / px;
+    int j = (int)(r % py);
+    int k = (int)(r / py);
+    float v = 0.0f;
+    if (i < g.nx && j < g.ny && k < g.nz) v = in[i + (long long)g.nx * (j + (long long)g.ny * k)];
+    out[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Ray-march (rayTraceKernel, CudaClasses.cu:180-228), with the sampling and
+// bounds of impt_jax.py:254-281: sample n sits at p = v + n*u (1 mm steps,
+// u = unit vector to the source), continuous index c = (p + iso) / s; a sample
+// counts only if every c_a is in [0, n_a); its RLSP is the zero-padded
+// trilinear interpolant.  WET(cm) = -0.05 + 0.1 * sum max(rlsp, 0).
+//
+// The samples inside the grid form one run n = 0 .. n_exit-1 (the grid is
+// convex and sample 0 is the voxel centre).  The run length is found
+// analytically; all but the last couple of samples are provably inside by a
+// margin far above float32 rounding and are read without tests, the tail is
+// tested sample by sample exactly as the per-step mask does.
+struct F2 {
+    float a, b;
+};
+
+__device__ __forceinline__ float trilinear_padded(const float *__restrict__ rho, int pnx, int pnxy, float cx,
+                                                  float cy, float cz) {
+    float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    float ax = cx - fx, ay = cy - fy, az = cz - fz;
+    unsigned base = (unsigned)(int)fx + (unsigned)pnx * ((unsigned)(int)fy) + (unsigned)pnxy * ((unsigned)(int)fz);
+    F2 r00, r10, r01, r11;
+    __builtin_memcpy(&r00, rho + base, sizeof(F2));
+    __builtin_memcpy(&r10, rho + base + pnx, sizeof(F2));
+    __builtin_memcpy(&r01, rho + base + pnxy, sizeof(F2));
+    __builtin_memcpy(&r11, rho + base + pnxy + pnx, sizeof(F2));
+    float x00 = fmaf(ax, r00.b - r00.a, r00.a);
+    float x10 = fmaf(ax, r10.b - r10.a, r10.a);
+    float x01 = fmaf(ax, r01.b - r01.a, r01.a);
+    float x11 = fmaf(ax, r11.b - r11.a, r11.a);
+    float y0 = fmaf(ay, x10 - x00, x00);
+    float y1 = fmaf(ay, x11 - x01, x01);
+    return fmaf(az, y1 - y0, y0);
+}
+
+// Steps until the ray leaves [0, n) along one axis (real-valued).
+__device__ __forceinline__ float axis_exit(float c0, float du, int n) {
+    if (du > 0.0f) return ((float)n - c0) / du;
+    if (du < 0.0f) return c0 / (-du);
+    return 3.0e38f;
+}
+
+__global__ __launch_bounds__(256) void k_wet_raymarch(const float *__restrict__ rho, float *__restrict__ wet, Grid g,
+                                                     Geom b, BrickGrid bg, unsigned nblocks) {
+    const unsigned blk = xcd_remap(blockIdx.x, nblocks);
+    const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
+    if (brick >= bg.count()) return;
+    const int lane = threadIdx.x & 63;
+    const int bi = (int)(brick % bg.bx);
+    const long long bt = brick / bg.bx;
+    const int bj = (int)(bt % bg.by), bk = (int)(bt / bg.by);
+    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
+    if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+
+    const float3 v = voxel_xyz(g, b, i, j, k);
+    const float dx = b.src[0] - v.x, dy = b.src[1] - v.y, dz = b.src[2] - v.z;
+    // correctly rounded sqrt and divisions: an axis-aligned ray gets an exactly
+    // axis-aligned u, so its samples land exactly on grid planes as in float64
+    const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float ux = dx / len, uy = dy / len, uz = dz / len;
+
+    const float c0x = (v.x + b.iso[0]) * g.inv_s, c0y = (v.y + b.iso[1]) * g.inv_s, c0z = (v.z + b.iso[2]) * g.inv_s;
+    float t_exit = fminf(axis_exit(c0x, ux * g.inv_s, g.nx),
+                         fminf(axis_exit(c0y, uy * g.inv_s, g.ny), axis_exit(c0z, uz * g.inv_s, g.nz)));
+    t_exit = fminf(t_exit, 1.0e7f);
+    const int n_safe = max(0, (int)floorf(t_exit) - 1);
+    const int n_end = (int)floorf(t_exit) + 3;
+
+    const int pnx = g.nx + 1, pnxy = (g.nx + 1) * (g.ny + 1);
+    float acc = -0.05f;
+    int n = 0;
+    for (; n < n_safe; ++n) {
+        const float fn = (float)n;
+        const float cx = (fmaf(ux, fn, v.x) + b.iso[0]) * g.inv_s;
+        const float cy = (fmaf(uy, fn, v.y) + b.iso[1]) * g.inv_s;
+        const float cz = (fmaf(uz, fn, v.z) + b.iso[2]) * g.inv_s;
+        acc = fmaf(fmaxf(trilinear_padded(rho, pnx, pnxy, cx, cy, cz), 0.0f), 0.1f, acc);
+    }
+    for (; n < n_end; ++n) {
+        const float fn = (float)n;
+        const float cx = (fmaf(ux, fn, v.x) + b.iso[0]) * g.inv_s;
+        const float cy = (fmaf(uy, fn, v.y) + b.iso[1]) * g.inv_s;
+        const float cz = (fmaf(uz, fn, v.z) + b.iso[2]) * g.inv_s;
+        const bool inside = cx >= 0.0f && cx < (float)g.nx && cy >= 0.0f && cy < (float)g.ny && cz >= 0.0f &&
+                            cz < (float)g.nz;
+        if (inside) acc = fmaf(fmaxf(trilinear_padded(rho, pnx, pnxy, cx, cy, cz), 0.0f), 0.1f, acc);
+    }
+    wet[i + (long long)g.nx * (j + (long long)g.ny * k)] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Lateral WET smoothing (smoothRayKernel, IMPTClasses.cu:139-202;
+// _smooth_wet_kernel, impt_jax.py:317-418).  Directions k*60 deg use exact
+// table values; the neighbour index is round-half-to-even of (x + iso) / s
+// (true division, so exact half-voxel ties stay exact).
+__constant__ float c_dir_cos[6] = {1.0f, 0.5f, -0.5f, -1.0f, -0.5f, 0.5f};
+__constant__ float c_dir_sin[6] = {0.0f, 0.86602540378443865f, 0.86602540378443865f,
+                                   0.0f, -0.86602540378443865f, -0.86602540378443865f};
+
+__global__ __launch_bounds__(256) void k_wet_smooth(const float *__restrict__ raw, float *__restrict__ out, Grid g,
+                                                   Geom b, long long nvox) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvox) return;
+    const int i = (int)(t % g.nx);
+    const long long r = t / g.nx;
+    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+
+    const float wc = raw[t];
+    const float max_dr = fminf(wc * 10.0f, 10.0f);
+    const float3 h = image_to_head(b, voxel_xyz(g, b, i, j, k));
+    float sum = wc;
+    int cnt = 1;
+    for (int a = 0; a < 6; ++a) {
+        const float ca = c_dir_cos[a], sa = c_dir_sin[a];
+        for (int d = 1; d <= 9; ++d) {
+            const float dr = (float)d;
+            if (!(dr < max_dr)) break;
+            const float3 ip = head_to_image(b, make_float3(h.x + dr * ca, h.y + dr * sa, h.z));
+            const int ni = (int)rintf((ip.x + b.iso[0]) / g.s);
+            const int nj = (int)rintf((ip.y + b.iso[1]) / g.s);
+            const int nk = (int)rintf((ip.z + b.iso[2]) / g.s);
+            if ((unsigned)ni < (unsigned)g.nx && (unsigned)nj < (unsigned)g.ny && (unsigned)nk < (unsigned)g.nz) {
+                sum += raw[ni + (long long)g.nx * (nj + (long long)g.ny * nk)];
+                ++cnt;
+            }
+        }
+    }
+    out[t] = sum / (float)cnt;
+}
+
+static bool grid_ok(const dh_grid &g) {
+    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0.0f)) {
+        set_error("invalid grid %d x %d x %d, spacing %g", g.nx, g.ny, g.nz, (double)g.spacing);
+        return false;
+    }
+    // 32-bit element offsets inside the kernels
+    if ((double)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) >= 2147483647.0) {
+        set_error("grid %d x %d x %d too large (padded volume must stay below 2^31 voxels)", g.nx, g.ny, g.nz);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace dh
+
+using namespace dh;
+
+extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, const double *hu_knots,
+                               const double *rlsp_knots, int32_t n_knots, void *stream) {
+    if (!grid_ok(g)) return DH_ERR_ARG;
+    if (!hu || !rlsp_padded || !hu_knots || !rlsp_knots || n_knots < 2 || n_knots > MAX_KNOTS) {
+        set_error("dh_rlsp_from_hu: bad arguments (n_knots=%d, max %d)", n_knots, MAX_KNOTS);
+        return DH_ERR_ARG;
+    }
+    Knots kn;
+    kn.n = n_knots;
+    for (int t = 0; t < n_knots; ++t) {
+        kn.hu[t] = hu_knots[t];
+        kn.rlsp[t] = rlsp_knots[t];
+        if (t && !(hu_knots[t] > hu_knots[t - 1])) {
+            set_error("dh_rlsp_from_hu: HU knots must be strictly ascending");
+            return DH_ERR_ARG;
+        }
+    }
+    for (int t = n_knots; t < MAX_KNOTS; ++t) kn.hu[t] = kn.rlsp[t] = 0.0;
+    const long long npad = dh_padded_len(g);
+    const unsigned nb = (unsigned)((npad + 255) / 256);
+    hipLaunchKernelGGL(k_rlsp_from_hu, dim3(nb), dim3(256), 0, (hipStream_t)stream, hu, rlsp_padded, make_grid(g),
+                       kn, npad);
+    return check_launch("k_rlsp_from_hu");
+}
+
+extern "C" int dh_rlsp_pad(const float *rlsp, float *rlsp_padded, dh_grid g, void *stream) {
+    if (!grid_ok(g)) return DH_ERR_ARG;
+    if (!rlsp || !rlsp_padded) {
+        set_error("dh_rlsp_pad: null pointer");
+        return DH_ERR_ARG;
+    }
+    const long long npad = dh_padded_len(g);
+    const unsigned nb = (unsigned)((npad + 255) / 256);
+    hipLaunchKernelGGL(k_rlsp_pad, dim3(nb), dim3(256), 0, (hipStream_t)stream, rlsp, rlsp_padded, make_grid(g), npad);
+    return check_launch("k_rlsp_pad");
+}
+
+extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid g, const dh_beam_geom *geom,
+                               void *stream) {
+    if (!grid_ok(g)) return DH_ERR_ARG;
+    if (!rlsp_padded || !wet_raw || !geom) {
+        set_error("dh_wet_raymarch: null pointer");
+        return DH_ERR_ARG;
+    }
+    const Grid gg = make_grid(g);
+    const BrickGrid bg = brick_grid(gg);
+    const unsigned nb = (unsigned)((bg.count() + 3) / 4);
+    hipLaunchKernelGGL(k_wet_raymarch, dim3(nb), dim3(256), 0, (hipStream_t)stream, rlsp_padded, wet_raw, gg,
+                       make_geom(*geom), bg, nb);
+    return check_launch("k_wet_raymarch");
+}
+
+extern "C" int dh_wet_smooth(const float *wet_raw, float *wet, dh_grid g, const dh_beam_geom *geom, void *stream) {
+    if (!grid_ok(g)) return DH_ERR_ARG;
+    if (!wet_raw || !wet || !geom || wet_raw == wet) {
+        set_error("dh_wet_smooth: null or aliased pointers");
+        return DH_ERR_ARG;
+    }
+    const long long nvox = (long long)g.nx * g.ny * g.nz;
+    const unsigned nb = (unsigned)((nvox + 255) / 256);
+    hipLaunchKernelGGL(k_wet_smooth, dim3(nb), dim3(256), 0, (hipStream_t)stream, wet_raw, wet, make_grid(g),
+                       make_geom(*geom), nvox);
+    return check_launch("k_wet_smooth");
+}

@@ -1,0 +1,167 @@
+/*
+ * dosehip.h - C ABI of the gfx950 proton pencil-beam dose library (libdosehip.so).
+ *
+ * Replaces the proton half of the reference extension `dose_kernels`
+ * (DoseCUDA/dose_kernels/dosemodule.cu:104-374, IMPTClasses.cu:274-340):
+ *   proton_raytrace_cuda  ->  dh_rlsp_pad / dh_rlsp_from_hu, dh_wet_raymarch, dh_wet_smooth
+ *   proton_spot_cuda      ->  dh_pb_forward
+ *   (jax.grad of DoseCUDA/Jax/impt_jax.py, no CUDA counterpart) -> dh_pb_adjoint
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers (HIP global memory on the current
+ *    device), borrowed for the duration of the stream work; the library never
+ *    allocates, frees or synchronises.  Work is enqueued on `stream`
+ *    (a hipStream_t passed as void*, NULL = default stream).
+ *  - Volumes are float32, indexed v = i + nx*(j + ny*k) with i along x
+ *    (numpy axis 2 of a (z, y, x) array), as in CudaClasses.cuh:189-191.
+ *  - Padded RLSP volumes have one extra zero plane at the high end of every
+ *    axis: (nx+1)*(ny+1)*(nz+1) floats, index i + (nx+1)*(j + (ny+1)*k).
+ *  - Return value 0 = success; negative = error, message via dh_last_error().
+ *  - Entry points are re-entrant; per-thread error text.
+ */
+#ifndef DOSEHIP_H
+#define DOSEHIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_OK            0
+#define DH_ERR_ARG      -1   /* invalid argument (sizes, null pointers, workspace too small) */
+#define DH_ERR_HIP      -2   /* HIP runtime error from a launch */
+
+/* Dose grid: sizes along x (numpy axis 2), y (axis 1), z (axis 0); isotropic
+ * spacing in mm (only spacing[0] is used, dosemodule.cu:146). */
+typedef struct dh_grid {
+    int32_t nx, ny, nz;
+    float spacing;
+} dh_grid;
+
+/* Beam geometry (CudaBeam, CudaClasses.cu:17-51), all float32:
+ *  iso      isocentre minus grid origin (mm)            dosemodule.cu:160-164
+ *  src      source position relative to the isocentre  = R_couch(R_gantry((0, SAD, 0)))
+ *  sin_g/cos_g  of the adjusted gantry angle (gantry + 180 deg, dosemodule.cu:152)
+ *  sin_c/cos_c  of the couch angle
+ *  vsadx/vsady  virtual source-axis distances (mm) */
+typedef struct dh_beam_geom {
+    float iso[3];
+    float src[3];
+    float sin_g, cos_g, sin_c, cos_c;
+    float vsadx, vsady;
+} dh_beam_geom;
+
+/* One spot, sorted by energy id (Spot, IMPTClasses.cuh:8-15). */
+typedef struct dh_spot {
+    float x, y;      /* mm at the isocentre plane, beam's-eye view */
+    float mu;        /* weight */
+    int32_t energy_id;
+} dh_spot;
+
+/* One energy layer = a run of equal energy ids in the sorted spot array
+ * (Layer, IMPTClasses.cuh:17-26; built like importLayers, IMPTClasses.cu:37-67). */
+typedef struct dh_layer {
+    int32_t spot_start;
+    int32_t n_spots;
+    int32_t energy_id;
+    float r80;       /* mm */
+    float energy;    /* MeV */
+} dh_layer;
+
+/* Beam-model lookup tables on the device (IMPTBeamModel, plan_impt.py:63-140).
+ *  idd_sigma  float2 [n_energies][lut_len] = (IDD, sigma_ms) interleaved
+ *  depths     float  [n_energies][lut_len] (mm), ascending per energy
+ *  dvp        float  [n_energies][dvp_len] = (E, R80, A, B, C, ...)
+ *  depth_uniform != 0 asserts depths[e][k] == depth0 + k*ddepth for every e, k
+ *  (true of the shipped tables: 0..399 mm); the lookup then indexes directly
+ *  instead of bisecting (lowerBound, CudaClasses.cuh:130-144). */
+typedef struct dh_beam_model {
+    const float *idd_sigma;
+    const float *depths;
+    const float *dvp;
+    int32_t n_energies;
+    int32_t lut_len;
+    int32_t dvp_len;
+    int32_t depth_uniform;
+    float depth0;
+    float ddepth;
+} dh_beam_model;
+
+/* Spots of one beam on the device. */
+typedef struct dh_spot_set {
+    const dh_spot *spots;    /* [n_spots], sorted by energy id */
+    const dh_layer *layers;  /* [n_layers] */
+    int32_t n_spots;
+    int32_t n_layers;
+} dh_spot_set;
+
+/* Library version, e.g. 100 for 0.1.0. */
+int dh_version(void);
+
+/* Copy the last error message of the calling thread into buf (NUL-terminated,
+ * truncated to len).  Returns the length of the full message. */
+int dh_last_error(char *buf, size_t len);
+
+/* Number of floats of a padded RLSP volume: (nx+1)*(ny+1)*(nz+1). */
+int64_t dh_padded_len(dh_grid g);
+
+/* HU -> RLSP, piecewise linear through (hu_knots, rlsp_knots), clamped at both
+ * ends (np.interp, plan_impt.py:158-168), written as a padded volume.
+ * hu: [nz*ny*nx] device; knots: HOST float64 arrays (as parsed from
+ * HU_RLSP.csv), strictly ascending hu, 2 <= n_knots <= 32.  The interpolation
+ * runs in float64 and is rounded to float32, like the reference's
+ * np.interp(...).astype(np.single). */
+int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g,
+                    const double *hu_knots, const double *rlsp_knots, int32_t n_knots,
+                    void *stream);
+
+/* Copy an unpadded RLSP volume into the padded layout. */
+int dh_rlsp_pad(const float *rlsp, float *rlsp_padded, dh_grid g, void *stream);
+
+/* Raw water-equivalent thickness (cm) of every voxel: 1 mm steps from the voxel
+ * centre toward the source, zero-padded trilinear RLSP, samples with continuous
+ * index outside [0, n) dropped, WET = -0.05 + 0.1 * sum(max(rlsp, 0))
+ * (rayTraceKernel, CudaClasses.cu:180-228; bounds as impt_jax.py:262-277). */
+int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid g,
+                    const dh_beam_geom *geom, void *stream);
+
+/* Lateral smoothing of raw WET: mean of the voxel and the nearest voxels
+ * (round half to even) at dr = 1..9 mm, dr < 10*WET, along 6 beam's-eye-view
+ * directions k*60 deg that fall inside the grid (smoothRayKernel,
+ * IMPTClasses.cu:139-202; impt_jax.py:317-418). */
+int dh_wet_smooth(const float *wet_raw, float *wet, dh_grid g,
+                  const dh_beam_geom *geom, void *stream);
+
+/* Bytes of device workspace dh_pb_forward / dh_pb_adjoint need for n_spots
+ * spots in n_layers layers (256-byte aligned pointer required). */
+int64_t dh_pb_workspace_size(int32_t n_spots, int32_t n_layers, dh_grid g);
+
+/* Double-Gaussian pencil-beam dose of one beam (pencilBeamKernel,
+ * IMPTClasses.cu:204-271; impt_jax.py:425-541).  wet: smoothed WET (cm).
+ * dose: overwritten (accumulate == 0) or added to (accumulate != 0).
+ * cutoff_sigma > 0 skips (voxel brick, spot) pairs whose beam's-eye-view
+ * distance exceeds cutoff_sigma * (largest halo sigma of the brick); each
+ * skipped term is below exp(-cutoff_sigma^2 / 2) of that spot's peak.
+ * cutoff_sigma <= 0 evaluates every pair as the reference does. */
+int dh_pb_forward(const float *wet, float *dose, dh_grid g, const dh_beam_geom *geom,
+                  const dh_beam_model *model, const dh_spot_set *spots,
+                  float cutoff_sigma, int32_t accumulate,
+                  void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Adjoint of dh_pb_forward: given grad_dose = dL/dDose (per voxel), writes
+ *   grad_mu[s]      = dL/dmu_s
+ *   grad_xy[2s+0/1] = dL/dx_s, dL/dy_s
+ * for the sorted spots.  WET-derived per-voxel terms are recomputed, not stored.
+ * Deterministic (fixed-order two-pass reduction, no atomics). */
+int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g,
+                  const dh_beam_geom *geom, const dh_beam_model *model,
+                  const dh_spot_set *spots, float cutoff_sigma,
+                  float *grad_mu, float *grad_xy,
+                  void *workspace, int64_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DOSEHIP_H */
