@@ -1,0 +1,114 @@
+"""The C ABI library: loads, exports every declared symbol, struct layouts agree
+with the header, argument errors come back without touching a GPU."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dosehip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t)\s+(dh_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from DoseCUDA import _native as N
+    lib = N.load()
+    names = _declared()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in N.SIGNATURES, "ctypes binding missing for %s" % name
+    assert lib.dh_version() >= 100
+
+
+def test_struct_layout_matches_c(tmp_path):
+    from DoseCUDA import _native as N
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "dosehip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(dh_grid), sizeof(dh_beam_geom), sizeof(dh_spot),
+         sizeof(dh_layer), sizeof(dh_beam_model), sizeof(dh_spot_set));
+  printf("%zu %zu %zu %zu\n", offsetof(dh_beam_geom, sin_g), offsetof(dh_beam_model, n_energies),
+         offsetof(dh_beam_model, depth0), offsetof(dh_spot_set, n_spots));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    sizes = [int(v) for v in out[0].split()]
+    offs = [int(v) for v in out[1].split()]
+    assert sizes == [ctypes.sizeof(c) for c in (N.dh_grid, N.dh_beam_geom, N.dh_spot, N.dh_layer,
+                                                N.dh_beam_model, N.dh_spot_set)]
+    assert offs == [N.dh_beam_geom.sin_g.offset, N.dh_beam_model.n_energies.offset,
+                    N.dh_beam_model.depth0.offset, N.dh_spot_set.n_spots.offset]
+
+
+def test_argument_errors_without_gpu():
+    from DoseCUDA import _native as N
+    lib = N.load()
+    bad = N.dh_grid(0, 4, 4, 2.0)
+    geo = N.dh_beam_geom()
+    rc = lib.dh_wet_raymarch(None, None, bad, ctypes.byref(geo), None)
+    assert rc == N.DH_ERR_ARG
+    assert "invalid grid" in N.last_error()
+    with pytest.raises(ValueError):
+        N.check(rc, "dh_wet_raymarch")
+    good = N.dh_grid(8, 8, 8, 2.0)
+    assert lib.dh_padded_len(good) == 9 * 9 * 9
+    huge = N.dh_grid(2000, 2000, 2000, 1.0)
+    assert lib.dh_rlsp_pad(ctypes.c_void_p(16), ctypes.c_void_p(16), huge, None) == N.DH_ERR_ARG
+    knots = (ctypes.c_double * 2)(0.0, 0.0)
+    assert lib.dh_rlsp_from_hu(ctypes.c_void_p(16), ctypes.c_void_p(16), good, knots, knots, 2, None) \
+        == N.DH_ERR_ARG  # knots not ascending
+    assert lib.dh_pb_workspace_size(0, 0, good) == 0
+    ws = lib.dh_pb_workspace_size(1000, 10, good)
+    assert ws >= 1000 * 32 and ws % 256 == 0
+    ss = N.dh_spot_set(None, None, 5, 1)
+    model = N.dh_beam_model()
+    rc = lib.dh_pb_forward(ctypes.c_void_p(16), ctypes.c_void_p(16), good, ctypes.byref(geo),
+                           ctypes.byref(model), ctypes.byref(ss), 0.0, 0, None, 0, None)
+    assert rc == N.DH_ERR_ARG
+
+
+def test_geometry_matches_oracle_restatement():
+    from DoseCUDA.geometry import beam_geometry
+    from oracle import impt_oracle as O
+    for ga, ca in [(0, 0), (90, 0), (120, 30), (240, -15), (359.5, 270)]:
+        a = beam_geometry(np.array([1.5, -2.0, 3.0], np.float32), np.array([-100, -90, -80], np.float32), ga, ca,
+                          1340.0, 1930.0)
+        b = O.beam_geometry(np.array([1.5, -2.0, 3.0], np.float32), np.array([-100, -90, -80], np.float32), ga,
+                            ca, 1340.0, 1930.0)
+        flat = np.concatenate([a["iso"], a["src"], [a["sin_g"], a["cos_g"], a["sin_c"], a["cos_c"],
+                                                    a["vsadx"], a["vsady"]]])
+        np.testing.assert_allclose(flat, b, rtol=0, atol=1e-12)
+
+
+def test_layer_table_matches_oracle(plan_models):
+    from DoseCUDA.geometry import sort_spots, layer_table
+    from oracle import impt_oracle as O
+    dvp = plan_models.beam_models[0].divergence_params
+    rng = np.random.default_rng(0)
+    sl = np.column_stack([rng.uniform(-20, 20, 300), rng.uniform(-20, 20, 300), rng.uniform(0, 1, 300),
+                          rng.integers(0, 110, 300)]).astype(np.float32)
+    order, ids = sort_spots(sl)
+    tab, r80, energy = layer_table(ids, dvp)
+    _, _, lay_o, r80_o = O.spots_and_layers(sl, dvp)
+    np.testing.assert_array_equal(tab, lay_o)
+    np.testing.assert_array_equal(r80.astype(np.float64), r80_o)
+    # spatial order inside a layer only permutes spots of equal energy
+    assert np.all(np.diff(ids) >= 0)
+    sl[5, 3] = -2
+    order, ids = sort_spots(sl)
+    assert layer_table(ids, dvp)[0].shape == (0, 3)
