@@ -24,8 +24,15 @@
 
 namespace dh {
 
+// Spot record with the constants of its beam-axis geometry.  With
+// t = (tx, ty, -1) the spot axis direction, a head-frame point h sits at
+// in-plane offset e = (hx - x + hz tx, hy - y + hz ty) from where the axis
+// crosses the plane z = hz, and its squared distance to the axis is the
+// quadratic form  d2 = m11 e_x^2 + 2 m12 e_x e_y + m22 e_y^2  of
+// M = I - t_xy t_xy^T / |t|^2.  Unlike |r|^2 - (t.r)^2/|t|^2 this has no
+// cancellation far from the isocentre plane (|r| ~ 150 mm vs d ~ 1 mm).
 struct SpotExt {
-    float x, y, mu, tx, ty, itt, pad0, pad1;
+    float x, y, tx, ty, mu, m11, m12x2, m22;
 };
 
 struct Model {
@@ -51,8 +58,10 @@ __global__ __launch_bounds__(256) void k_spot_prep(const dh_spot *__restrict__ i
     e.mu = p.mu;
     e.tx = p.x * inv_vsadx;
     e.ty = p.y * inv_vsady;
-    e.itt = 1.0f / (e.tx * e.tx + e.ty * e.ty + 1.0f);
-    e.pad0 = e.pad1 = 0.0f;
+    const float itt = 1.0f / (e.tx * e.tx + e.ty * e.ty + 1.0f);
+    e.m11 = 1.0f - e.tx * e.tx * itt;
+    e.m22 = 1.0f - e.ty * e.ty * itt;
+    e.m12x2 = -2.0f * e.tx * e.ty * itt;
     out[s] = e;
 }
 
@@ -163,13 +172,11 @@ __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerK &L, f
 }
 
 // Squared distance from head-frame point (hx, hy, hz) to a spot axis.
-__device__ __forceinline__ float axis_d2(float hx, float hy, float hz2, float nhz, const SpotExt &q, float &tr,
-                                         float &dx, float &dy) {
-    dx = hx - q.x;
-    dy = hy - q.y;
-    const float rr = fmaf(dx, dx, fmaf(dy, dy, hz2));
-    tr = fmaf(q.tx, dx, fmaf(q.ty, dy, nhz));
-    return fmaf(-tr * q.itt, tr, rr);
+__device__ __forceinline__ float axis_d2(float hx, float hy, float hz, const SpotExt &q) {
+    const float u = fmaf(hz, q.tx, hx - q.x);
+    const float v = fmaf(hz, q.ty, hy - q.y);
+    const float w = fmaf(q.m12x2, v, q.m11 * u);
+    return fmaf(u, w, (q.m22 * v) * v);
 }
 
 __device__ __forceinline__ void brick_of(const BrickGrid &bg, long long brick, int &bi, int &bj, int &bk) {
@@ -207,7 +214,6 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
     const float3 h = image_to_head(b, v);
     const float sx = b.src[0] - v.x, sy = b.src[1] - v.y, sz = b.src[2] - v.z;
     const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
-    const float hz2 = h.z * h.z, nhz = -h.z;
 
     float3 hc = make_float3(0.f, 0.f, 0.f);
     if (CULL) {
@@ -240,8 +246,7 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
                 bool rel = false;
                 if (lane < nin) {
                     const SpotExt q = lsp[c0 + lane];
-                    float tr, dx, dy;
-                    rel = axis_d2(hc.x, hc.y, hc.z * hc.z, -hc.z, q, tr, dx, dy) < rc2;
+                    rel = axis_d2(hc.x, hc.y, hc.z, q) < rc2;
                 }
                 mask = __ballot(rel);
             } else {
@@ -251,8 +256,7 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
                 const int s = __builtin_ctzll(mask);
                 mask &= mask - 1ull;
                 const SpotExt q = lsp[c0 + s];
-                float tr, dx, dy;
-                const float d2 = axis_d2(h.x, h.y, hz2, nhz, q, tr, dx, dy);
+                const float d2 = axis_d2(h.x, h.y, h.z, q);
                 const float e1 = __builtin_amdgcn_exp2f(T.a1 * d2);
                 const float e2 = __builtin_amdgcn_exp2f(T.a2 * d2);
                 P = fmaf(q.mu, e1, P);
@@ -271,13 +275,15 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
 // terms, compacts the voxels that matter (in range, nonzero dL/dD) into LDS,
 // then every spot lane walks them with broadcast LDS reads.
 //   dL/dmu_s = sum_v g_v [A e1 + B e2]
-//   dL/dx_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADX) (dx - q tx)
-//   dL/dy_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADY) (dy - q ty)
-// with q = (t . r) / |t|^2 and a1, a2 = -1/(2 sigma^2) (natural-log scale).
+//   dL/dx_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADX) p_x
+//   dL/dy_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADY) p_y
+// with r = (hx - x_s, hy - y_s, hz) the voxel relative to the spot,
+// q = (t . r) / |t|^2, p = r - q t (its part perpendicular to the axis) and
+// a1, a2 = -1/(2 sigma^2) (natural-log scale).
 struct LdsVox {
-    float hx, hy, hz2, nhz;
-    float a1, a2, GA, GB;
-    float GAa, GBb, pad0, pad1;
+    float hx, hy, hz, a1;
+    float a2, GA, GB, GAa;
+    float GBb, pad0, pad1, pad2;
 };
 
 template <bool CULL>
@@ -308,7 +314,8 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
     const int spot = L.start + c0 + lane;
     SpotExt q;
     if (sval) q = sp[spot];
-    else q = SpotExt{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f};
+    else q = SpotExt{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f};
+    const float itt = 1.0f / (q.tx * q.tx + q.ty * q.ty + 1.0f);
     const float ivx = 1.0f / b.vsadx, ivy = 1.0f / b.vsady;
 
     const long long nb = bg.count();
@@ -349,8 +356,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
             const float3 hc = image_to_head(b, make_float3(fmaf((float)bi * BRICK + half, g.s, -b.iso[0]),
                                                            fmaf((float)bj * BRICK + half, g.s, -b.iso[1]),
                                                            fmaf((float)bk * BRICK + half, g.s, -b.iso[2])));
-            float tr, dx, dy;
-            rel = sval && axis_d2(hc.x, hc.y, hc.z * hc.z, -hc.z, q, tr, dx, dy) < rc * rc;
+            rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
             if (!__ballot(rel)) continue;
         }
         if (act) {
@@ -358,8 +364,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
             LdsVox e;
             e.hx = h.x;
             e.hy = h.y;
-            e.hz2 = h.z * h.z;
-            e.nhz = -h.z;
+            e.hz = h.z;
             e.a1 = T.a1;
             e.a2 = T.a2;
             e.GA = gv * T.Af;
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
             // derivative scales in natural-log units: a_nat = a_exp2 / log2(e)
             e.GAa = e.GA * (T.a1 * (1.0f / kLog2e));
             e.GBb = e.GB * (T.a2 * (1.0f / kLog2e));
-            e.pad0 = e.pad1 = 0.0f;
+            e.pad0 = e.pad1 = e.pad2 = 0.0f;
             s_vox[pos] = e;
         }
         __syncthreads();
@@ -375,17 +380,22 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
         if (rel) {
             for (int t = 0; t < na; ++t) {
                 const LdsVox e = s_vox[t];
-                float tr, dx, dy;
-                const float d2 = axis_d2(e.hx, e.hy, e.hz2, e.nhz, q, tr, dx, dy);
+                // in-plane offset e_xy, its part perpendicular to the axis p,
+                // q_e = (e.t)/|t|^2; d2 = e_xy . p (no cancellation)
+                const float u = fmaf(e.hz, q.tx, e.hx - q.x);
+                const float v = fmaf(e.hz, q.ty, e.hy - q.y);
+                const float qe = fmaf(u, q.tx, v * q.ty) * itt;
+                const float px = fmaf(-qe, q.tx, u), py = fmaf(-qe, q.ty, v);
+                const float d2 = fmaf(u, px, v * py);
                 const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
                 const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
                 K = fmaf(e.GA, e1, K);
                 K = fmaf(e.GB, e2, K);
                 const float F = fmaf(e.GAa, e1, e.GBb * e2);
-                const float qq = tr * q.itt;
-                const float px = fmaf(-qq, q.tx, dx), py = fmaf(-qq, q.ty, dy);
-                FX = fmaf(F * fmaf(qq, ivx, 1.0f), px, FX);
-                FY = fmaf(F * fmaf(qq, ivy, 1.0f), py, FY);
+                // q of the ray r = e - hz t is qe - hz
+                const float qr = qe - e.hz;
+                FX = fmaf(F * fmaf(qr, ivx, 1.0f), px, FX);
+                FY = fmaf(F * fmaf(qr, ivy, 1.0f), py, FY);
             }
         }
         __syncthreads();

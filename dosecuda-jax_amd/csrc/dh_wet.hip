@@ -154,15 +154,35 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(const float *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // Lateral WET smoothing (smoothRayKernel, IMPTClasses.cu:139-202;
-// _smooth_wet_kernel, impt_jax.py:317-418).  Directions k*60 deg use exact
-// table values; the neighbour index is round-half-to-even of (x + iso) / s
-// (true division, so exact half-voxel ties stay exact).
-__constant__ float c_dir_cos[6] = {1.0f, 0.5f, -0.5f, -1.0f, -0.5f, 0.5f};
-__constant__ float c_dir_sin[6] = {0.0f, 0.86602540378443865f, 0.86602540378443865f,
-                                   0.0f, -0.86602540378443865f, -0.86602540378443865f};
+// _smooth_wet_kernel, impt_jax.py:317-418).
+//
+// A neighbour of voxel (i, j, k) is the voxel nearest to the voxel centre
+// moved by dr mm along beam's-eye-view direction a*60 deg.  The move is the
+// same for every voxel: in voxel units it is D = R^T (dr cos, dr sin, 0) / s
+// (R = image -> head rotation, orthogonal), so the neighbour is
+// round_half_even((i, j, k) + D) per axis.  The host tabulates D for the 54
+// (direction, dr) pairs in float64 as an integer base offset plus a tie flag
+// for components whose fractional part is a half (within 1e-6): those round to
+// the even index, which depends on the voxel's parity.  This is the
+// real-number value of the reference's rotate / round sequence; float32 (or
+// float64) evaluation of the rotations would break exact half-voxel ties at
+// random, and such ties are common (gantry angles that are multiples of 30 deg
+// put the 60-deg directions on rational offsets).
+constexpr int N_STENCIL = 54;
+
+struct Stencil {
+    // per entry: (dx, dy, dz) integer base offsets, tie bits (1 = x, 2 = y, 4 = z), dr
+    signed char dx[N_STENCIL], dy[N_STENCIL], dz[N_STENCIL];
+    unsigned char tie[N_STENCIL], dr[N_STENCIL];
+};
+
+__device__ __forceinline__ int stencil_axis(int i, int base, bool tie) {
+    const int n = i + base;
+    return tie ? n + (n & 1) : n;
+}
 
 __global__ __launch_bounds__(256) void k_wet_smooth(const float *__restrict__ raw, float *__restrict__ out, Grid g,
-                                                   Geom b, long long nvox) {
+                                                   Stencil st, long long nvox) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nvox) return;
     const int i = (int)(t % g.nx);
@@ -171,18 +191,17 @@ __global__ __launch_bounds__(256) void k_wet_smooth(const float *__restrict__ ra
 
     const float wc = raw[t];
     const float max_dr = fminf(wc * 10.0f, 10.0f);
-    const float3 h = image_to_head(b, voxel_xyz(g, b, i, j, k));
     float sum = wc;
     int cnt = 1;
+    // entries are ordered direction-major, dr = 1..9 within a direction
     for (int a = 0; a < 6; ++a) {
-        const float ca = c_dir_cos[a], sa = c_dir_sin[a];
-        for (int d = 1; d <= 9; ++d) {
-            const float dr = (float)d;
-            if (!(dr < max_dr)) break;
-            const float3 ip = head_to_image(b, make_float3(h.x + dr * ca, h.y + dr * sa, h.z));
-            const int ni = (int)rintf((ip.x + b.iso[0]) / g.s);
-            const int nj = (int)rintf((ip.y + b.iso[1]) / g.s);
-            const int nk = (int)rintf((ip.z + b.iso[2]) / g.s);
+        for (int d = 0; d < 9; ++d) {
+            const int e = a * 9 + d;
+            if (!((float)st.dr[e] < max_dr)) break;
+            const unsigned tb = st.tie[e];
+            const int ni = stencil_axis(i, st.dx[e], tb & 1u);
+            const int nj = stencil_axis(j, st.dy[e], tb & 2u);
+            const int nk = stencil_axis(k, st.dz[e], tb & 4u);
             if ((unsigned)ni < (unsigned)g.nx && (unsigned)nj < (unsigned)g.ny && (unsigned)nk < (unsigned)g.nz) {
                 sum += raw[ni + (long long)g.nx * (nj + (long long)g.ny * nk)];
                 ++cnt;
@@ -190,6 +209,41 @@ __global__ __launch_bounds__(256) void k_wet_smooth(const float *__restrict__ ra
         }
     }
     out[t] = sum / (float)cnt;
+}
+
+// Host: tabulate the smoothing stencil of a beam.
+static void make_stencil(const dh_beam_geom &b, double s, Stencil &st) {
+    static const double kc[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+    static const double ks[6] = {0.0, 0.86602540378443864676, 0.86602540378443864676,
+                                 0.0, -0.86602540378443864676, -0.86602540378443864676};
+    const double sg = b.sin_g, cg = b.cos_g, sc = b.sin_c, cc = b.cos_c;
+    for (int a = 0; a < 6; ++a) {
+        for (int d = 1; d <= 9; ++d) {
+            const int e = a * 9 + (d - 1);
+            // head offset (hx, hy, 0) -> image offset, as head_to_image without translation
+            const double hx = d * kc[a], hy = d * ks[a];
+            const double xz = -hx, yz = 0.0, zz = hy;
+            const double xg = xz * cg - yz * sg, yg = xz * sg + yz * cg;
+            const double off[3] = {xg * cc + zz * sc, yg, -xg * sc + zz * cc};
+            int base[3];
+            unsigned tie = 0;
+            for (int ax = 0; ax < 3; ++ax) {
+                const double q = off[ax] / s;
+                const double fl = floor(q), fr = q - fl;
+                if (fabs(fr - 0.5) < 1e-6) {
+                    base[ax] = (int)fl;
+                    tie |= 1u << ax;
+                } else {
+                    base[ax] = (int)fl + (fr > 0.5 ? 1 : 0);
+                }
+            }
+            st.dx[e] = (signed char)base[0];
+            st.dy[e] = (signed char)base[1];
+            st.dz[e] = (signed char)base[2];
+            st.tie[e] = (unsigned char)tie;
+            st.dr[e] = (unsigned char)d;
+        }
+    }
 }
 
 static bool grid_ok(const dh_grid &g) {
@@ -267,9 +321,16 @@ extern "C" int dh_wet_smooth(const float *wet_raw, float *wet, dh_grid g, const 
         set_error("dh_wet_smooth: null or aliased pointers");
         return DH_ERR_ARG;
     }
+    if (g.spacing < 0.2f) {
+        // |offset| <= 9 mm must fit the signed-char stencil (45 voxels)
+        set_error("dh_wet_smooth: spacing %g mm below 0.2 mm", (double)g.spacing);
+        return DH_ERR_ARG;
+    }
+    Stencil st;
+    make_stencil(*geom, (double)g.spacing, st);
     const long long nvox = (long long)g.nx * g.ny * g.nz;
     const unsigned nb = (unsigned)((nvox + 255) / 256);
-    hipLaunchKernelGGL(k_wet_smooth, dim3(nb), dim3(256), 0, (hipStream_t)stream, wet_raw, wet, make_grid(g),
-                       make_geom(*geom), nvox);
+    hipLaunchKernelGGL(k_wet_smooth, dim3(nb), dim3(256), 0, (hipStream_t)stream, wet_raw, wet, make_grid(g), st,
+                       nvox);
     return check_launch("k_wet_smooth");
 }

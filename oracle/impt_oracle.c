@@ -20,7 +20,8 @@
  * Decisions where the JAX code leaves float-level behaviour open:
  *  - trilinear sampling with zero padding per corner (scipy 'grid-constant'),
  *    samples whose continuous index leaves [0, n) on any axis dropped;
- *  - neighbour rounding half to even (jnp.round), exact 60-degree table;
+ *  - neighbour rounding half to even (jnp.round) on the real-number
+ *    coordinate (half-voxel ties detected to 1e-6), exact 60-degree table;
  *  - a Gaussian whose sigma is not positive contributes 0.
  *
  * Compile with -DORC_FLOAT for a float32 build (benchmark baseline only).
@@ -168,6 +169,18 @@ static const double k_cos6[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
 static const double k_sin6[6] = {0.0, 0.86602540378443864676, 0.86602540378443864676,
                                  0.0, -0.86602540378443864676, -0.86602540378443864676};
 
+/* Nearest integer, ties to even, on the real-number coordinate: a value within
+ * 1e-6 of a half-integer is an exact tie of the exact arithmetic (offsets built
+ * from 60-degree directions and 30-degree-multiple gantry angles are rational),
+ * which float evaluation would otherwise break by rounding noise. */
+static long round_half_even_exact(real c) {
+    real fl = FLOOR(c);
+    real fr = c - fl;
+    long f = (long)fl;
+    if (fabs((double)fr - 0.5) < 1e-6) return (f & 1) ? f + 1 : f;
+    return (long)RINT(c);
+}
+
 /* neighbour list of one voxel for smoothing; returns count (<= 54) */
 static int smooth_neighbours(const beam_t *b, int nx, int ny, int nz, real s, int i, int j, int k, real wc,
                              int64_t *nb) {
@@ -184,9 +197,9 @@ static int smooth_neighbours(const beam_t *b, int nx, int ny, int nz, real s, in
             real hp[3] = {h[0] + dr * R(k_cos6[a]), h[1] + dr * R(k_sin6[a]), h[2]};
             real ip[3];
             head_to_image(b, hp, ip);
-            long ni = (long)RINT((ip[0] + b->iso[0]) / s);
-            long nj = (long)RINT((ip[1] + b->iso[1]) / s);
-            long nk = (long)RINT((ip[2] + b->iso[2]) / s);
+            long ni = round_half_even_exact((ip[0] + b->iso[0]) / s);
+            long nj = round_half_even_exact((ip[1] + b->iso[1]) / s);
+            long nk = round_half_even_exact((ip[2] + b->iso[2]) / s);
             if (ni >= 0 && ni < nx && nj >= 0 && nj < ny && nk >= 0 && nk < nz)
                 nb[cnt++] = vidx(nx, ny, (int)ni, (int)nj, (int)nk);
         }

@@ -1,9 +1,8 @@
 """HIP kernels vs the FP64 oracle (GPU).
 
 Tolerances (float32 kernels vs float64 oracle):
-  * WET (cm): max |dWET| <= 2e-4 cm over the volume, except for voxels whose
-    float32 ray lands within rounding of a grid plane (counted, must be a
-    vanishing fraction).
+  * WET (cm): max |dWET| <= 2e-4 cm (0.002 mm) over the volume, raw and
+    smoothed.
   * dose: max |dD| / D_max <= 1e-3 end to end (north-star bar 5e-3), and
     <= 2e-5 for the pencil-beam kernel alone on identical WET input; 2%/2 mm
     gamma pass rate 1.0.
@@ -76,9 +75,7 @@ def test_wet_small_phantom(dev, gantry, couch):
     d_raw = np.abs(raw - raw_o)
     d_wet = np.abs(wet - wet_o)
     _say("max |d raw| %.3e cm, max |d wet| %.3e cm" % (d_raw.max(), d_wet.max()))
-    assert np.mean(d_raw > 2e-4) < 1e-3, d_raw.max()
-    assert np.mean(d_wet > 2e-4) < 2e-3, d_wet.max()
-    assert np.abs(wet - wet_o).mean() < 1e-4
+    assert d_raw.max() <= 2e-4 and d_wet.max() <= 2e-4
 
 
 def test_rlsp_from_hu_matches_np_interp(dev):

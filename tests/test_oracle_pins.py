@@ -150,6 +150,15 @@ def test_raymarch_bruteforce(gantry, couch):
 
 # ---------------------------------------------------------------------------
 # smoothing
+def _round_tie_even(c):
+    """Nearest integer; half-integers (to 1e-6: exact ties of the real-number
+    coordinate) go to the even neighbour, like Python's round() on exact ties."""
+    f = math.floor(c)
+    if abs(c - f - 0.5) < 1e-6:
+        return round(f + 0.5)
+    return round(c)
+
+
 def _smooth_bruteforce(raw, s, g):
     nz, ny, nx = raw.shape
     out = np.empty_like(raw)
@@ -173,9 +182,8 @@ def _smooth_bruteforce(raw, s, g):
                         xz, yz, zz = -hx, hz, hy
                         xg, yg = xz * cg - yz * sg, xz * sg + yz * cg
                         x, y, z = xg * cc + zz * sc, yg, -xg * sc + zz * cc
-                        ii = round((x + iso[0]) / s)   # Python round: half to even
-                        jj = round((y + iso[1]) / s)
-                        kk = round((z + iso[2]) / s)
+                        ii, jj, kk = (_round_tie_even(c) for c in ((x + iso[0]) / s, (y + iso[1]) / s,
+                                                                   (z + iso[2]) / s))
                         if 0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nz:
                             tot += raw[kk, jj, ii]
                             cnt += 1
@@ -183,13 +191,30 @@ def _smooth_bruteforce(raw, s, g):
     return out
 
 
-@pytest.mark.parametrize("gantry,couch", [(0.0, 0.0), (120.0, 30.0)])
+@pytest.mark.parametrize("gantry,couch", [(0.0, 0.0), (120.0, 30.0), (120.0, 0.0)])
 def test_smooth_bruteforce(gantry, couch):
     rng = np.random.default_rng(9)
     raw = rng.uniform(-0.05, 2.0, size=(5, 6, 7))
     s = 2.0
     g = O.beam_geometry(np.zeros(3), np.array([-6.0, -5.0, -4.0]), gantry, couch, 1340.0, 1930.0)
     np.testing.assert_allclose(O.smooth(raw, s, g), _smooth_bruteforce(raw, s, g), rtol=0, atol=1e-13)
+
+
+def test_smooth_half_voxel_ties_go_even():
+    """Gantry 120 (BEV x at 300 deg), 2.5 mm voxels: direction 60 deg at
+    dr = 5 mm moves exactly half a voxel along image x; the neighbour must be
+    the even index whatever the float rounding of the rotation."""
+    s = 2.5
+    raw = np.zeros((3, 3, 8))
+    raw[1, 1, :] = np.arange(8) * 1.0 + 100.0     # distinct values along x
+    g = O.beam_geometry(np.zeros(3), np.array([-3.75 * 2, -2.5, -2.5]), 120.0, 0.0, 1340.0, 1930.0)
+    # direct check of the exact image offset of (a=1, dr=5): x = -2.5*... voxel units
+    hx, hy = 5 * 0.5, 5 * math.sqrt(3) / 2
+    sg, cg, sc, cc = g[6:10]
+    off_x = (-hx * cg) * cc + hy * sc
+    assert abs(off_x / s - round(off_x / s * 2) / 2) < 1e-12 and abs(abs(off_x / s) % 1 - 0.5) < 1e-12
+    ref = _smooth_bruteforce(raw, s, g)
+    np.testing.assert_allclose(O.smooth(raw, s, g), ref, rtol=0, atol=1e-13)
 
 
 def test_smooth_invariants():
