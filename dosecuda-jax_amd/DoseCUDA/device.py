@@ -173,5 +173,21 @@ def pb_adjoint(wet, grad_dose, grid, geom, dmodel, dspots, workspace, ws_bytes,
     return gmu, gxy
 
 
+def pb_count_pairs(wet, grad_dose, grid, geom, dmodel, dspots, workspace, ws_bytes,
+                   cutoff_sigma=DEFAULT_CUTOFF_SIGMA, counts=None):
+    """Accumulate the work counts of dh_pb_count_pairs into a device int64 [4]
+    tensor (forward dense, forward evaluated, adjoint dense, adjoint evaluated)."""
+    lib = N.load()
+    dev = wet.device
+    if counts is None:
+        counts = torch.zeros(4, dtype=torch.int64, device=dev)
+    rc = lib.dh_pb_count_pairs(N.ptr(wet), N.ptr(grad_dose) if grad_dose is not None else None, grid,
+                               ctypes.byref(geom), ctypes.byref(dmodel.c), ctypes.byref(dspots.c),
+                               float(cutoff_sigma), N.ptr(counts), N.ptr(workspace), int(ws_bytes),
+                               N.stream_handle(dev))
+    N.check(rc, "dh_pb_count_pairs")
+    return counts
+
+
 def geometry_for(beam, origin, model):
     return beam_geometry(beam.iso, origin, beam.gantry_angle, beam.couch_angle, model.VSADX, model.VSADY)

@@ -193,12 +193,15 @@ __device__ __forceinline__ void brick_of(const BrickGrid &bg, long long brick, i
 // lane-parallel against the brick (distance from the brick centre to the spot
 // axis vs cutoff * max halo sigma + brick radius) and only the surviving spots
 // are iterated, wave-uniformly (their records come in through scalar loads).
-template <bool CULL>
+// COUNT: no dose; counts[0] += in-range voxels x layer spots (the reference's
+// work), counts[1] += in-range voxels x spots surviving the cull (work done).
+template <bool CULL, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ wetv, float *__restrict__ dose, Grid g,
                                                    Geom b, BrickGrid bg, unsigned nblocks, Model m,
                                                    const dh_layer *__restrict__ layers, int n_layers,
                                                    const SpotExt *__restrict__ sp, int n_spots, float cut,
-                                                   float rbrick, int accumulate) {
+                                                   float rbrick, int accumulate,
+                                                   unsigned long long *__restrict__ counts = nullptr) {
     const unsigned blk = xcd_remap(blockIdx.x, nblocks);
     const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
     if (brick >= bg.count()) return;  // whole wave leaves together
@@ -225,11 +228,14 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
     }
 
     float acc = 0.0f;
+    unsigned long long n_dense = 0, n_exec = 0;
     for (int l = 0; l < n_layers; ++l) {
         const LayerK L = load_layer(layers, m, l);
         if (!layer_ok(L, m, n_spots)) continue;
         const bool act = wet <= L.lim;
-        if (!__ballot(act)) continue;
+        const unsigned long long amask = __ballot(act);
+        if (!amask) continue;
+        if (COUNT) n_dense += (unsigned long long)__popcll(amask) * (unsigned)L.n;
         const VoxTerms T = vox_terms(m, L, wet, dsrc);
         float rc2 = 0.0f;
         if (CULL) {
@@ -252,6 +258,10 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
             } else {
                 mask = nin == WAVE ? ~0ull : ((1ull << nin) - 1ull);
             }
+            if (COUNT) {
+                n_exec += (unsigned long long)__popcll(amask) * (unsigned)__popcll(mask);
+                continue;
+            }
             while (mask) {
                 const int s = __builtin_ctzll(mask);
                 mask &= mask - 1ull;
@@ -264,6 +274,13 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
             }
         }
         if (act) acc = fmaf(T.Af, P, fmaf(T.Bf, H, acc));
+    }
+    if (COUNT) {
+        if (lane == 0) {
+            atomicAdd(&counts[0], n_dense);
+            atomicAdd(&counts[1], n_exec);
+        }
+        return;
     }
     if (valid) dose[idx] = accumulate ? dose[idx] + acc : acc;
 }
@@ -286,13 +303,16 @@ struct LdsVox {
     float GBb, pad0, pad1, pad2;
 };
 
-template <bool CULL>
+// COUNT: no gradient; counts[2] += voxels (dL/dD != 0, in range) x chunk
+// spots (the reference's work), counts[3] += those voxels x surviving spots.
+template <bool CULL, bool COUNT = false>
 __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wetv, const float *__restrict__ gdose,
                                                   Grid g, Geom b, BrickGrid bg, Model m,
                                                   const dh_layer *__restrict__ layers, int n_layers,
                                                   const SpotExt *__restrict__ sp, int n_spots, float cut,
                                                   float rbrick, int nslab, const unsigned char *__restrict__ flags,
-                                                  float4 *__restrict__ partial) {
+                                                  float4 *__restrict__ partial,
+                                                  unsigned long long *__restrict__ counts = nullptr) {
     __shared__ LdsVox s_vox[WAVE];
     const int lane = threadIdx.x;
     // chunk -> (layer, first spot of the chunk within the layer)
@@ -322,6 +342,8 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
     const long long b0 = nb * slab / nslab, b1 = nb * (slab + 1) / nslab;
     float K = 0.0f, FX = 0.0f, FY = 0.0f;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const unsigned long long chunk_n = (unsigned long long)min(WAVE, L.n - c0);
+    unsigned long long n_dense = 0, n_exec = 0;
 
     // bricks flagged by k_brick_flags (any voxel with dL/dD != 0 in range),
     // 64 flags per load, visited in ascending order
@@ -341,6 +363,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
         const bool act = valid && gv != 0.0f && wet <= L.lim;
         const unsigned long long am = __ballot(act);
         if (!am) continue;
+        if (COUNT) n_dense += (unsigned long long)__popcll(am) * chunk_n;
 
         const float3 v = voxel_xyz(g, b, i, j, k);
         const float3 h = image_to_head(b, v);
@@ -358,6 +381,10 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
                                                            fmaf((float)bk * BRICK + half, g.s, -b.iso[2])));
             rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
             if (!__ballot(rel)) continue;
+        }
+        if (COUNT) {
+            n_exec += (unsigned long long)__popcll(am) * (unsigned)__popcll(__ballot(rel));
+            continue;
         }
         if (act) {
             const int pos = __popcll(am & lt_mask);
@@ -400,6 +427,13 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
         }
         __syncthreads();
       }
+    }
+    if (COUNT) {
+        if (lane == 0) {
+            atomicAdd(&counts[2], n_dense);
+            atomicAdd(&counts[3], n_exec);
+        }
+        return;
     }
     if (sval) partial[(long long)slab * n_spots + spot] = make_float4(K, FX, FY, 0.0f);
 }
@@ -595,4 +629,58 @@ extern "C" int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g
     hipLaunchKernelGGL(k_pb_reduce, dim3((S + 255) / 256), dim3(256), 0, st, partial, ext, S, nslab, grad_mu,
                        grad_xy);
     return check_launch("k_pb_reduce");
+}
+
+extern "C" int dh_pb_count_pairs(const float *wet, const float *grad_dose, dh_grid g, const dh_beam_geom *geom,
+                                 const dh_beam_model *model, const dh_spot_set *spots, float cutoff_sigma,
+                                 unsigned long long *counts, void *workspace, int64_t workspace_bytes,
+                                 void *stream) {
+    if (!common_args(g, geom, spots, workspace, workspace_bytes)) return DH_ERR_ARG;
+    if (!wet || !counts) {
+        set_error("dh_pb_count_pairs: null pointer");
+        return DH_ERR_ARG;
+    }
+    const int S = spots->n_spots;
+    if (S == 0) return DH_OK;  // counts are accumulated into; nothing to add
+    Model m;
+    if (!fill_model(model, m)) return DH_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const Grid gg = make_grid(g);
+    const Geom gb = make_geom(*geom);
+    SpotExt *ext = reinterpret_cast<SpotExt *>(workspace);
+    hipLaunchKernelGGL(k_spot_prep, dim3((S + 255) / 256), dim3(256), 0, st, spots->spots, ext, S,
+                       1.0f / geom->vsadx, 1.0f / geom->vsady);
+    int rc = check_launch("k_spot_prep");
+    if (rc) return rc;
+    const BrickGrid bg = brick_grid(gg);
+    const unsigned nb = (unsigned)((bg.count() + 3) / 4);
+    const float rbrick = 0.5f * (BRICK - 1) * 1.7320508075688772f * g.spacing * 1.0001f;
+    const bool cull = cutoff_sigma > 0.0f;
+    if (cull)
+        hipLaunchKernelGGL((k_pb_forward<true, true>), dim3(nb), dim3(256), 0, st, wet, nullptr, gg, gb, bg, nb, m,
+                           spots->layers, spots->n_layers, ext, S, cutoff_sigma, rbrick, 0, counts);
+    else
+        hipLaunchKernelGGL((k_pb_forward<false, true>), dim3(nb), dim3(256), 0, st, wet, nullptr, gg, gb, bg, nb, m,
+                           spots->layers, spots->n_layers, ext, S, 0.0f, rbrick, 0, counts);
+    rc = check_launch("k_pb_forward(count)");
+    if (rc || !grad_dose) return rc;
+    const int nslab = adjoint_slabs(g, S, spots->n_layers);
+    float4 *partial = reinterpret_cast<float4 *>(reinterpret_cast<char *>(workspace) +
+                                                 align_up((long long)S * (long long)sizeof(SpotExt)));
+    const long long np = (long long)nslab * S;
+    unsigned char *flags = reinterpret_cast<unsigned char *>(partial) + align_up(np * 16LL);
+    hipLaunchKernelGGL(k_brick_flags, dim3((unsigned)((bg.count() + 3) / 4)), dim3(256), 0, st, wet, grad_dose, gg,
+                       bg, spots->layers, spots->n_layers, flags);
+    rc = check_launch("k_brick_flags");
+    if (rc) return rc;
+    const unsigned nch = (unsigned)(((long long)S + 63LL * spots->n_layers) / 64 + 1);
+    if (cull)
+        hipLaunchKernelGGL((k_pb_adjoint<true, true>), dim3(nch, nslab), dim3(64), 0, st, wet, grad_dose, gg, gb, bg,
+                           m, spots->layers, spots->n_layers, ext, S, cutoff_sigma, rbrick, nslab, flags, partial,
+                           counts);
+    else
+        hipLaunchKernelGGL((k_pb_adjoint<false, true>), dim3(nch, nslab), dim3(64), 0, st, wet, grad_dose, gg, gb,
+                           bg, m, spots->layers, spots->n_layers, ext, S, 0.0f, rbrick, nslab, flags, partial,
+                           counts);
+    return check_launch("k_pb_adjoint(count)");
 }

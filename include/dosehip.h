@@ -161,6 +161,20 @@ int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g,
                   float *grad_mu, float *grad_xy,
                   void *workspace, int64_t workspace_bytes, void *stream);
 
+/* Work accounting for dh_pb_forward / dh_pb_adjoint with the same arguments
+ * (no dose or gradient is computed).  Adds to the DEVICE array counts[4]:
+ *   counts[0]  forward pairs the reference evaluates: voxels with
+ *              wet <= 1.1 R80 of a layer x that layer's spots
+ *   counts[1]  forward pairs evaluated here after the cutoff_sigma cull
+ *   counts[2]  adjoint pairs of the same kind over voxels with grad_dose != 0
+ *   counts[3]  adjoint pairs evaluated here after the cull
+ * grad_dose may be NULL (forward counts only). */
+int dh_pb_count_pairs(const float *wet, const float *grad_dose, dh_grid g,
+                      const dh_beam_geom *geom, const dh_beam_model *model,
+                      const dh_spot_set *spots, float cutoff_sigma,
+                      unsigned long long *counts,
+                      void *workspace, int64_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

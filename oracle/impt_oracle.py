@@ -197,7 +197,7 @@ def pb_dose(wet_cm, spacing, geom, spots, layers, r80, marr, idx=None, precision
     return out.reshape(w.shape) if ix is None else out
 
 
-def pb_grad(wet_cm, grad_dose, spacing, geom, spots, layers, r80, marr):
+def pb_grad(wet_cm, grad_dose, spacing, geom, spots, layers, r80, marr, precision="f64"):
     """(dL/dmu [S], dL/dxy [S, 2]) in sorted spot order."""
     w = np.ascontiguousarray(wet_cm, dtype=np.float64)
     gv = np.ascontiguousarray(grad_dose, dtype=np.float64)
@@ -209,7 +209,7 @@ def pb_grad(wet_cm, grad_dose, spacing, geom, spots, layers, r80, marr):
     S = sp.shape[0]
     gmu = np.zeros(S)
     gxy = np.zeros((S, 2))
-    lib().orc_pb_grad(_p(w), _p(gv), nx, ny, nz, float(spacing), _p(g), _p(sp), S, _p(la), _p(r8), la.shape[0],
+    lib(precision).orc_pb_grad(_p(w), _p(gv), nx, ny, nz, float(spacing), _p(g), _p(sp), S, _p(la), _p(r8), la.shape[0],
                       _p(marr.dvp), marr.dvp.shape[1], _p(marr.depths), _p(marr.sigmas), _p(marr.idds),
                       marr.depths.shape[1], _p(gmu), _p(gxy))
     return gmu, gxy
