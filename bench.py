@@ -117,7 +117,19 @@ def main():
     ap.add_argument("--cutoff-sigma", type=float, default=6.0)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the dense (cutoff 0) path; 0 = skip")
     ap.add_argument("--cpu-sample", type=int, default=60000, help="voxels in the CPU-baseline sample; 0 = skip")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE "
+                         "counters against a copy of known size) and exit")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        import torch
+        x = torch.ones(args.copy_probe_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the dose of one plan is computed "

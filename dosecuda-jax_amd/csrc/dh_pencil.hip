@@ -1,10 +1,12 @@
 // Double-Gaussian pencil-beam spot x voxel kernels (gfx950, wave64).
 //
-//  k_spot_prep     spot records + per-spot constants of the beam-axis geometry
-//  k_pb_forward    dose: lanes = voxels of a 4x4x4 brick, spots wave-uniform
-//  k_pb_adjoint    dL/dmu, dL/dx, dL/dy partials: lanes = spots, voxel terms
-//                  broadcast from LDS
-//  k_pb_reduce     fixed-order sum of the adjoint partials
+//  k_spot_prep      spot records + per-spot constants of the beam-axis geometry
+//  k_layer_prep     per-layer record: constants, reach of its spots
+//  k_pb_forward     dose: lanes = voxels of a 4x4x4 brick, spots wave-uniform
+//  k_brick_summary  per-brick WET range of the voxels with dL/dD != 0
+//  k_pb_adjoint     dL/dmu, dL/dx, dL/dy partials: lanes = spots, voxel terms
+//                   broadcast from LDS
+//  k_pb_reduce      fixed-order sum of the adjoint partials
 //
 // Per (voxel, layer) terms follow pencilBeamKernel (IMPTClasses.cu:204-271) and
 // _pencil_beam_single_layer (impt_jax.py:425-541):
@@ -20,6 +22,12 @@
 // axis, through (x_s, y_s, 0) along t = (x_s/VSADX, y_s/VSADY, -1)
 // (caxDistance, IMPTClasses.cu:123-133).  A Gaussian whose sigma is not
 // positive contributes nothing (the -inf exponent scale of impt_jax.py:491-492).
+//
+// Skipping work (cutoff_sigma > 0): a spot whose axis stays further than
+// cutoff_sigma halo sigmas from every voxel of a brick contributes less than
+// exp(-cutoff^2/2) of its peak there and is not evaluated.  Tests run
+// lane-parallel: the 64 lanes of a wave judge 64 candidates (layers, spots or
+// bricks) at once and the wave then walks only the survivors.
 #include "dh_common.h"
 
 namespace dh {
@@ -43,8 +51,30 @@ struct Model {
     float depth0, inv_ddepth;
 };
 
+// Layer record (64 bytes): the layer's spot range and energy, the constants of
+// its sigma_air polynomial, and the reach of its spots: bounding box of the
+// positions at the isocentre plane, the smallest eigenvalue 1/(1 + |t_xy|^2)
+// of their distance forms and max |sigma_ms| of the energy's lookup row.
+struct LayerC {
+    int start, n, eid, ok;
+    float r80, A, B, C;
+    float xmin, xmax, ymin, ymax;
+    float lam_min, sms_max, lim, pad;
+};
+
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kTwoPi = 6.283185307179586f;
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// natural log through v_log_f32 (log2)
+__device__ __forceinline__ float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_spot_prep(const dh_spot *__restrict__ in, SpotExt *__restrict__ out, int n,
@@ -65,6 +95,60 @@ __global__ __launch_bounds__(256) void k_spot_prep(const dh_spot *__restrict__ i
     out[s] = e;
 }
 
+// Reach of spots [start, start + n) (lane-parallel) into c.
+__device__ __forceinline__ void reach_of(const SpotExt *__restrict__ sp, int start, int n, const Model &m, int eid,
+                                         int lane, LayerC &c) {
+    float xmin = 3.0e38f, xmax = -3.0e38f, ymin = 3.0e38f, ymax = -3.0e38f, t2max = 0.0f, smax = 0.0f;
+    for (int s = lane; s < n; s += WAVE) {
+        const SpotExt q = sp[start + s];
+        xmin = fminf(xmin, q.x);
+        xmax = fmaxf(xmax, q.x);
+        ymin = fminf(ymin, q.y);
+        ymax = fmaxf(ymax, q.y);
+        t2max = fmaxf(t2max, q.tx * q.tx + q.ty * q.ty);
+    }
+    const float2 *row = m.idd_sigma + (long long)eid * m.lut_len;
+    for (int t = lane; t < m.lut_len; t += WAVE) smax = fmaxf(smax, fabsf(row[t].y));
+    c.xmin = wave_min(xmin);
+    c.xmax = wave_max(xmax);
+    c.ymin = wave_min(ymin);
+    c.ymax = wave_max(ymax);
+    c.lam_min = 1.0f / (1.0f + wave_max(t2max));
+    c.sms_max = wave_max(smax);
+}
+
+// One wave per layer.  A record the kernels cannot use safely (energy id
+// outside the tables, spot range outside the spot array) gets ok = 0 and is
+// skipped rather than read out of bounds.
+__global__ __launch_bounds__(64) void k_layer_prep(const SpotExt *__restrict__ sp, int n_spots,
+                                                  const dh_layer *__restrict__ layers, Model m,
+                                                  LayerC *__restrict__ out) {
+    const int l = blockIdx.x, lane = threadIdx.x;
+    const dh_layer L = layers[l];
+    LayerC c;
+    c.ok = (unsigned)L.energy_id < (unsigned)m.n_energies && L.spot_start >= 0 && L.n_spots > 0 &&
+           L.spot_start <= n_spots - L.n_spots;
+    c.start = c.ok ? L.spot_start : 0;
+    c.n = c.ok ? L.n_spots : 0;
+    c.eid = c.ok ? L.energy_id : 0;
+    c.r80 = L.r80;
+    c.lim = 1.1f * L.r80;
+    const float *d = m.dvp + (long long)c.eid * m.dvp_len;
+    c.A = d[2];
+    c.B = d[3];
+    c.C = d[4];
+    c.pad = 0.0f;
+    if (c.ok) {
+        reach_of(sp, c.start, c.n, m, c.eid, lane, c);
+    } else {
+        c.xmin = c.ymin = -3.0e38f;
+        c.xmax = c.ymax = 3.0e38f;
+        c.lam_min = 0.0f;
+        c.sms_max = 0.0f;
+    }
+    if (lane == 0) out[l] = c;
+}
+
 // LUT lookup (interpolateProtonLUT, IMPTClasses.cu:69-92): i = first depth >= wet;
 // i == 0 -> row 0, i == len -> last row, else lerp between i-1 and i.
 __device__ __forceinline__ void lut_lookup(const Model &m, int eid, float wet, float &idd, float &sig) {
@@ -75,13 +159,15 @@ __device__ __forceinline__ void lut_lookup(const Model &m, int eid, float wet, f
     if (m.uniform) {
         const float q = (wet - m.depth0) * m.inv_ddepth;
         if (!(q > 0.0f)) {
-            float2 r = row[0];
-            idd = r.x; sig = r.y;
+            const float2 r = row[0];
+            idd = r.x;
+            sig = r.y;
             return;
         }
         if (q > (float)(L - 1)) {
-            float2 r = row[L - 1];
-            idd = r.x; sig = r.y;
+            const float2 r = row[L - 1];
+            idd = r.x;
+            sig = r.y;
             return;
         }
         lo = min((int)q, L - 2);
@@ -90,12 +176,13 @@ __device__ __forceinline__ void lut_lookup(const Model &m, int eid, float wet, f
         const float *dep = m.depths + (long long)eid * L;
         int l = 0, r = L;
         while (l < r) {
-            int mid = (l + r) >> 1;
+            const int mid = (l + r) >> 1;
             if (wet <= dep[mid]) r = mid; else l = mid + 1;
         }
         if (l == 0 || l == L) {
-            float2 v = row[l == 0 ? 0 : L - 1];
-            idd = v.x; sig = v.y;
+            const float2 v = row[l == 0 ? 0 : L - 1];
+            idd = v.x;
+            sig = v.y;
             return;
         }
         lo = l - 1;
@@ -106,33 +193,57 @@ __device__ __forceinline__ void lut_lookup(const Model &m, int eid, float wet, f
     sig = fmaf(c.y - a.y, f, a.y);
 }
 
-// Per-layer constants, wave-uniform.
-struct LayerK {
-    int start, n, eid;
-    float r80, lim, A, B, C;
+// Brick summary for the reach test: centre in the beam's-eye view, distance
+// of the centre to the source, WET range (mm) of the voxels that matter.
+struct BrickInfo {
+    float3 hc;
+    float dsrc, wlo, whi;
 };
 
-__device__ __forceinline__ LayerK load_layer(const dh_layer *__restrict__ layers, const Model &m, int l) {
-    dh_layer L = layers[l];
-    LayerK k;
-    k.start = L.spot_start;
-    k.n = L.n_spots;
-    k.eid = L.energy_id;
-    k.r80 = L.r80;
-    k.lim = 1.1f * L.r80;
-    const int e = (unsigned)L.energy_id < (unsigned)m.n_energies ? L.energy_id : 0;
-    const float *c = m.dvp + (long long)e * m.dvp_len;
-    k.A = c[2];
-    k.B = c[3];
-    k.C = c[4];
-    return k;
+__device__ __forceinline__ float3 brick_centre(const Grid &g, const Geom &b, int bi, int bj, int bk) {
+    const float half = 0.5f * (BRICK - 1);
+    return make_float3(fmaf((float)bi * BRICK + half, g.s, -b.iso[0]), fmaf((float)bj * BRICK + half, g.s, -b.iso[1]),
+                       fmaf((float)bk * BRICK + half, g.s, -b.iso[2]));
 }
 
-// A layer record the kernels can use safely: energy id inside the tables and
-// spot range inside the spot array (the host builds such tables; records from
-// elsewhere that break this are skipped rather than read out of bounds).
-__device__ __forceinline__ bool layer_ok(const LayerK &L, const Model &m, int n_spots) {
-    return (unsigned)L.eid < (unsigned)m.n_energies && L.start >= 0 && L.n > 0 && L.start <= n_spots - L.n;
+__device__ __forceinline__ BrickInfo brick_info_at(const Geom &b, float3 vc, float wlo, float whi) {
+    BrickInfo B;
+    B.hc = image_to_head(b, vc);
+    const float sx = b.src[0] - vc.x, sy = b.src[1] - vc.y, sz = b.src[2] - vc.z;
+    B.dsrc = fast_sqrt(sx * sx + sy * sy + sz * sz);
+    B.wlo = wlo;
+    B.whi = whi;
+    return B;
+}
+
+// True when every spot of the reach record c is negligible for the whole
+// brick: the brick centre's distance to the nearest spot axis, less the brick
+// radius, exceeds cut * (an upper bound of the halo sigma over the brick).  The
+// bound takes sigma_air over the range of d = |src - v| - wet + 0.7 R80 (a
+// quadratic: ends and vertex) plus max |sigma_ms|, and the halo width without
+// its negative terms, over the brick's WET range.
+__device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, float ivx, float ivy, float cut,
+                                           float rbrick) {
+    if (!(c.r80 >= 5.0f)) return false;
+    const float fx = fmaf(-B.hc.z, ivx, 1.0f), fy = fmaf(-B.hc.z, ivy, 1.0f);
+    const float ex = fmaxf(0.0f, fmaxf(c.xmin * fx - B.hc.x, B.hc.x - c.xmax * fx));
+    const float ey = fmaxf(0.0f, fmaxf(c.ymin * fy - B.hc.y, B.hc.y - c.ymax * fy));
+    const float dlb = 0.9999f * fast_sqrt(c.lam_min * (ex * ex + ey * ey)) - rbrick;
+    if (!(dlb > 0.0f)) return false;
+    const float whi = fminf(B.whi, c.lim);
+    const float w_ub = fminf(fmaxf(whi, 0.1f), c.r80 - 0.1f);
+    const float sn_ub = 2.85f + 0.0014f * c.r80 * fast_log(w_ub + 3.0f) + 0.06f * w_ub;
+    const float d07 = 0.7f * c.r80;
+    const float dlo = B.dsrc - rbrick - whi + d07;
+    const float dhi = B.dsrc + rbrick - B.wlo + d07;
+    float sa = fmaxf(fabsf(fmaf(fmaf(c.A, dlo, c.B), dlo, c.C)), fabsf(fmaf(fmaf(c.A, dhi, c.B), dhi, c.C)));
+    if (c.A != 0.0f) {
+        const float vx = -c.B / (2.0f * c.A);
+        if (vx > dlo && vx < dhi) sa = fmaxf(sa, fabsf(fmaf(fmaf(c.A, vx, c.B), vx, c.C)));
+    }
+    const float st_ub = sa + c.sms_max;
+    const float sh_ub = 1.001f * fast_sqrt(st_ub * st_ub + sn_ub * sn_ub);
+    return dlb > cut * sh_ub;
 }
 
 // Per (voxel, layer) Gaussian parameters.
@@ -142,32 +253,33 @@ struct VoxTerms {
     float sh;        // halo sigma (culling bound)
 };
 
-__device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerK &L, float wet, float dsrc) {
+__device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, float wet, float dsrc) {
     float idd, sms;
-    lut_lookup(m, L.eid, wet, idd, sms);
-    const float r80 = L.r80;
+    lut_lookup(m, c.eid, wet, idd, sms);
+    const float r80 = c.r80;
     const float d = dsrc - wet + 0.7f * r80;
-    const float st = fmaf(fmaf(L.A, d, L.B), d, L.C) + sms;
+    const float st = fmaf(fmaf(c.A, d, c.B), d, c.C) + sms;
 
     // nuclear halo (nuclearHalo, IMPTClasses.cu:104-121)
     const float w = fminf(fmaxf(wet, 0.1f), r80 - 0.1f);
     const float e5 = w - r80 - 5.0f;
     const float r3 = r80 + 3.0f;
-    float sn = 2.85f + 0.0014f * r80 * logf(w + 3.0f) + 0.06f * w - 7.4e-5f * w * w - 0.22f * r80 / (e5 * e5);
-    float un = 0.052f * logf(1.13f + w / (11.2f - 0.023f * r80)) +
-               0.35f * (0.0017f * r80 * r80 - r80) / (r3 * r3 - w * w) - 1.61e-9f * w * r3 * r3;
+    float sn = 2.85f + 0.0014f * r80 * fast_log(w + 3.0f) + 0.06f * w - 7.4e-5f * w * w -
+               0.22f * r80 * fast_rcp(e5 * e5);
+    float un = 0.052f * fast_log(1.13f + w * fast_rcp(11.2f - 0.023f * r80)) +
+               0.35f * (0.0017f * r80 * r80 - r80) * fast_rcp(r3 * r3 - w * w) - 1.61e-9f * w * r3 * r3;
     sn = fmaxf(sn, 0.0f);
     un = fminf(fmaxf(un, 0.0f), 0.9f);
 
     const float st2 = st * st;
-    const float sh2 = st2 + sn * sn;
+    const float sh2 = fmaf(sn, sn, st2);
     VoxTerms t;
     const bool pos1 = st > 0.0f, pos2 = sh2 > 0.0f;
-    t.Af = pos1 ? (1.0f - un) * idd / (kTwoPi * st2 + 1e-8f) : 0.0f;
-    t.Bf = pos2 ? un * idd / (kTwoPi * sh2 + 1e-8f) : 0.0f;
-    t.a1 = pos1 ? (-0.5f * kLog2e) / st2 : 0.0f;
-    t.a2 = pos2 ? (-0.5f * kLog2e) / sh2 : 0.0f;
-    t.sh = sqrtf(sh2);
+    t.Af = pos1 ? (1.0f - un) * idd * fast_rcp(fmaf(kTwoPi, st2, 1e-8f)) : 0.0f;
+    t.Bf = pos2 ? un * idd * fast_rcp(fmaf(kTwoPi, sh2, 1e-8f)) : 0.0f;
+    t.a1 = pos1 ? (-0.5f * kLog2e) * fast_rcp(st2) : 0.0f;
+    t.a2 = pos2 ? (-0.5f * kLog2e) * fast_rcp(sh2) : 0.0f;
+    t.sh = fast_sqrt(sh2);
     return t;
 }
 
@@ -181,27 +293,36 @@ __device__ __forceinline__ float axis_d2(float hx, float hy, float hz, const Spo
 
 __device__ __forceinline__ void brick_of(const BrickGrid &bg, long long brick, int &bi, int &bj, int &bk) {
     bi = (int)(brick % bg.bx);
-    long long t = brick / bg.bx;
+    const long long t = brick / bg.bx;
     bj = (int)(t % bg.by);
     bk = (int)(t / bg.by);
 }
 
 // ---------------------------------------------------------------------------
 // Forward.  One wave per 4x4x4 brick, one voxel per lane; a lane owns its
-// voxel's sum (deterministic, no atomics).  Layers whose range no lane reaches
-// are skipped by ballot.  With culling, each 64-spot chunk of a layer is tested
-// lane-parallel against the brick (distance from the brick centre to the spot
-// axis vs cutoff * max halo sigma + brick radius) and only the surviving spots
-// are iterated, wave-uniformly (their records come in through scalar loads).
+// voxel's sum (deterministic, no atomics).  The lanes first judge 64 layers at
+// a time (reaches the brick's WET range, and with culling: some spot of the
+// layer within reach of the brick); the wave then walks the surviving layers.
+// Per layer, each 64-spot chunk is judged lane-parallel against the brick
+// (distance from the brick centre to the spot axis vs cutoff * max halo sigma
+// + brick radius) and the surviving spots are evaluated four at a time, their
+// records coming in through scalar loads.
+//
 // COUNT: no dose; counts[0] += in-range voxels x layer spots (the reference's
-// work), counts[1] += in-range voxels x spots surviving the cull (work done).
+// work), counts[1] += in-range voxels x spots evaluated here.
+__device__ __forceinline__ void spot_eval(const SpotExt &q, float hx, float hy, float hz, float a1, float a2,
+                                          float &P, float &H) {
+    const float d2 = axis_d2(hx, hy, hz, q);
+    P = fmaf(q.mu, __builtin_amdgcn_exp2f(a1 * d2), P);
+    H = fmaf(q.mu, __builtin_amdgcn_exp2f(a2 * d2), H);
+}
+
 template <bool CULL, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ wetv, float *__restrict__ dose, Grid g,
                                                    Geom b, BrickGrid bg, unsigned nblocks, Model m,
-                                                   const dh_layer *__restrict__ layers, int n_layers,
-                                                   const SpotExt *__restrict__ sp, int n_spots, float cut,
-                                                   float rbrick, int accumulate,
-                                                   unsigned long long *__restrict__ counts = nullptr) {
+                                                   const LayerC *__restrict__ lc, int n_layers,
+                                                   const SpotExt *__restrict__ sp, float cut, float rbrick,
+                                                   int accumulate, unsigned long long *__restrict__ counts = nullptr) {
     const unsigned blk = xcd_remap(blockIdx.x, nblocks);
     const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
     if (brick >= bg.count()) return;  // whole wave leaves together
@@ -217,65 +338,79 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
     const float3 h = image_to_head(b, v);
     const float sx = b.src[0] - v.x, sy = b.src[1] - v.y, sz = b.src[2] - v.z;
     const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
-
-    float3 hc = make_float3(0.f, 0.f, 0.f);
-    if (CULL) {
-        const float half = 0.5f * (BRICK - 1);
-        const float3 vc = make_float3(fmaf((float)bi * BRICK + half, g.s, -b.iso[0]),
-                                      fmaf((float)bj * BRICK + half, g.s, -b.iso[1]),
-                                      fmaf((float)bk * BRICK + half, g.s, -b.iso[2]));
-        hc = image_to_head(b, vc);
-    }
+    const float ivx = 1.0f / b.vsadx, ivy = 1.0f / b.vsady;
+    const BrickInfo B = brick_info_at(b, brick_centre(g, b, bi, bj, bk), wave_min(wet),
+                                      wave_max(valid ? wet : -3.0e38f));
 
     float acc = 0.0f;
     unsigned long long n_dense = 0, n_exec = 0;
-    for (int l = 0; l < n_layers; ++l) {
-        const LayerK L = load_layer(layers, m, l);
-        if (!layer_ok(L, m, n_spots)) continue;
-        const bool act = wet <= L.lim;
-        const unsigned long long amask = __ballot(act);
-        if (!amask) continue;
-        if (COUNT) n_dense += (unsigned long long)__popcll(amask) * (unsigned)L.n;
-        const VoxTerms T = vox_terms(m, L, wet, dsrc);
-        float rc2 = 0.0f;
-        if (CULL) {
-            const float shmax = wave_max(act ? T.sh : 0.0f);
-            const float rc = fmaf(cut, shmax, rbrick);
-            rc2 = rc * rc;
+    for (int l0 = 0; l0 < n_layers; l0 += WAVE) {
+        bool lrel = false;
+        if (l0 + lane < n_layers) {
+            const LayerC c = lc[l0 + lane];
+            lrel = c.ok && B.wlo <= c.lim;
+            if (CULL && lrel) lrel = !reach_skip(c, B, ivx, ivy, cut, rbrick);
         }
-        float P = 0.0f, H = 0.0f;
-        const SpotExt *lsp = sp + L.start;
-        for (int c0 = 0; c0 < L.n; c0 += WAVE) {
-            const int nin = min(WAVE, L.n - c0);
-            unsigned long long mask;
+        unsigned long long lmask = __ballot(lrel);
+        while (lmask) {
+            const int l = l0 + __builtin_ctzll(lmask);
+            lmask &= lmask - 1ull;
+            const LayerC c = lc[l];
+            const bool act = wet <= c.lim;
+            const unsigned long long amask = __ballot(act);
+            if (!amask) continue;
+            const VoxTerms T = vox_terms(m, c, wet, dsrc);
+            float rc2 = 0.0f;
             if (CULL) {
-                bool rel = false;
-                if (lane < nin) {
-                    const SpotExt q = lsp[c0 + lane];
-                    rel = axis_d2(hc.x, hc.y, hc.z, q) < rc2;
+                const float rc = fmaf(cut, wave_max(act ? T.sh : 0.0f), rbrick);
+                rc2 = rc * rc;
+            }
+            float P = 0.0f, H = 0.0f;
+            const SpotExt *lsp = sp + c.start;
+            for (int c0 = 0; c0 < c.n; c0 += WAVE) {
+                const int nin = min(WAVE, c.n - c0);
+                unsigned long long mask;
+                if (CULL) {
+                    bool rel = false;
+                    if (lane < nin) rel = axis_d2(B.hc.x, B.hc.y, B.hc.z, lsp[c0 + lane]) < rc2;
+                    mask = __ballot(rel);
+                } else {
+                    mask = nin == WAVE ? ~0ull : ((1ull << nin) - 1ull);
                 }
-                mask = __ballot(rel);
-            } else {
-                mask = nin == WAVE ? ~0ull : ((1ull << nin) - 1ull);
+                if (COUNT) {
+                    n_exec += (unsigned long long)__popcll(amask) * (unsigned)__popcll(mask);
+                    continue;
+                }
+                const SpotExt *csp = lsp + c0;
+                while (__popcll(mask) >= 4) {
+                    const int s0 = __builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    const int s1 = __builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    const int s2 = __builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    const int s3 = __builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    const SpotExt q0 = csp[s0], q1 = csp[s1], q2 = csp[s2], q3 = csp[s3];
+                    spot_eval(q0, h.x, h.y, h.z, T.a1, T.a2, P, H);
+                    spot_eval(q1, h.x, h.y, h.z, T.a1, T.a2, P, H);
+                    spot_eval(q2, h.x, h.y, h.z, T.a1, T.a2, P, H);
+                    spot_eval(q3, h.x, h.y, h.z, T.a1, T.a2, P, H);
+                }
+                while (mask) {
+                    const int s0 = __builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    spot_eval(csp[s0], h.x, h.y, h.z, T.a1, T.a2, P, H);
+                }
             }
-            if (COUNT) {
-                n_exec += (unsigned long long)__popcll(amask) * (unsigned)__popcll(mask);
-                continue;
-            }
-            while (mask) {
-                const int s = __builtin_ctzll(mask);
-                mask &= mask - 1ull;
-                const SpotExt q = lsp[c0 + s];
-                const float d2 = axis_d2(h.x, h.y, h.z, q);
-                const float e1 = __builtin_amdgcn_exp2f(T.a1 * d2);
-                const float e2 = __builtin_amdgcn_exp2f(T.a2 * d2);
-                P = fmaf(q.mu, e1, P);
-                H = fmaf(q.mu, e2, H);
-            }
+            if (act) acc = fmaf(T.Af, P, fmaf(T.Bf, H, acc));
         }
-        if (act) acc = fmaf(T.Af, P, fmaf(T.Bf, H, acc));
     }
     if (COUNT) {
+        for (int l = 0; l < n_layers; ++l) {
+            const LayerC c = lc[l];
+            if (c.ok) n_dense += (unsigned long long)__popcll(__ballot(wet <= c.lim)) * (unsigned)c.n;
+        }
         if (lane == 0) {
             atomicAdd(&counts[0], n_dense);
             atomicAdd(&counts[1], n_exec);
@@ -286,31 +421,56 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
 }
 
 // ---------------------------------------------------------------------------
+// One wave per brick: WET range (mm) of the voxels with dL/dD != 0; an empty
+// brick gets (+big, -big).
+__global__ __launch_bounds__(256) void k_brick_summary(const float *__restrict__ wetv,
+                                                      const float *__restrict__ gdose, Grid g, BrickGrid bg,
+                                                      float2 *__restrict__ out) {
+    const long long brick = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (brick >= bg.count()) return;
+    const int lane = threadIdx.x & 63;
+    int bi, bj, bk;
+    brick_of(bg, brick, bi, bj, bk);
+    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
+    float lo = 3.0e38f, hi = -3.0e38f;
+    if (i < g.nx && j < g.ny && k < g.nz) {
+        const long long idx = i + (long long)g.nx * (j + (long long)g.ny * k);
+        if (gdose[idx] != 0.0f) lo = hi = wetv[idx] * 10.0f;
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (lane == 0) out[brick] = make_float2(lo, hi);
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint.  Workgroup = one wave = one 64-spot chunk of one layer x one slab of
 // bricks.  Lane = spot: its three sums stay in registers across the whole
-// slab.  Per brick the wave first acts as lanes = voxels to compute the voxel
-// terms, compacts the voxels that matter (in range, nonzero dL/dD) into LDS,
-// then every spot lane walks them with broadcast LDS reads.
+// slab.  The lanes judge 64 bricks at a time (holds voxels with dL/dD != 0
+// within the layer's range, and with culling: within reach of the chunk);
+// per surviving brick the wave acts as lanes = voxels to compute the voxel
+// terms, compacts the voxels that matter into LDS, then every spot lane walks
+// them with broadcast LDS reads.
 //   dL/dmu_s = sum_v g_v [A e1 + B e2]
 //   dL/dx_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADX) p_x
 //   dL/dy_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADY) p_y
 // with r = (hx - x_s, hy - y_s, hz) the voxel relative to the spot,
 // q = (t . r) / |t|^2, p = r - q t (its part perpendicular to the axis) and
 // a1, a2 = -1/(2 sigma^2) (natural-log scale).
+//
+// COUNT: no gradient; counts[2] += voxels (dL/dD != 0, in range) x chunk spots
+// (the reference's work), counts[3] += those voxels x spots evaluated here.
 struct LdsVox {
     float hx, hy, hz, a1;
     float a2, GA, GB, GAa;
     float GBb, pad0, pad1, pad2;
 };
 
-// COUNT: no gradient; counts[2] += voxels (dL/dD != 0, in range) x chunk
-// spots (the reference's work), counts[3] += those voxels x surviving spots.
 template <bool CULL, bool COUNT = false>
 __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wetv, const float *__restrict__ gdose,
                                                   Grid g, Geom b, BrickGrid bg, Model m,
-                                                  const dh_layer *__restrict__ layers, int n_layers,
+                                                  const LayerC *__restrict__ lc, int n_layers,
                                                   const SpotExt *__restrict__ sp, int n_spots, float cut,
-                                                  float rbrick, int nslab, const unsigned char *__restrict__ flags,
+                                                  float rbrick, int nslab, const float2 *__restrict__ bsum,
                                                   float4 *__restrict__ partial,
                                                   unsigned long long *__restrict__ counts = nullptr) {
     __shared__ LdsVox s_vox[WAVE];
@@ -318,8 +478,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
     // chunk -> (layer, first spot of the chunk within the layer)
     int cid = blockIdx.x, l = 0, c0 = -1;
     for (; l < n_layers; ++l) {
-        const int nl = layers[l].n_spots;
-        const int nch = (nl + WAVE - 1) / WAVE;
+        const int nch = (lc[l].n + WAVE - 1) / WAVE;
         if (cid < nch) {
             c0 = cid * WAVE;
             break;
@@ -327,8 +486,8 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
         cid -= nch;
     }
     if (c0 < 0) return;  // surplus workgroup of the upper-bound grid
-    const LayerK L = load_layer(layers, m, l);
-    if (!layer_ok(L, m, n_spots)) return;
+    const LayerC L = lc[l];
+    if (!L.ok) return;
     const int slab = blockIdx.y;
     const bool sval = c0 + lane < L.n;
     const int spot = L.start + c0 + lane;
@@ -337,96 +496,102 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
     else q = SpotExt{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f};
     const float itt = 1.0f / (q.tx * q.tx + q.ty * q.ty + 1.0f);
     const float ivx = 1.0f / b.vsadx, ivy = 1.0f / b.vsady;
+    const unsigned long long chunk_n = (unsigned long long)min(WAVE, L.n - c0);
+    LayerC R = L;  // reach of this chunk's spots only
+    if (CULL) reach_of(sp, L.start + c0, (int)chunk_n, m, L.eid, lane, R);
 
     const long long nb = bg.count();
     const long long b0 = nb * slab / nslab, b1 = nb * (slab + 1) / nslab;
     float K = 0.0f, FX = 0.0f, FY = 0.0f;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    const unsigned long long chunk_n = (unsigned long long)min(WAVE, L.n - c0);
     unsigned long long n_dense = 0, n_exec = 0;
 
-    // bricks flagged by k_brick_flags (any voxel with dL/dD != 0 in range),
-    // 64 flags per load, visited in ascending order
     for (long long base = b0; base < b1; base += WAVE) {
-      const bool fl = base + lane < b1 && flags[base + lane] != 0;
-      unsigned long long fm = __ballot(fl);
-      while (fm) {
-        const long long brick = base + __builtin_ctzll(fm);
-        fm &= fm - 1ull;
-        int bi, bj, bk;
-        brick_of(bg, brick, bi, bj, bk);
-        const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
-        const bool valid = i < g.nx && j < g.ny && k < g.nz;
-        const long long idx = valid ? i + (long long)g.nx * (j + (long long)g.ny * k) : 0;
-        const float gv = valid ? gdose[idx] : 0.0f;
-        const float wet = valid ? wetv[idx] * 10.0f : 3.0e38f;
-        const bool act = valid && gv != 0.0f && wet <= L.lim;
-        const unsigned long long am = __ballot(act);
-        if (!am) continue;
-        if (COUNT) n_dense += (unsigned long long)__popcll(am) * chunk_n;
-
-        const float3 v = voxel_xyz(g, b, i, j, k);
-        const float3 h = image_to_head(b, v);
-        const float sx = b.src[0] - v.x, sy = b.src[1] - v.y, sz = b.src[2] - v.z;
-        const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
-        const VoxTerms T = vox_terms(m, L, wet, dsrc);
-
-        bool rel = sval;
-        if (CULL) {
-            const float shmax = wave_max(act ? T.sh : 0.0f);
-            const float rc = fmaf(cut, shmax, rbrick);
-            const float half = 0.5f * (BRICK - 1);
-            const float3 hc = image_to_head(b, make_float3(fmaf((float)bi * BRICK + half, g.s, -b.iso[0]),
-                                                           fmaf((float)bj * BRICK + half, g.s, -b.iso[1]),
-                                                           fmaf((float)bk * BRICK + half, g.s, -b.iso[2])));
-            rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
-            if (!__ballot(rel)) continue;
-        }
-        if (COUNT) {
-            n_exec += (unsigned long long)__popcll(am) * (unsigned)__popcll(__ballot(rel));
-            continue;
-        }
-        if (act) {
-            const int pos = __popcll(am & lt_mask);
-            LdsVox e;
-            e.hx = h.x;
-            e.hy = h.y;
-            e.hz = h.z;
-            e.a1 = T.a1;
-            e.a2 = T.a2;
-            e.GA = gv * T.Af;
-            e.GB = gv * T.Bf;
-            // derivative scales in natural-log units: a_nat = a_exp2 / log2(e)
-            e.GAa = e.GA * (T.a1 * (1.0f / kLog2e));
-            e.GBb = e.GB * (T.a2 * (1.0f / kLog2e));
-            e.pad0 = e.pad1 = e.pad2 = 0.0f;
-            s_vox[pos] = e;
-        }
-        __syncthreads();
-        const int na = __popcll(am);
-        if (rel) {
-            for (int t = 0; t < na; ++t) {
-                const LdsVox e = s_vox[t];
-                // in-plane offset e_xy, its part perpendicular to the axis p,
-                // q_e = (e.t)/|t|^2; d2 = e_xy . p (no cancellation)
-                const float u = fmaf(e.hz, q.tx, e.hx - q.x);
-                const float v = fmaf(e.hz, q.ty, e.hy - q.y);
-                const float qe = fmaf(u, q.tx, v * q.ty) * itt;
-                const float px = fmaf(-qe, q.tx, u), py = fmaf(-qe, q.ty, v);
-                const float d2 = fmaf(u, px, v * py);
-                const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
-                const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
-                K = fmaf(e.GA, e1, K);
-                K = fmaf(e.GB, e2, K);
-                const float F = fmaf(e.GAa, e1, e.GBb * e2);
-                // q of the ray r = e - hz t is qe - hz
-                const float qr = qe - e.hz;
-                FX = fmaf(F * fmaf(qr, ivx, 1.0f), px, FX);
-                FY = fmaf(F * fmaf(qr, ivy, 1.0f), py, FY);
+        bool brel = false;
+        if (base + lane < b1) {
+            const float2 ws = bsum[base + lane];
+            brel = ws.x <= ws.y && ws.x <= L.lim;
+            if (CULL && brel && !COUNT) {
+                int ti, tj, tk;
+                brick_of(bg, base + lane, ti, tj, tk);
+                brel = !reach_skip(R, brick_info_at(b, brick_centre(g, b, ti, tj, tk), ws.x, ws.y), ivx, ivy, cut,
+                                   rbrick);
             }
         }
-        __syncthreads();
-      }
+        unsigned long long fm = __ballot(brel);
+        while (fm) {
+            const long long brick = base + __builtin_ctzll(fm);
+            fm &= fm - 1ull;
+            int bi, bj, bk;
+            brick_of(bg, brick, bi, bj, bk);
+            const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
+            const bool valid = i < g.nx && j < g.ny && k < g.nz;
+            const long long idx = valid ? i + (long long)g.nx * (j + (long long)g.ny * k) : 0;
+            const float gv = valid ? gdose[idx] : 0.0f;
+            const float wet = valid ? wetv[idx] * 10.0f : 3.0e38f;
+            const bool act = valid && gv != 0.0f && wet <= L.lim;
+            const unsigned long long am = __ballot(act);
+            if (!am) continue;
+            if (COUNT) n_dense += (unsigned long long)__popcll(am) * chunk_n;
+
+            const float3 v = voxel_xyz(g, b, i, j, k);
+            const float3 h = image_to_head(b, v);
+            const float sx = b.src[0] - v.x, sy = b.src[1] - v.y, sz = b.src[2] - v.z;
+            const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
+            const VoxTerms T = vox_terms(m, L, wet, dsrc);
+
+            bool rel = sval;
+            if (CULL) {
+                const float rc = fmaf(cut, wave_max(act ? T.sh : 0.0f), rbrick);
+                const float3 hc = image_to_head(b, brick_centre(g, b, bi, bj, bk));
+                rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
+            }
+            if (COUNT) {
+                n_exec += (unsigned long long)__popcll(am) * (unsigned)__popcll(__ballot(rel));
+                continue;
+            }
+            if (!__ballot(rel)) continue;
+            if (act) {
+                const int pos = __popcll(am & lt_mask);
+                LdsVox e;
+                e.hx = h.x;
+                e.hy = h.y;
+                e.hz = h.z;
+                e.a1 = T.a1;
+                e.a2 = T.a2;
+                e.GA = gv * T.Af;
+                e.GB = gv * T.Bf;
+                // derivative scales in natural-log units: a_nat = a_exp2 / log2(e)
+                e.GAa = e.GA * (T.a1 * (1.0f / kLog2e));
+                e.GBb = e.GB * (T.a2 * (1.0f / kLog2e));
+                e.pad0 = e.pad1 = e.pad2 = 0.0f;
+                s_vox[pos] = e;
+            }
+            __syncthreads();
+            const int na = __popcll(am);
+            if (rel) {
+                for (int t = 0; t < na; ++t) {
+                    const LdsVox e = s_vox[t];
+                    // in-plane offset e_xy, its part perpendicular to the axis p,
+                    // q_e = (e.t)/|t|^2; d2 = e_xy . p (no cancellation)
+                    const float u = fmaf(e.hz, q.tx, e.hx - q.x);
+                    const float w = fmaf(e.hz, q.ty, e.hy - q.y);
+                    const float qe = fmaf(u, q.tx, w * q.ty) * itt;
+                    const float px = fmaf(-qe, q.tx, u), py = fmaf(-qe, q.ty, w);
+                    const float d2 = fmaf(u, px, w * py);
+                    const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
+                    const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
+                    K = fmaf(e.GA, e1, K);
+                    K = fmaf(e.GB, e2, K);
+                    const float F = fmaf(e.GAa, e1, e.GBb * e2);
+                    // q of the ray r = e - hz t is qe - hz
+                    const float qr = qe - e.hz;
+                    FX = fmaf(F * fmaf(qr, ivx, 1.0f), px, FX);
+                    FY = fmaf(F * fmaf(qr, ivy, 1.0f), py, FY);
+                }
+            }
+            __syncthreads();
+        }
     }
     if (COUNT) {
         if (lane == 0) {
@@ -438,30 +603,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
     if (sval) partial[(long long)slab * n_spots + spot] = make_float4(K, FX, FY, 0.0f);
 }
 
-// One wave per brick: flag bricks holding a voxel with dL/dD != 0 whose WET
-// is within reach of the deepest layer (wet <= 1.1 max R80).
-__global__ __launch_bounds__(256) void k_brick_flags(const float *__restrict__ wetv, const float *__restrict__ gdose,
-                                                    Grid g, BrickGrid bg, const dh_layer *__restrict__ layers,
-                                                    int n_layers, unsigned char *__restrict__ flags) {
-    const long long brick = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (brick >= bg.count()) return;
-    const int lane = threadIdx.x & 63;
-    float lim = -3.0e38f;
-    for (int l = 0; l < n_layers; ++l) lim = fmaxf(lim, 1.1f * layers[l].r80);
-    int bi, bj, bk;
-    brick_of(bg, brick, bi, bj, bk);
-    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
-    bool act = false;
-    if (i < g.nx && j < g.ny && k < g.nz) {
-        const long long idx = i + (long long)g.nx * (j + (long long)g.ny * k);
-        act = gdose[idx] != 0.0f && wetv[idx] * 10.0f <= lim;
-    }
-    const unsigned long long am = __ballot(act);
-    if (lane == 0) flags[brick] = am ? 1 : 0;
-}
-
-// Spots that no chunk covers (never happens for a consistent layer table, but
-// keeps the outputs defined) are zeroed by the reduce reading zeroed partials.
+// Spots that no chunk covers keep the zeroed partials (k_zero4).
 __global__ __launch_bounds__(256) void k_zero4(float4 *__restrict__ p, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -499,6 +641,34 @@ static int adjoint_slabs(const dh_grid &g, int n_spots, int n_layers) {
     return (int)d;
 }
 
+// Workspace: [SpotExt x S][LayerC x L][adjoint partials float4 x nslab*S][brick summaries float2]
+struct Workspace {
+    SpotExt *ext;
+    LayerC *layer;
+    float4 *partial;
+    float2 *bsum;
+    int nslab;
+    long long bytes;
+};
+
+static Workspace carve(void *base, const dh_grid &g, int S, int L) {
+    Workspace w;
+    char *p = reinterpret_cast<char *>(base);
+    w.nslab = adjoint_slabs(g, S, L);
+    const long long nb = (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
+    long long off = 0;
+    w.ext = reinterpret_cast<SpotExt *>(p + off);
+    off += align_up((long long)S * (long long)sizeof(SpotExt));
+    w.layer = reinterpret_cast<LayerC *>(p + off);
+    off += align_up((long long)(L > 0 ? L : 1) * (long long)sizeof(LayerC));
+    w.partial = reinterpret_cast<float4 *>(p + off);
+    off += align_up((long long)w.nslab * S * 16LL);
+    w.bsum = reinterpret_cast<float2 *>(p + off);
+    off += align_up(nb * 8LL);
+    w.bytes = off;
+    return w;
+}
+
 static bool fill_model(const dh_beam_model *bm, Model &m) {
     if (!bm || !bm->idd_sigma || !bm->dvp || bm->n_energies <= 0 || bm->lut_len < 2 || bm->dvp_len < 5 ||
         (!bm->depth_uniform && !bm->depths) || (bm->depth_uniform && !(bm->ddepth > 0.0f))) {
@@ -523,6 +693,10 @@ static bool common_args(const dh_grid &g, const dh_beam_geom *geom, const dh_spo
         set_error("invalid grid, geometry or spot set");
         return false;
     }
+    if ((double)g.nx * g.ny * g.nz >= 2147483647.0) {
+        set_error("grid %d x %d x %d too large (below 2^31 voxels required)", g.nx, g.ny, g.nz);
+        return false;
+    }
     if (ss->n_spots < 0 || ss->n_layers < 0 || (ss->n_spots > 0 && (!ss->spots || !ss->layers))) {
         set_error("invalid spot set (n_spots=%d, n_layers=%d)", ss->n_spots, ss->n_layers);
         return false;
@@ -536,16 +710,70 @@ static bool common_args(const dh_grid &g, const dh_beam_geom *geom, const dh_spo
     return true;
 }
 
+// Spot records and per-layer reach into the workspace.
+static int prepare(hipStream_t st, const dh_spot_set *spots, const dh_beam_geom *geom, const Model &m,
+                   const Workspace &w) {
+    const int S = spots->n_spots, L = spots->n_layers;
+    hipLaunchKernelGGL(k_spot_prep, dim3((S + 255) / 256), dim3(256), 0, st, spots->spots, w.ext, S,
+                       1.0f / geom->vsadx, 1.0f / geom->vsady);
+    int rc = check_launch("k_spot_prep");
+    if (rc || L == 0) return rc;
+    hipLaunchKernelGGL(k_layer_prep, dim3(L), dim3(64), 0, st, w.ext, S, spots->layers, m, w.layer);
+    return check_launch("k_layer_prep");
+}
+
+static float brick_radius(const dh_grid &g) { return 0.5f * (BRICK - 1) * 1.7320508075688772f * g.spacing * 1.0001f; }
+
+template <bool COUNT>
+static int launch_forward(hipStream_t st, const float *wet, float *dose, const dh_grid &g, const dh_beam_geom *geom,
+                          const Model &m, const dh_spot_set *spots, const Workspace &w, float cutoff_sigma,
+                          int accumulate, unsigned long long *counts) {
+    const Grid gg = make_grid(g);
+    const Geom gb = make_geom(*geom);
+    const BrickGrid bg = brick_grid(gg);
+    const unsigned nb = (unsigned)((bg.count() + 3) / 4);
+    const int S = spots->n_spots;
+    const int nl = S > 0 ? spots->n_layers : 0;
+    if (cutoff_sigma > 0.0f)
+        hipLaunchKernelGGL((k_pb_forward<true, COUNT>), dim3(nb), dim3(256), 0, st, wet, dose, gg, gb, bg, nb, m,
+                           w.layer, nl, w.ext, cutoff_sigma, brick_radius(g), accumulate, counts);
+    else
+        hipLaunchKernelGGL((k_pb_forward<false, COUNT>), dim3(nb), dim3(256), 0, st, wet, dose, gg, gb, bg, nb, m,
+                           w.layer, nl, w.ext, 0.0f, brick_radius(g), accumulate, counts);
+    return check_launch(COUNT ? "k_pb_forward(count)" : "k_pb_forward");
+}
+
+template <bool COUNT>
+static int launch_adjoint(hipStream_t st, const float *wet, const float *grad_dose, const dh_grid &g,
+                          const dh_beam_geom *geom, const Model &m, const dh_spot_set *spots, const Workspace &w,
+                          float cutoff_sigma, unsigned long long *counts) {
+    const Grid gg = make_grid(g);
+    const Geom gb = make_geom(*geom);
+    const BrickGrid bg = brick_grid(gg);
+    const int S = spots->n_spots;
+    hipLaunchKernelGGL(k_brick_summary, dim3((unsigned)((bg.count() + 3) / 4)), dim3(256), 0, st, wet, grad_dose,
+                       gg, bg, w.bsum);
+    int rc = check_launch("k_brick_summary");
+    if (rc) return rc;
+    const unsigned nch = (unsigned)(((long long)S + 63LL * spots->n_layers) / 64 + 1);
+    if (cutoff_sigma > 0.0f)
+        hipLaunchKernelGGL((k_pb_adjoint<true, COUNT>), dim3(nch, w.nslab), dim3(64), 0, st, wet, grad_dose, gg, gb,
+                           bg, m, w.layer, spots->n_layers, w.ext, S, cutoff_sigma, brick_radius(g), w.nslab, w.bsum,
+                           w.partial, counts);
+    else
+        hipLaunchKernelGGL((k_pb_adjoint<false, COUNT>), dim3(nch, w.nslab), dim3(64), 0, st, wet, grad_dose, gg, gb,
+                           bg, m, w.layer, spots->n_layers, w.ext, S, 0.0f, brick_radius(g), w.nslab, w.bsum,
+                           w.partial, counts);
+    return check_launch(COUNT ? "k_pb_adjoint(count)" : "k_pb_adjoint");
+}
+
 }  // namespace dh
 
 using namespace dh;
 
 extern "C" int64_t dh_pb_workspace_size(int32_t n_spots, int32_t n_layers, dh_grid g) {
     if (n_spots <= 0) return 0;
-    const int d = adjoint_slabs(g, n_spots, n_layers);
-    const long long nb = (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
-    return align_up((long long)n_spots * (long long)sizeof(SpotExt)) + align_up((long long)d * n_spots * 16LL) +
-           align_up(nb);
+    return carve(nullptr, g, n_spots, n_layers).bytes;
 }
 
 extern "C" int dh_pb_forward(const float *wet, float *dose, dh_grid g, const dh_beam_geom *geom,
@@ -559,27 +787,12 @@ extern "C" int dh_pb_forward(const float *wet, float *dose, dh_grid g, const dh_
     Model m;
     if (!fill_model(model, m)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    const Grid gg = make_grid(g);
-    const Geom gb = make_geom(*geom);
-    SpotExt *ext = reinterpret_cast<SpotExt *>(workspace);
-    const int S = spots->n_spots;
-    if (S > 0) {
-        hipLaunchKernelGGL(k_spot_prep, dim3((S + 255) / 256), dim3(256), 0, st, spots->spots, ext, S,
-                           1.0f / geom->vsadx, 1.0f / geom->vsady);
-        int rc = check_launch("k_spot_prep");
+    const Workspace w = carve(workspace, g, spots->n_spots, spots->n_layers);
+    if (spots->n_spots > 0) {
+        const int rc = prepare(st, spots, geom, m, w);
         if (rc) return rc;
     }
-    const BrickGrid bg = brick_grid(gg);
-    const unsigned nb = (unsigned)((bg.count() + 3) / 4);
-    const float rbrick = 0.5f * (BRICK - 1) * 1.7320508075688772f * g.spacing * 1.0001f;
-    const int nl = S > 0 ? spots->n_layers : 0;
-    if (cutoff_sigma > 0.0f)
-        hipLaunchKernelGGL(k_pb_forward<true>, dim3(nb), dim3(256), 0, st, wet, dose, gg, gb, bg, nb, m,
-                           spots->layers, nl, ext, S, cutoff_sigma, rbrick, accumulate);
-    else
-        hipLaunchKernelGGL(k_pb_forward<false>, dim3(nb), dim3(256), 0, st, wet, dose, gg, gb, bg, nb, m,
-                           spots->layers, nl, ext, S, 0.0f, rbrick, accumulate);
-    return check_launch("k_pb_forward");
+    return launch_forward<false>(st, wet, dose, g, geom, m, spots, w, cutoff_sigma, accumulate, nullptr);
 }
 
 extern "C" int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g, const dh_beam_geom *geom,
@@ -596,37 +809,16 @@ extern "C" int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g
     Model m;
     if (!fill_model(model, m)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    const Grid gg = make_grid(g);
-    const Geom gb = make_geom(*geom);
-    SpotExt *ext = reinterpret_cast<SpotExt *>(workspace);
-    float4 *partial = reinterpret_cast<float4 *>(reinterpret_cast<char *>(workspace) +
-                                                 align_up((long long)S * (long long)sizeof(SpotExt)));
-    const int nslab = adjoint_slabs(g, S, spots->n_layers);
-    hipLaunchKernelGGL(k_spot_prep, dim3((S + 255) / 256), dim3(256), 0, st, spots->spots, ext, S,
-                       1.0f / geom->vsadx, 1.0f / geom->vsady);
-    int rc = check_launch("k_spot_prep");
+    const Workspace w = carve(workspace, g, S, spots->n_layers);
+    int rc = prepare(st, spots, geom, m, w);
     if (rc) return rc;
-    const long long np = (long long)nslab * S;
-    hipLaunchKernelGGL(k_zero4, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, partial, np);
+    const long long np = (long long)w.nslab * S;
+    hipLaunchKernelGGL(k_zero4, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, w.partial, np);
     rc = check_launch("k_zero4");
     if (rc) return rc;
-    const BrickGrid bg = brick_grid(gg);
-    unsigned char *flags = reinterpret_cast<unsigned char *>(partial) + align_up(np * 16LL);
-    hipLaunchKernelGGL(k_brick_flags, dim3((unsigned)((bg.count() + 3) / 4)), dim3(256), 0, st, wet, grad_dose, gg,
-                       bg, spots->layers, spots->n_layers, flags);
-    rc = check_launch("k_brick_flags");
+    rc = launch_adjoint<false>(st, wet, grad_dose, g, geom, m, spots, w, cutoff_sigma, nullptr);
     if (rc) return rc;
-    const unsigned nch = (unsigned)(((long long)S + 63LL * spots->n_layers) / 64 + 1);
-    const float rbrick = 0.5f * (BRICK - 1) * 1.7320508075688772f * g.spacing * 1.0001f;
-    if (cutoff_sigma > 0.0f)
-        hipLaunchKernelGGL(k_pb_adjoint<true>, dim3(nch, nslab), dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
-                           spots->layers, spots->n_layers, ext, S, cutoff_sigma, rbrick, nslab, flags, partial);
-    else
-        hipLaunchKernelGGL(k_pb_adjoint<false>, dim3(nch, nslab), dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
-                           spots->layers, spots->n_layers, ext, S, 0.0f, rbrick, nslab, flags, partial);
-    rc = check_launch("k_pb_adjoint");
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_pb_reduce, dim3((S + 255) / 256), dim3(256), 0, st, partial, ext, S, nslab, grad_mu,
+    hipLaunchKernelGGL(k_pb_reduce, dim3((S + 255) / 256), dim3(256), 0, st, w.partial, w.ext, S, w.nslab, grad_mu,
                        grad_xy);
     return check_launch("k_pb_reduce");
 }
@@ -645,42 +837,10 @@ extern "C" int dh_pb_count_pairs(const float *wet, const float *grad_dose, dh_gr
     Model m;
     if (!fill_model(model, m)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    const Grid gg = make_grid(g);
-    const Geom gb = make_geom(*geom);
-    SpotExt *ext = reinterpret_cast<SpotExt *>(workspace);
-    hipLaunchKernelGGL(k_spot_prep, dim3((S + 255) / 256), dim3(256), 0, st, spots->spots, ext, S,
-                       1.0f / geom->vsadx, 1.0f / geom->vsady);
-    int rc = check_launch("k_spot_prep");
+    const Workspace w = carve(workspace, g, S, spots->n_layers);
+    int rc = prepare(st, spots, geom, m, w);
     if (rc) return rc;
-    const BrickGrid bg = brick_grid(gg);
-    const unsigned nb = (unsigned)((bg.count() + 3) / 4);
-    const float rbrick = 0.5f * (BRICK - 1) * 1.7320508075688772f * g.spacing * 1.0001f;
-    const bool cull = cutoff_sigma > 0.0f;
-    if (cull)
-        hipLaunchKernelGGL((k_pb_forward<true, true>), dim3(nb), dim3(256), 0, st, wet, nullptr, gg, gb, bg, nb, m,
-                           spots->layers, spots->n_layers, ext, S, cutoff_sigma, rbrick, 0, counts);
-    else
-        hipLaunchKernelGGL((k_pb_forward<false, true>), dim3(nb), dim3(256), 0, st, wet, nullptr, gg, gb, bg, nb, m,
-                           spots->layers, spots->n_layers, ext, S, 0.0f, rbrick, 0, counts);
-    rc = check_launch("k_pb_forward(count)");
+    rc = launch_forward<true>(st, wet, nullptr, g, geom, m, spots, w, cutoff_sigma, 0, counts);
     if (rc || !grad_dose) return rc;
-    const int nslab = adjoint_slabs(g, S, spots->n_layers);
-    float4 *partial = reinterpret_cast<float4 *>(reinterpret_cast<char *>(workspace) +
-                                                 align_up((long long)S * (long long)sizeof(SpotExt)));
-    const long long np = (long long)nslab * S;
-    unsigned char *flags = reinterpret_cast<unsigned char *>(partial) + align_up(np * 16LL);
-    hipLaunchKernelGGL(k_brick_flags, dim3((unsigned)((bg.count() + 3) / 4)), dim3(256), 0, st, wet, grad_dose, gg,
-                       bg, spots->layers, spots->n_layers, flags);
-    rc = check_launch("k_brick_flags");
-    if (rc) return rc;
-    const unsigned nch = (unsigned)(((long long)S + 63LL * spots->n_layers) / 64 + 1);
-    if (cull)
-        hipLaunchKernelGGL((k_pb_adjoint<true, true>), dim3(nch, nslab), dim3(64), 0, st, wet, grad_dose, gg, gb, bg,
-                           m, spots->layers, spots->n_layers, ext, S, cutoff_sigma, rbrick, nslab, flags, partial,
-                           counts);
-    else
-        hipLaunchKernelGGL((k_pb_adjoint<false, true>), dim3(nch, nslab), dim3(64), 0, st, wet, grad_dose, gg, gb,
-                           bg, m, spots->layers, spots->n_layers, ext, S, 0.0f, rbrick, nslab, flags, partial,
-                           counts);
-    return check_launch("k_pb_adjoint(count)");
+    return launch_adjoint<true>(st, wet, grad_dose, g, geom, m, spots, w, cutoff_sigma, counts);
 }

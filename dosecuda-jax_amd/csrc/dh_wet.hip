@@ -104,6 +104,33 @@ __device__ __forceinline__ float axis_exit(float c0, float du, int n) {
     return 3.0e38f;
 }
 
+// Bulk samples: position by one fma per axis from the voxel-centre index,
+// corners fetched as two-float pairs with buffer loads (32-bit offsets, the
+// per-corner row / slice strides as scalar offsets, out-of-range reads give 0).
+__device__ __forceinline__ float trilinear_buf(__amdgpu_buffer_rsrc_t rs, float cx, float cy, float cz, int pnx,
+                                               int pnxy, int so_y, int so_z, int so_yz) {
+    // c >= 0 here, so truncation is floor
+    const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
+    const float ax = __builtin_amdgcn_fractf(cx), ay = __builtin_amdgcn_fractf(cy), az = __builtin_amdgcn_fractf(cz);
+    const unsigned off = ((unsigned)ix + __umul24((unsigned)iy, (unsigned)pnx) + __umul24((unsigned)iz, (unsigned)pnxy))
+                         << 2;
+    const auto r00 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+    const auto r10 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, so_y, 0);
+    const auto r01 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, so_z, 0);
+    const auto r11 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, so_yz, 0);
+    const float a00 = __uint_as_float(r00[0]), b00 = __uint_as_float(r00[1]);
+    const float a10 = __uint_as_float(r10[0]), b10 = __uint_as_float(r10[1]);
+    const float a01 = __uint_as_float(r01[0]), b01 = __uint_as_float(r01[1]);
+    const float a11 = __uint_as_float(r11[0]), b11 = __uint_as_float(r11[1]);
+    const float x00 = fmaf(ax, b00 - a00, a00);
+    const float x10 = fmaf(ax, b10 - a10, a10);
+    const float x01 = fmaf(ax, b01 - a01, a01);
+    const float x11 = fmaf(ax, b11 - a11, a11);
+    const float y0 = fmaf(ay, x10 - x00, x00);
+    const float y1 = fmaf(ay, x11 - x01, x01);
+    return fmaf(az, y1 - y0, y0);
+}
+
 __global__ __launch_bounds__(256) void k_wet_raymarch(const float *__restrict__ rho, float *__restrict__ wet, Grid g,
                                                      Geom b, BrickGrid bg, unsigned nblocks) {
     const unsigned blk = xcd_remap(blockIdx.x, nblocks);
@@ -124,22 +151,38 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(const float *__restrict__ 
     const float ux = dx / len, uy = dy / len, uz = dz / len;
 
     const float c0x = (v.x + b.iso[0]) * g.inv_s, c0y = (v.y + b.iso[1]) * g.inv_s, c0z = (v.z + b.iso[2]) * g.inv_s;
-    float t_exit = fminf(axis_exit(c0x, ux * g.inv_s, g.nx),
-                         fminf(axis_exit(c0y, uy * g.inv_s, g.ny), axis_exit(c0z, uz * g.inv_s, g.nz)));
+    const float dux = ux * g.inv_s, duy = uy * g.inv_s, duz = uz * g.inv_s;
+    float t_exit = fminf(axis_exit(c0x, dux, g.nx), fminf(axis_exit(c0y, duy, g.ny), axis_exit(c0z, duz, g.nz)));
     t_exit = fminf(t_exit, 1.0e7f);
     const int n_safe = max(0, (int)floorf(t_exit) - 1);
     const int n_end = (int)floorf(t_exit) + 3;
 
     const int pnx = g.nx + 1, pnxy = (g.nx + 1) * (g.ny + 1);
-    float acc = -0.05f;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)rho, (short)0, (int)((unsigned)pnxy * (unsigned)(g.nz + 1) * 4u),
+                                          0x00020000);
+    const int so_y = pnx * 4, so_z = pnxy * 4, so_yz = (pnxy + pnx) * 4;
+    float acc0 = -0.05f, acc1 = 0.0f;
     int n = 0;
-    for (; n < n_safe; ++n) {
-        const float fn = (float)n;
-        const float cx = (fmaf(ux, fn, v.x) + b.iso[0]) * g.inv_s;
-        const float cy = (fmaf(uy, fn, v.y) + b.iso[1]) * g.inv_s;
-        const float cz = (fmaf(uz, fn, v.z) + b.iso[2]) * g.inv_s;
-        acc = fmaf(fmaxf(trilinear_padded(rho, pnx, pnxy, cx, cy, cz), 0.0f), 0.1f, acc);
+    // two samples per iteration: independent loads in flight, two accumulators
+    for (; n + 1 < n_safe; n += 2) {
+        const float f0 = (float)n, f1 = (float)(n + 1);
+        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), pnx, pnxy,
+                                       so_y, so_z, so_yz);
+        const float v1 = trilinear_buf(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z), pnx, pnxy,
+                                       so_y, so_z, so_yz);
+        acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
+        acc1 = fmaf(fmaxf(v1, 0.0f), 0.1f, acc1);
     }
+    if (n < n_safe) {
+        const float f0 = (float)n;
+        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), pnx, pnxy,
+                                       so_y, so_z, so_yz);
+        acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
+        ++n;
+    }
+    float acc = acc0 + acc1;
+    // tail: the reference's per-sample position arithmetic and bounds test
     for (; n < n_end; ++n) {
         const float fn = (float)n;
         const float cx = (fmaf(ux, fn, v.x) + b.iso[0]) * g.inv_s;
@@ -251,9 +294,9 @@ static bool grid_ok(const dh_grid &g) {
         set_error("invalid grid %d x %d x %d, spacing %g", g.nx, g.ny, g.nz, (double)g.spacing);
         return false;
     }
-    // 32-bit element offsets inside the kernels
-    if ((double)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) >= 2147483647.0) {
-        set_error("grid %d x %d x %d too large (padded volume must stay below 2^31 voxels)", g.nx, g.ny, g.nz);
+    // 32-bit byte offsets of the ray-march buffer loads
+    if ((double)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) >= 1073741824.0) {
+        set_error("grid %d x %d x %d too large (padded volume must stay below 2^30 voxels)", g.nx, g.ny, g.nz);
         return false;
     }
     return true;
