@@ -45,5 +45,5 @@ extern "C" int dh_last_error(char *buf, size_t len) {
 }
 
 extern "C" int64_t dh_padded_len(dh_grid g) {
-    return (int64_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    return (int64_t)4 * g.nx * g.ny * (g.nz + 1);
 }

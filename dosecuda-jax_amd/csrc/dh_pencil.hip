@@ -323,8 +323,9 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
                                                    const LayerC *__restrict__ lc, int n_layers,
                                                    const SpotExt *__restrict__ sp, float cut, float rbrick,
                                                    int accumulate, unsigned long long *__restrict__ counts = nullptr) {
-    const unsigned blk = xcd_remap(blockIdx.x, nblocks);
-    const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
+    // plain dispatch order: consecutive bricks go to different XCDs, which
+    // spreads the heavy bricks of the beam path over the whole chip
+    const long long brick = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (brick >= bg.count()) return;  // whole wave leaves together
     const int lane = threadIdx.x & 63;
     int bi, bj, bk;
@@ -631,11 +632,15 @@ __global__ __launch_bounds__(256) void k_pb_reduce(const float4 *__restrict__ pa
 static constexpr long long kAlign = 256;
 static long long align_up(long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
+// Slabs of bricks per spot chunk: enough (chunk, slab) workgroups (~32k) that
+// the few heavy ones (chunks and slabs in the beam path) do not leave the chip
+// idle at the end, with at least 64 bricks per slab.
 static int adjoint_slabs(const dh_grid &g, int n_spots, int n_layers) {
     const long long nb = (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
     const long long nch = ((long long)n_spots + 63LL * n_layers) / 64 + 1;
-    long long d = (8192 + nch - 1) / nch;
-    if (d > 256) d = 256;
+    long long d = (32768 + nch - 1) / nch;
+    if (d > 1024) d = 1024;
+    if (d > nb / 64) d = nb / 64;
     if (d > nb) d = nb;
     if (d < 1) d = 1;
     return (int)d;

@@ -1,13 +1,17 @@
 // HU -> RLSP and water-equivalent-thickness kernels (gfx950).
 //
-//  k_rlsp_from_hu  fused np.interp HU->RLSP into the padded volume layout
-//  k_rlsp_pad      unpadded -> padded copy (drop-in path: RLSP comes from the host)
+//  k_rlsp_from_hu  fused np.interp HU->RLSP into the quad volume layout
+//  k_rlsp_pad      plain RLSP -> quad layout (drop-in path: RLSP comes from the host)
 //  k_wet_raymarch  per-voxel 1 mm march toward the source, manual trilinear
 //  k_wet_smooth    6-direction lateral mean of the raw WET
 //
-// The padded layout (one zero plane past the high end of every axis) lets the
-// ray-march read all 8 trilinear corners without per-corner bounds tests: a
-// sample with continuous index c in [0, n) has floor(c) + 1 <= n.
+// Quad layout: element (i, j, k), i < nx, j < ny, k <= nz, is the float4
+// (rho(i,j,k), rho(i+1,j,k), rho(i,j+1,k), rho(i+1,j+1,k)) with rho = 0 past
+// the high end of every axis (i = nx, j = ny, k = nz).  A trilinear sample
+// with continuous index c in [0, n) then needs two 16-byte loads (k0, k0 + 1)
+// and no per-corner bounds tests.  On gfx950 the ray-march is bound by the
+// texture-address rate of its gathers (one address per lane per load), so two
+// 16-byte loads per sample instead of four 8-byte ones halve that work.
 #include "dh_common.h"
 
 namespace dh {
@@ -33,33 +37,44 @@ __device__ __forceinline__ double interp_knots(const Knots &kn, double x) {
     return __dadd_rn(__dmul_rn(slope, x - kn.hu[j]), kn.rlsp[j]);
 }
 
-__global__ __launch_bounds__(256) void k_rlsp_from_hu(const float *__restrict__ hu, float *__restrict__ out,
-                                                     Grid g, Knots kn, long long npad) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npad) return;
-    const int px = g.nx + 1, py = g.ny + 1;
-    int i = (int)(t % px);
-    long long r = t / px;
-    int j = (int)(r % py);
-    int k = (int)(r / py);
-    float v = 0.0f;
-    if (i < g.nx && j < g.ny && k < g.nz)
-        v = (float)interp_knots(kn, (double)hu[i + (long long)g.nx * (j + (long long)g.ny * k)]);
-    out[t] = v;
+__device__ __forceinline__ void quad_coords(const Grid &g, long long t, int &i, int &j, int &k) {
+    i = (int)(t % g.nx);
+    const long long r = t / g.nx;
+    j = (int)(r % g.ny);
+    k = (int)(r / g.ny);
 }
 
-__global__ __launch_bounds__(256) void k_rlsp_pad(const float *__restrict__ in, float *__restrict__ out, Grid g,
-                                                 long long npad) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npad) return;
-    const int px = g.nx + 1, py = g.ny + 1;
-    int i = (int)(t % px);
-    long long r = t / px;
-    int j = (int)(r % py);
-    int k = (int)(r / py);
-    float v = 0.0f;
-    if (i < g.nx && j < g.ny && k < g.nz) v = in[i + (long long)g.nx * (j + (long long)g.ny * k)];
-    out[t] = v;
+template <typename F>
+__device__ __forceinline__ float4 make_quad(const Grid &g, int i, int j, int k, F rho) {
+    // rho(i, j, k) for in-range indices, 0 past the high end
+    const bool xi = i + 1 < g.nx, yj = j + 1 < g.ny, zk = k < g.nz;
+    float4 q;
+    q.x = zk ? rho(i, j, k) : 0.0f;
+    q.y = (zk && xi) ? rho(i + 1, j, k) : 0.0f;
+    q.z = (zk && yj) ? rho(i, j + 1, k) : 0.0f;
+    q.w = (zk && xi && yj) ? rho(i + 1, j + 1, k) : 0.0f;
+    return q;
+}
+
+__global__ __launch_bounds__(256) void k_rlsp_from_hu(const float *__restrict__ hu, float4 *__restrict__ out,
+                                                     Grid g, Knots kn, long long nquad) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nquad) return;
+    int i, j, k;
+    quad_coords(g, t, i, j, k);
+    out[t] = make_quad(g, i, j, k, [&](int a, int b, int c) {
+        return (float)interp_knots(kn, (double)hu[a + (long long)g.nx * (b + (long long)g.ny * c)]);
+    });
+}
+
+__global__ __launch_bounds__(256) void k_rlsp_pad(const float *__restrict__ in, float4 *__restrict__ out, Grid g,
+                                                 long long nquad) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nquad) return;
+    int i, j, k;
+    quad_coords(g, t, i, j, k);
+    out[t] = make_quad(g, i, j, k,
+                       [&](int a, int b, int c) { return in[a + (long long)g.nx * (b + (long long)g.ny * c)]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -74,27 +89,22 @@ __global__ __launch_bounds__(256) void k_rlsp_pad(const float *__restrict__ in, 
 // analytically; all but the last couple of samples are provably inside by a
 // margin far above float32 rounding and are read without tests, the tail is
 // tested sample by sample exactly as the per-step mask does.
-struct F2 {
-    float a, b;
-};
-
-__device__ __forceinline__ float trilinear_padded(const float *__restrict__ rho, int pnx, int pnxy, float cx,
-                                                  float cy, float cz) {
-    float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    float ax = cx - fx, ay = cy - fy, az = cz - fz;
-    unsigned base = (unsigned)(int)fx + (unsigned)pnx * ((unsigned)(int)fy) + (unsigned)pnxy * ((unsigned)(int)fz);
-    F2 r00, r10, r01, r11;
-    __builtin_memcpy(&r00, rho + base, sizeof(F2));
-    __builtin_memcpy(&r10, rho + base + pnx, sizeof(F2));
-    __builtin_memcpy(&r01, rho + base + pnxy, sizeof(F2));
-    __builtin_memcpy(&r11, rho + base + pnxy + pnx, sizeof(F2));
-    float x00 = fmaf(ax, r00.b - r00.a, r00.a);
-    float x10 = fmaf(ax, r10.b - r10.a, r10.a);
-    float x01 = fmaf(ax, r01.b - r01.a, r01.a);
-    float x11 = fmaf(ax, r11.b - r11.a, r11.a);
-    float y0 = fmaf(ay, x10 - x00, x00);
-    float y1 = fmaf(ay, x11 - x01, x01);
+__device__ __forceinline__ float blend_quads(float4 q0, float4 q1, float ax, float ay, float az) {
+    const float x00 = fmaf(ax, q0.y - q0.x, q0.x);
+    const float x10 = fmaf(ax, q0.w - q0.z, q0.z);
+    const float x01 = fmaf(ax, q1.y - q1.x, q1.x);
+    const float x11 = fmaf(ax, q1.w - q1.z, q1.z);
+    const float y0 = fmaf(ay, x10 - x00, x00);
+    const float y1 = fmaf(ay, x11 - x01, x01);
     return fmaf(az, y1 - y0, y0);
+}
+
+// Tail samples (plain loads; c inside the grid).
+__device__ __forceinline__ float trilinear_quad(const float4 *__restrict__ qv, int nx, int nxy, float cx, float cy,
+                                                float cz) {
+    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    const long long base = (long long)(int)fx + (long long)nx * (int)fy + (long long)nxy * (int)fz;
+    return blend_quads(qv[base], qv[base + nxy], cx - fx, cy - fy, cz - fz);
 }
 
 // Steps until the ray leaves [0, n) along one axis (real-valued).
@@ -104,34 +114,26 @@ __device__ __forceinline__ float axis_exit(float c0, float du, int n) {
     return 3.0e38f;
 }
 
-// Bulk samples: position by one fma per axis from the voxel-centre index,
-// corners fetched as two-float pairs with buffer loads (32-bit offsets, the
-// per-corner row / slice strides as scalar offsets, out-of-range reads give 0).
-__device__ __forceinline__ float trilinear_buf(__amdgpu_buffer_rsrc_t rs, float cx, float cy, float cz, int pnx,
-                                               int pnxy, int so_y, int so_z, int so_yz) {
+// Bulk samples: position by one fma per axis from the voxel-centre index, the
+// two quads fetched with buffer loads (32-bit offsets, slice stride as a scalar
+// offset; out-of-range reads give 0).
+__device__ __forceinline__ float trilinear_buf(__amdgpu_buffer_rsrc_t rs, float cx, float cy, float cz, int nx,
+                                               int nxy, int so_z) {
     // c >= 0 here, so truncation is floor
     const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
     const float ax = __builtin_amdgcn_fractf(cx), ay = __builtin_amdgcn_fractf(cy), az = __builtin_amdgcn_fractf(cz);
-    const unsigned off = ((unsigned)ix + __umul24((unsigned)iy, (unsigned)pnx) + __umul24((unsigned)iz, (unsigned)pnxy))
-                         << 2;
-    const auto r00 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
-    const auto r10 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, so_y, 0);
-    const auto r01 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, so_z, 0);
-    const auto r11 = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, so_yz, 0);
-    const float a00 = __uint_as_float(r00[0]), b00 = __uint_as_float(r00[1]);
-    const float a10 = __uint_as_float(r10[0]), b10 = __uint_as_float(r10[1]);
-    const float a01 = __uint_as_float(r01[0]), b01 = __uint_as_float(r01[1]);
-    const float a11 = __uint_as_float(r11[0]), b11 = __uint_as_float(r11[1]);
-    const float x00 = fmaf(ax, b00 - a00, a00);
-    const float x10 = fmaf(ax, b10 - a10, a10);
-    const float x01 = fmaf(ax, b01 - a01, a01);
-    const float x11 = fmaf(ax, b11 - a11, a11);
-    const float y0 = fmaf(ay, x10 - x00, x00);
-    const float y1 = fmaf(ay, x11 - x01, x01);
-    return fmaf(az, y1 - y0, y0);
+    const unsigned off = ((unsigned)ix + __umul24((unsigned)iy, (unsigned)nx) + __umul24((unsigned)iz, (unsigned)nxy))
+                         << 4;
+    const auto r0 = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+    const auto r1 = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, so_z, 0);
+    const float4 q0 = make_float4(__uint_as_float(r0[0]), __uint_as_float(r0[1]), __uint_as_float(r0[2]),
+                                  __uint_as_float(r0[3]));
+    const float4 q1 = make_float4(__uint_as_float(r1[0]), __uint_as_float(r1[1]), __uint_as_float(r1[2]),
+                                  __uint_as_float(r1[3]));
+    return blend_quads(q0, q1, ax, ay, az);
 }
 
-__global__ __launch_bounds__(256) void k_wet_raymarch(const float *__restrict__ rho, float *__restrict__ wet, Grid g,
+__global__ __launch_bounds__(256) void k_wet_raymarch(const float4 *__restrict__ qv, float *__restrict__ wet, Grid g,
                                                      Geom b, BrickGrid bg, unsigned nblocks) {
     const unsigned blk = xcd_remap(blockIdx.x, nblocks);
     const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
@@ -157,27 +159,23 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(const float *__restrict__ 
     const int n_safe = max(0, (int)floorf(t_exit) - 1);
     const int n_end = (int)floorf(t_exit) + 3;
 
-    const int pnx = g.nx + 1, pnxy = (g.nx + 1) * (g.ny + 1);
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)rho, (short)0, (int)((unsigned)pnxy * (unsigned)(g.nz + 1) * 4u),
-                                          0x00020000);
-    const int so_y = pnx * 4, so_z = pnxy * 4, so_yz = (pnxy + pnx) * 4;
+    const int nxy = g.nx * g.ny;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)qv, (short)0, (int)((unsigned)nxy * (unsigned)(g.nz + 1) * 16u), 0x00020000);
+    const int so_z = nxy * 16;
     float acc0 = -0.05f, acc1 = 0.0f;
     int n = 0;
     // two samples per iteration: independent loads in flight, two accumulators
     for (; n + 1 < n_safe; n += 2) {
         const float f0 = (float)n, f1 = (float)(n + 1);
-        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), pnx, pnxy,
-                                       so_y, so_z, so_yz);
-        const float v1 = trilinear_buf(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z), pnx, pnxy,
-                                       so_y, so_z, so_yz);
+        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), g.nx, nxy, so_z);
+        const float v1 = trilinear_buf(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z), g.nx, nxy, so_z);
         acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
         acc1 = fmaf(fmaxf(v1, 0.0f), 0.1f, acc1);
     }
     if (n < n_safe) {
         const float f0 = (float)n;
-        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), pnx, pnxy,
-                                       so_y, so_z, so_yz);
+        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), g.nx, nxy, so_z);
         acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
         ++n;
     }
@@ -190,7 +188,7 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(const float *__restrict__ 
         const float cz = (fmaf(uz, fn, v.z) + b.iso[2]) * g.inv_s;
         const bool inside = cx >= 0.0f && cx < (float)g.nx && cy >= 0.0f && cy < (float)g.ny && cz >= 0.0f &&
                             cz < (float)g.nz;
-        if (inside) acc = fmaf(fmaxf(trilinear_padded(rho, pnx, pnxy, cx, cy, cz), 0.0f), 0.1f, acc);
+        if (inside) acc = fmaf(fmaxf(trilinear_quad(qv, g.nx, nxy, cx, cy, cz), 0.0f), 0.1f, acc);
     }
     wet[i + (long long)g.nx * (j + (long long)g.ny * k)] = acc;
 }
@@ -294,9 +292,9 @@ static bool grid_ok(const dh_grid &g) {
         set_error("invalid grid %d x %d x %d, spacing %g", g.nx, g.ny, g.nz, (double)g.spacing);
         return false;
     }
-    // 32-bit byte offsets of the ray-march buffer loads
-    if ((double)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) >= 1073741824.0) {
-        set_error("grid %d x %d x %d too large (padded volume must stay below 2^30 voxels)", g.nx, g.ny, g.nz);
+    // 32-bit byte offsets of the ray-march buffer loads: 16 * nx * ny * (nz + 1) < 2^32
+    if ((double)g.nx * g.ny * (g.nz + 1) >= 268435456.0) {
+        set_error("grid %d x %d x %d too large (nx*ny*(nz+1) must stay below 2^28)", g.nx, g.ny, g.nz);
         return false;
     }
     return true;
@@ -324,10 +322,10 @@ extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, c
         }
     }
     for (int t = n_knots; t < MAX_KNOTS; ++t) kn.hu[t] = kn.rlsp[t] = 0.0;
-    const long long npad = dh_padded_len(g);
-    const unsigned nb = (unsigned)((npad + 255) / 256);
-    hipLaunchKernelGGL(k_rlsp_from_hu, dim3(nb), dim3(256), 0, (hipStream_t)stream, hu, rlsp_padded, make_grid(g),
-                       kn, npad);
+    const long long nquad = dh_padded_len(g) / 4;
+    const unsigned nb = (unsigned)((nquad + 255) / 256);
+    hipLaunchKernelGGL(k_rlsp_from_hu, dim3(nb), dim3(256), 0, (hipStream_t)stream, hu,
+                       reinterpret_cast<float4 *>(rlsp_padded), make_grid(g), kn, nquad);
     return check_launch("k_rlsp_from_hu");
 }
 
@@ -337,9 +335,10 @@ extern "C" int dh_rlsp_pad(const float *rlsp, float *rlsp_padded, dh_grid g, voi
         set_error("dh_rlsp_pad: null pointer");
         return DH_ERR_ARG;
     }
-    const long long npad = dh_padded_len(g);
-    const unsigned nb = (unsigned)((npad + 255) / 256);
-    hipLaunchKernelGGL(k_rlsp_pad, dim3(nb), dim3(256), 0, (hipStream_t)stream, rlsp, rlsp_padded, make_grid(g), npad);
+    const long long nquad = dh_padded_len(g) / 4;
+    const unsigned nb = (unsigned)((nquad + 255) / 256);
+    hipLaunchKernelGGL(k_rlsp_pad, dim3(nb), dim3(256), 0, (hipStream_t)stream, rlsp,
+                       reinterpret_cast<float4 *>(rlsp_padded), make_grid(g), nquad);
     return check_launch("k_rlsp_pad");
 }
 
@@ -353,8 +352,12 @@ extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid
     const Grid gg = make_grid(g);
     const BrickGrid bg = brick_grid(gg);
     const unsigned nb = (unsigned)((bg.count() + 3) / 4);
-    hipLaunchKernelGGL(k_wet_raymarch, dim3(nb), dim3(256), 0, (hipStream_t)stream, rlsp_padded, wet_raw, gg,
-                       make_geom(*geom), bg, nb);
+    if (((uintptr_t)rlsp_padded & 15u) != 0) {
+        set_error("dh_wet_raymarch: the quad RLSP volume must be 16-byte aligned");
+        return DH_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_wet_raymarch, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float4 *>(rlsp_padded), wet_raw, gg, make_geom(*geom), bg, nb);
     return check_launch("k_wet_raymarch");
 }
 

@@ -14,8 +14,10 @@
  *    (a hipStream_t passed as void*, NULL = default stream).
  *  - Volumes are float32, indexed v = i + nx*(j + ny*k) with i along x
  *    (numpy axis 2 of a (z, y, x) array), as in CudaClasses.cuh:189-191.
- *  - Padded RLSP volumes have one extra zero plane at the high end of every
- *    axis: (nx+1)*(ny+1)*(nz+1) floats, index i + (nx+1)*(j + (ny+1)*k).
+ *  - The ray-march reads RLSP in a "quad" layout: element (i, j, k) with
+ *    i < nx, j < ny, k <= nz, index i + nx*(j + ny*k), is the float4
+ *    (rho(i,j,k), rho(i+1,j,k), rho(i,j+1,k), rho(i+1,j+1,k)), rho = 0 past
+ *    the high end of each axis; 4*nx*ny*(nz+1) floats, 16-byte aligned.
  *  - Return value 0 = success; negative = error, message via dh_last_error().
  *  - Entry points are re-entrant; per-thread error text.
  */
@@ -104,11 +106,11 @@ int dh_version(void);
  * truncated to len).  Returns the length of the full message. */
 int dh_last_error(char *buf, size_t len);
 
-/* Number of floats of a padded RLSP volume: (nx+1)*(ny+1)*(nz+1). */
+/* Number of floats of a quad RLSP volume: 4*nx*ny*(nz+1). */
 int64_t dh_padded_len(dh_grid g);
 
 /* HU -> RLSP, piecewise linear through (hu_knots, rlsp_knots), clamped at both
- * ends (np.interp, plan_impt.py:158-168), written as a padded volume.
+ * ends (np.interp, plan_impt.py:158-168), written in the quad layout.
  * hu: [nz*ny*nx] device; knots: HOST float64 arrays (as parsed from
  * HU_RLSP.csv), strictly ascending hu, 2 <= n_knots <= 32.  The interpolation
  * runs in float64 and is rounded to float32, like the reference's
@@ -117,7 +119,7 @@ int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g,
                     const double *hu_knots, const double *rlsp_knots, int32_t n_knots,
                     void *stream);
 
-/* Copy an unpadded RLSP volume into the padded layout. */
+/* Copy a plain RLSP volume (nz*ny*nx floats) into the quad layout. */
 int dh_rlsp_pad(const float *rlsp, float *rlsp_padded, dh_grid g, void *stream);
 
 /* Raw water-equivalent thickness (cm) of every voxel: 1 mm steps from the voxel

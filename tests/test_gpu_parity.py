@@ -89,9 +89,12 @@ def test_rlsp_from_hu_matches_np_interp(dev):
     ref = np.array(np.interp(hu, hk, rk), dtype=np.single)
     grid = D.make_grid(hu.shape, 2.0)
     out = D.rlsp_from_hu(torch.from_numpy(hu).to(dev), grid, hk, rk, dev).cpu().numpy()
-    out = out.reshape(hu.shape[0] + 1, hu.shape[1] + 1, hu.shape[2] + 1)
-    assert np.all(out[-1] == 0) and np.all(out[:, -1] == 0) and np.all(out[:, :, -1] == 0)
-    np.testing.assert_array_max_ulp(out[:-1, :-1, :-1], ref, maxulp=1)
+    nz, ny, nx = hu.shape
+    q = out.reshape(nz + 1, ny, nx, 4)
+    pad = np.zeros((nz + 1, ny + 1, nx + 1), np.float32)
+    pad[:nz, :ny, :nx] = ref
+    expect = np.stack([pad[:, :ny, :nx], pad[:, :ny, 1:], pad[:, 1:, :nx], pad[:, 1:, 1:]], axis=-1)
+    np.testing.assert_array_max_ulp(q, expect, maxulp=1)
 
 
 def test_water_slab_end_to_end(dev, machine_dir):

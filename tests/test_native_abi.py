@@ -66,7 +66,7 @@ def test_argument_errors_without_gpu():
     with pytest.raises(ValueError):
         N.check(rc, "dh_wet_raymarch")
     good = N.dh_grid(8, 8, 8, 2.0)
-    assert lib.dh_padded_len(good) == 9 * 9 * 9
+    assert lib.dh_padded_len(good) == 4 * 8 * 8 * 9
     huge = N.dh_grid(2000, 2000, 2000, 1.0)
     assert lib.dh_rlsp_pad(ctypes.c_void_p(16), ctypes.c_void_p(16), huge, None) == N.DH_ERR_ARG
     knots = (ctypes.c_double * 2)(0.0, 0.0)
