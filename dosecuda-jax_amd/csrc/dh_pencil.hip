@@ -466,7 +466,7 @@ struct LdsVox {
     float GBb, pad0, pad1, pad2;
 };
 
-template <bool CULL, bool COUNT = false>
+template <bool CULL, int CS, bool COUNT = false>
 __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wetv, const float *__restrict__ gdose,
                                                   Grid g, Geom b, BrickGrid bg, Model m,
                                                   const LayerC *__restrict__ lc, int n_layers,
@@ -475,13 +475,15 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
                                                   float4 *__restrict__ partial,
                                                   unsigned long long *__restrict__ counts = nullptr) {
     __shared__ LdsVox s_vox[WAVE];
+    constexpr int G = WAVE / CS;  // voxel subsets per wave: lane = group * CS + slot
     const int lane = threadIdx.x;
+    const int slot = lane % CS, group = lane / CS;
     // chunk -> (layer, first spot of the chunk within the layer)
     int cid = blockIdx.x, l = 0, c0 = -1;
     for (; l < n_layers; ++l) {
-        const int nch = (lc[l].n + WAVE - 1) / WAVE;
+        const int nch = (lc[l].n + CS - 1) / CS;
         if (cid < nch) {
-            c0 = cid * WAVE;
+            c0 = cid * CS;
             break;
         }
         cid -= nch;
@@ -490,14 +492,14 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
     const LayerC L = lc[l];
     if (!L.ok) return;
     const int slab = blockIdx.y;
-    const bool sval = c0 + lane < L.n;
-    const int spot = L.start + c0 + lane;
+    const bool sval = c0 + slot < L.n;
+    const int spot = L.start + c0 + slot;
     SpotExt q;
     if (sval) q = sp[spot];
     else q = SpotExt{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f};
     const float itt = 1.0f / (q.tx * q.tx + q.ty * q.ty + 1.0f);
     const float ivx = 1.0f / b.vsadx, ivy = 1.0f / b.vsady;
-    const unsigned long long chunk_n = (unsigned long long)min(WAVE, L.n - c0);
+    const unsigned long long chunk_n = (unsigned long long)min(CS, L.n - c0);
     LayerC R = L;  // reach of this chunk's spots only
     if (CULL) reach_of(sp, L.start + c0, (int)chunk_n, m, L.eid, lane, R);
 
@@ -548,7 +550,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
                 rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
             }
             if (COUNT) {
-                n_exec += (unsigned long long)__popcll(am) * (unsigned)__popcll(__ballot(rel));
+                n_exec += (unsigned long long)__popcll(am) * (unsigned)__popcll(__ballot(rel && group == 0));
                 continue;
             }
             if (!__ballot(rel)) continue;
@@ -571,7 +573,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
             __syncthreads();
             const int na = __popcll(am);
             if (rel) {
-                for (int t = 0; t < na; ++t) {
+                for (int t = group; t < na; t += G) {
                     const LdsVox e = s_vox[t];
                     // in-plane offset e_xy, its part perpendicular to the axis p,
                     // q_e = (e.t)/|t|^2; d2 = e_xy . p (no cancellation)
@@ -601,7 +603,14 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
         }
         return;
     }
-    if (sval) partial[(long long)slab * n_spots + spot] = make_float4(K, FX, FY, 0.0f);
+    // fold the voxel subsets (fixed order: deterministic)
+#pragma unroll
+    for (int off = CS; off < WAVE; off <<= 1) {
+        K += __shfl_xor(K, off);
+        FX += __shfl_xor(FX, off);
+        FY += __shfl_xor(FY, off);
+    }
+    if (sval && group == 0) partial[(long long)slab * n_spots + spot] = make_float4(K, FX, FY, 0.0f);
 }
 
 // Spots that no chunk covers keep the zeroed partials (k_zero4).
@@ -610,22 +619,37 @@ __global__ __launch_bounds__(256) void k_zero4(float4 *__restrict__ p, long long
     if (t < n) p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// 256 threads = 64 spots x 4 slab groups; each group sums every 4th slab,
+// then the 4 group sums are added in a fixed order.
 __global__ __launch_bounds__(256) void k_pb_reduce(const float4 *__restrict__ partial, const SpotExt *__restrict__ sp,
                                                   int n_spots, int nslab, float *__restrict__ grad_mu,
                                                   float *__restrict__ grad_xy) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_spots) return;
-    float K = 0.f, FX = 0.f, FY = 0.f;
-    for (int d = 0; d < nslab; ++d) {
-        const float4 p = partial[(long long)d * n_spots + s];
-        K += p.x;
-        FX += p.y;
-        FY += p.z;
+    __shared__ float4 s_part[4][64];
+    const int sl = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int s = blockIdx.x * 64 + sl;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (s < n_spots) {
+        for (int d = grp; d < nslab; d += 4) {
+            const float4 p = partial[(long long)d * n_spots + s];
+            acc.x += p.x;
+            acc.y += p.y;
+            acc.z += p.z;
+        }
     }
-    const float mu2 = -2.0f * sp[s].mu;
-    grad_mu[s] = K;
-    grad_xy[2 * s] = mu2 * FX;
-    grad_xy[2 * s + 1] = mu2 * FY;
+    s_part[grp][sl] = acc;
+    __syncthreads();
+    if (grp == 0 && s < n_spots) {
+        float K = 0.f, FX = 0.f, FY = 0.f;
+        for (int q = 0; q < 4; ++q) {
+            K += s_part[q][sl].x;
+            FX += s_part[q][sl].y;
+            FY += s_part[q][sl].z;
+        }
+        const float mu2 = -2.0f * sp[s].mu;
+        grad_mu[s] = K;
+        grad_xy[2 * s] = mu2 * FX;
+        grad_xy[2 * s + 1] = mu2 * FY;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -635,9 +659,21 @@ static long long align_up(long long x) { return (x + kAlign - 1) / kAlign * kAli
 // Slabs of bricks per spot chunk: enough (chunk, slab) workgroups (~32k) that
 // the few heavy ones (chunks and slabs in the beam path) do not leave the chip
 // idle at the end, with at least 64 bricks per slab.
+// Spots per adjoint workgroup: 64, or 32 (with two voxel subsets per wave) when
+// the average layer is small enough that 64-spot chunks would leave many lanes
+// idle.  Scores: lane use / (1 + per-brick overhead share), the overhead share
+// growing as chunks shrink.
+static int adjoint_chunk(int n_spots, int n_layers) {
+    if (n_layers <= 0) return 64;
+    const double n = (double)n_spots / n_layers;
+    const double u64 = n / (ceil(n / 64.0) * 64.0), u32 = n / (ceil(n / 32.0) * 32.0);
+    return (u32 / 1.38 > u64 / 1.23) ? 32 : 64;
+}
+
 static int adjoint_slabs(const dh_grid &g, int n_spots, int n_layers) {
     const long long nb = (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
-    const long long nch = ((long long)n_spots + 63LL * n_layers) / 64 + 1;
+    const int cs = adjoint_chunk(n_spots, n_layers);
+    const long long nch = ((long long)n_spots + (cs - 1LL) * n_layers) / cs + 1;
     long long d = (32768 + nch - 1) / nch;
     if (d > 1024) d = 1024;
     if (d > nb / 64) d = nb / 64;
@@ -760,15 +796,28 @@ static int launch_adjoint(hipStream_t st, const float *wet, const float *grad_do
                        gg, bg, w.bsum);
     int rc = check_launch("k_brick_summary");
     if (rc) return rc;
-    const unsigned nch = (unsigned)(((long long)S + 63LL * spots->n_layers) / 64 + 1);
-    if (cutoff_sigma > 0.0f)
-        hipLaunchKernelGGL((k_pb_adjoint<true, COUNT>), dim3(nch, w.nslab), dim3(64), 0, st, wet, grad_dose, gg, gb,
-                           bg, m, w.layer, spots->n_layers, w.ext, S, cutoff_sigma, brick_radius(g), w.nslab, w.bsum,
-                           w.partial, counts);
-    else
-        hipLaunchKernelGGL((k_pb_adjoint<false, COUNT>), dim3(nch, w.nslab), dim3(64), 0, st, wet, grad_dose, gg, gb,
-                           bg, m, w.layer, spots->n_layers, w.ext, S, 0.0f, brick_radius(g), w.nslab, w.bsum,
-                           w.partial, counts);
+    const int cs = adjoint_chunk(S, spots->n_layers);
+    const unsigned nch = (unsigned)(((long long)S + (cs - 1LL) * spots->n_layers) / cs + 1);
+    const dim3 grid(nch, w.nslab);
+    const float rb = brick_radius(g);
+    const bool cull = cutoff_sigma > 0.0f;
+    if (cs == 32) {
+        if (cull)
+            hipLaunchKernelGGL((k_pb_adjoint<true, 32, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
+                               w.layer, spots->n_layers, w.ext, S, cutoff_sigma, rb, w.nslab, w.bsum, w.partial,
+                               counts);
+        else
+            hipLaunchKernelGGL((k_pb_adjoint<false, 32, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
+                               w.layer, spots->n_layers, w.ext, S, 0.0f, rb, w.nslab, w.bsum, w.partial, counts);
+    } else {
+        if (cull)
+            hipLaunchKernelGGL((k_pb_adjoint<true, 64, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
+                               w.layer, spots->n_layers, w.ext, S, cutoff_sigma, rb, w.nslab, w.bsum, w.partial,
+                               counts);
+        else
+            hipLaunchKernelGGL((k_pb_adjoint<false, 64, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
+                               w.layer, spots->n_layers, w.ext, S, 0.0f, rb, w.nslab, w.bsum, w.partial, counts);
+    }
     return check_launch(COUNT ? "k_pb_adjoint(count)" : "k_pb_adjoint");
 }
 
@@ -823,7 +872,7 @@ extern "C" int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g
     if (rc) return rc;
     rc = launch_adjoint<false>(st, wet, grad_dose, g, geom, m, spots, w, cutoff_sigma, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pb_reduce, dim3((S + 255) / 256), dim3(256), 0, st, w.partial, w.ext, S, w.nslab, grad_mu,
+    hipLaunchKernelGGL(k_pb_reduce, dim3((S + 63) / 64), dim3(256), 0, st, w.partial, w.ext, S, w.nslab, grad_mu,
                        grad_xy);
     return check_launch("k_pb_reduce");
 }

@@ -190,3 +190,109 @@ def test_forward_deterministic(dev):
     a1 = eng.adjoint(gd)[0]
     a2 = eng.adjoint(gd)[0]
     assert torch.equal(a1[0], a2[0]) and torch.equal(a1[1], a2[1])
+
+
+def _small_plan(gantries, couches, layers, radius=15.0, pitch=3.0, seed=5):
+    from DoseCUDA import IMPTPlan
+    from DoseCUDA.workloads import _beam, grid_spots
+    rng = np.random.default_rng(seed)
+    plan = IMPTPlan()
+    c = np.arange(-radius, radius + 1e-6, pitch)
+    for ga, ca in zip(gantries, couches):
+        sl = grid_spots(layers, c, c, disk_radius=radius)
+        sl[:, 2] = rng.uniform(0.2, 1.0, size=sl.shape[0]).astype(np.float32)
+        plan.addBeam(_beam(ga, sl, couch=ca))
+    return plan
+
+
+def test_oblique_multibeam_end_to_end(dev, machine_dir):
+    """Heterogeneous phantom, beams at gantry 120 / couch 30 and gantry 240,
+    81-spot layers: drop-in computeIMPTPlan vs the oracle plan."""
+    _say("oblique multi-beam end to end")
+    g = _phantom_small(shape=(40, 56, 48), spacing=2.5, seed=3)
+    plan = _small_plan([120.0, 240.0], [30.0, 0.0], [22, 27, 31])
+    plan.n_fractions = 2
+    g.computeIMPTPlan(plan)
+    ref, ref_beams = O.compute_plan(g, plan, machine_dir)
+    m = O.compare(g.dose, ref)
+    gam = O.gamma_pass_rate(g.dose, ref, 2.5)
+    _say("max|dD|/Dmax %.3e gamma %.4f" % (m["max_rel_to_dmax"], gam))
+    assert m["max_rel_to_dmax"] <= 1e-3 and gam == 1.0
+    for got, want in zip(g.beam_doses, ref_beams):
+        assert O.compare(got, want)["max_rel_to_dmax"] <= 1e-3
+
+
+def test_adjoint_81_spot_layers(dev):
+    """Layers of 81 spots take the 32-spot / two-voxel-subset adjoint path."""
+    from DoseCUDA import device as D
+    _say("adjoint, 81-spot layers, oblique beam")
+    g = _phantom_small(shape=(40, 56, 48), spacing=2.5, seed=4)
+    plan = _small_plan([120.0], [30.0], [22, 27, 31, 35])
+    beam = plan.beam_list[0]
+    model = plan.beam_models[0]
+    rlsp = g.RLSPFromHU(plan.machine_name)
+    s = float(np.float32(g.spacing[0]))
+    geom = _oracle_geom(g, beam, model)
+    wet_o = O.wet(rlsp, s, geom).astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(2)
+    gvol = rng.normal(size=g.HU.shape).astype(np.float32)
+    gvol[rng.uniform(size=g.HU.shape) < 0.5] = 0.0
+    D_, model, grid, geo, dmodel, dspots, ws, nb, wet = _pb_inputs(g, plan, dev, wet_o)
+    assert dspots.n_spots == 4 * 81
+    gmu, gxy = D.pb_adjoint(wet, torch.from_numpy(gvol).to(dev), grid, geo, dmodel, dspots, ws, nb, cutoff_sigma=6.0)
+    gmu = dspots.to_input_order(gmu).cpu().numpy()
+    gxy = dspots.to_input_order(gxy).cpu().numpy()
+    order, spots, layers, r80 = O.spots_and_layers(beam.spot_list, model.divergence_params)
+    rmu, rxy = O.pb_grad(wet_o, gvol, s, geom, spots, layers, r80, O.ModelArrays(model))
+    ref_mu = np.empty_like(rmu)
+    ref_mu[order] = rmu
+    ref_xy = np.empty_like(rxy)
+    ref_xy[order] = rxy
+    e_mu = np.abs(gmu - ref_mu).max() / np.abs(ref_mu).max()
+    e_xy = np.abs(gxy - ref_xy).max() / np.abs(ref_xy).max()
+    _say("rel err mu %.3e xy %.3e" % (e_mu, e_xy))
+    assert e_mu <= 1e-4 and e_xy <= 1e-4
+
+
+class _NonUniformModel:
+    """A beam model whose depth grid is not uniform (exercises the bisection
+    lookup): every energy's table resampled at depths k^1.5 / 20^0.5 mm."""
+
+    def __init__(self, base):
+        k = np.arange(base.lut_depths.shape[1], dtype=np.float64)
+        dep = (k ** 1.5 / np.sqrt(20.0)).astype(np.float32)
+        self.VSADX, self.VSADY = base.VSADX, base.VSADY
+        self.divergence_params = base.divergence_params
+        self.lut_depths = np.tile(dep, (base.lut_depths.shape[0], 1))
+        self.lut_sigmas = np.stack([np.interp(dep, base.lut_depths[e], base.lut_sigmas[e])
+                                    for e in range(base.lut_depths.shape[0])]).astype(np.float32)
+        self.lut_idds = np.stack([np.interp(dep, base.lut_depths[e], base.lut_idds[e])
+                                  for e in range(base.lut_depths.shape[0])]).astype(np.float32)
+
+
+def test_nonuniform_lut_depths(dev):
+    from DoseCUDA import device as D
+    from DoseCUDA.workloads import water_slab_plan
+    _say("non-uniform LUT depth grid")
+    g, plan = water_slab_plan(n=48, spacing=2.5)
+    beam = plan.beam_list[0]
+    model = _NonUniformModel(plan.beam_models[0])
+    rlsp = g.RLSPFromHU(plan.machine_name)
+    s = float(np.float32(g.spacing[0]))
+    geom = _oracle_geom(g, beam, model)
+    wet_o = O.wet(rlsp, s, geom).astype(np.float32).astype(np.float64)
+    dmodel = D.DeviceBeamModel(model, dev)
+    assert not dmodel.uniform
+    grid = D.make_grid(g.HU.shape, s)
+    geo = D.make_geom(D.beam_geometry(np.asarray(beam.iso, np.float32), np.asarray(g.origin, np.float32),
+                                      beam.gantry_angle, beam.couch_angle, model.VSADX, model.VSADY))
+    dspots = D.DeviceSpots(beam.spot_list, model.divergence_params, dev)
+    ws, nb = D.alloc_workspace(dspots.n_spots, dspots.n_layers, grid, dev)
+    wet = torch.from_numpy(wet_o.astype(np.float32)).to(dev)
+    dose = torch.empty_like(wet)
+    D.pb_forward(wet, dose, grid, geo, dmodel, dspots, ws, nb, cutoff_sigma=6.0)
+    _, spots, layers, r80 = O.spots_and_layers(beam.spot_list, model.divergence_params)
+    ref = O.pb_dose(wet_o, s, geom, spots, layers, r80, O.ModelArrays(model))
+    m = O.compare(dose.cpu().numpy(), ref)
+    _say("max|dD|/Dmax %.3e" % m["max_rel_to_dmax"])
+    assert m["max_rel_to_dmax"] <= 2e-5
