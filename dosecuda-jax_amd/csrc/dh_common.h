@@ -105,6 +105,128 @@ __device__ __forceinline__ float3 head_to_image(const Geom &b, float3 h) {
 }
 
 // ---------------------------------------------------------------------------
+// Shared records (device memory)
+
+// Spot record with the constants of its beam-axis geometry.  With
+// t = (tx, ty, -1) the spot axis direction, a head-frame point h sits at
+// in-plane offset e = (hx - x + hz tx, hy - y + hz ty) from where the axis
+// crosses the plane z = hz, and its squared distance to the axis is the
+// quadratic form  d2 = m11 e_x^2 + 2 m12 e_x e_y + m22 e_y^2  of
+// M = I - t_xy t_xy^T / |t|^2.  Unlike |r|^2 - (t.r)^2/|t|^2 this has no
+// cancellation far from the isocentre plane (|r| ~ 150 mm vs d ~ 1 mm).
+struct SpotExt {
+    float x, y, tx, ty, mu, m11, m12x2, m22;
+};
+
+// Beam-model tables on the device.
+struct Model {
+    const float2 *idd_sigma;
+    const float *depths;
+    const float *dvp;
+    int n_energies, lut_len, dvp_len, uniform;
+    float depth0, inv_ddepth;
+};
+
+// Layer record (64 bytes): the layer's spot range and energy, the constants of
+// its sigma_air polynomial, and the reach of its spots: bounding box of the
+// positions at the isocentre plane, the smallest eigenvalue 1/(1 + |t_xy|^2)
+// of their distance forms and max |sigma_ms| of the energy's lookup row.
+struct LayerC {
+    int start, n, eid, ok;
+    float r80, A, B, C;
+    float xmin, xmax, ymin, ymax;
+    float lam_min, sms_max, lim, pad;
+};
+
+// Smoothing neighbour stencil of a beam: per (direction, dr) entry, direction
+// major, dr = 1..9 mm: integer base offsets, tie bits (1 = x, 2 = y, 4 = z).
+constexpr int N_STENCIL = 54;
+struct Stencil {
+    signed char dx[N_STENCIL], dy[N_STENCIL], dz[N_STENCIL];
+    unsigned char tie[N_STENCIL], dr[N_STENCIL];
+};
+
+// Everything the kernels of one beam read or write.  Passed by value to the
+// single-beam kernels and read from a device table by the batched ones.
+struct BeamCtx {
+    const float4 *rlsp_q;   // quad RLSP of the beam's plan (ray-march input)
+    float *wet_raw;         // ray-march output
+    float *wet;             // smoothed WET (cm): smoothing output, dose input
+    float *dose;            // dose output of the beam's plan
+    const float *gdose;     // dL/dD of the beam's plan (adjoint input)
+    const dh_spot *spots_in;
+    const dh_layer *layers_in;
+    SpotExt *sp;            // prepared spot records
+    LayerC *lc;             // prepared layer records
+    float4 *partial;        // adjoint partials [nslab][n_spots]
+    float2 *bsum;           // per-brick WET range of voxels with dL/dD != 0
+    float *grad_mu, *grad_xy;
+    const Stencil *stencil; // smoothing stencil (batched smoothing)
+    Model m;
+    Grid g;
+    Geom b;
+    BrickGrid bg;
+    int n_spots, n_layers, nslab, cs;
+    float cut, rbrick;
+    int pad0, pad1;
+};
+
+// A plan of a batch: its dose volume and its beams [beam0, beam0 + n_beams) of
+// the beam-context table.
+struct PlanCtx {
+    float *dose;
+    long long nvox;
+    int beam0, n_beams;
+};
+
+// Adjoint schedule: spots per workgroup (cs: 32, 64, or 0 = chosen from the
+// average layer size) and how bricks are cut into slabs (about wg_target
+// workgroups per beam, at least min_bricks bricks per slab, at most max_slabs
+// slabs).
+struct AdjTune {
+    int cs, wg_target, min_bricks, max_slabs;
+};
+
+// Pencil-beam workspace of one beam (device memory).
+struct BeamWs {
+    SpotExt *ext;
+    LayerC *layer;
+    float4 *partial;
+    float2 *bsum;
+    int nslab, cs;
+    long long bytes;
+};
+
+// ---------------------------------------------------------------------------
+// host helpers shared by the single-beam and batched entry points
+// dh_wet.hip
+void make_stencil(const dh_beam_geom &b, double s, Stencil &st);
+bool grid_ok(const dh_grid &g);
+int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
+                     long long total_bricks, long long total_blocks, hipStream_t st);
+// dh_pencil.hip
+int adjoint_chunk(int n_spots, int n_layers);
+int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t);
+AdjTune default_tune();
+bool tune_ok(const AdjTune &t);
+BeamWs carve_beam(void *base, long long nbricks, int S, int L, const AdjTune &t);
+bool fill_model(const dh_beam_model *bm, Model &m);
+bool pencil_grid_ok(const dh_grid &g);
+bool spot_set_ok(const dh_spot_set *ss);
+float brick_radius(float spacing);
+void fill_pencil_ctx(BeamCtx &c, const dh_grid &g, const dh_beam_geom &geom, const Model &m, const dh_spot_set &ss,
+                     const BeamWs &w, float cutoff_sigma);
+int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long long *layer_off, int n_beams,
+                      long long total_spots, long long total_layers, hipStream_t st);
+int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
+                         const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
+                         bool cull, hipStream_t st);
+int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
+                         const long long *part_off, long long total_part, const long long *item_off32,
+                         long long total_items32, const long long *item_off64, long long total_items64,
+                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st);
+
+// ---------------------------------------------------------------------------
 // wave64 helpers
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -113,5 +235,49 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// Index of this thread's wave in its workgroup, as a wave-uniform value (the
+// compiler cannot tell that threadIdx.x / 64 is uniform; readfirstlane lets
+// everything derived from it live in scalar registers and load through the
+// scalar cache).
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// Pointers read from device memory (the batched kernels' context table) are
+// generic pointers to the compiler, which then uses flat loads with 64-bit
+// per-lane addresses and no scalar loads.  Every pointer in a context points to
+// global memory: forming the address in the global address space
+// (address_space(1)) and converting only the final element address back lets
+// address-space inference turn the access into a global (or, when uniform,
+// scalar) load.  A bare round trip p -> global -> generic would be folded away,
+// hence base + offset inside the global space.
+#define DH_GLOBAL __attribute__((address_space(1)))
+
+template <class T>
+__device__ __forceinline__ T *gptr(T *p, long long off) {
+    return (T *)((DH_GLOBAL T *)p + off);
+}
+
+template <class T>
+__device__ __forceinline__ T gld(const T *p, long long i) {
+    return *gptr(p, i);
+}
+
+template <class T>
+__device__ __forceinline__ void gst(T *p, long long i, const T &v) {
+    *gptr(p, i) = v;
+}
+
+__device__ __forceinline__ const BeamCtx &load_ctx(const BeamCtx *__restrict__ ctx, int b) { return ctx[b]; }
+
+// Owner of an item in a table of ascending start offsets off[0..n) with
+// off[0] = 0: the largest b with off[b] <= item.
+__device__ __forceinline__ int find_owner(const long long *__restrict__ off, int n, long long item) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= item) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 
 }  // namespace dh

@@ -8,6 +8,10 @@
 //                   broadcast from LDS
 //  k_pb_reduce      fixed-order sum of the adjoint partials
 //
+// Each kernel has a batched twin (*_batch) that walks the items of many beams
+// (and plans) in one launch, reading per-beam contexts from a device table;
+// both run the same per-item device function, so their results are identical.
+//
 // Per (voxel, layer) terms follow pencilBeamKernel (IMPTClasses.cu:204-271) and
 // _pencil_beam_single_layer (impt_jax.py:425-541):
 //   wet  = 10 * WET(cm), skip when wet > 1.1 R80
@@ -30,37 +34,9 @@
 // bricks) at once and the wave then walks only the survivors.
 #include "dh_common.h"
 
+#include <math.h>
+
 namespace dh {
-
-// Spot record with the constants of its beam-axis geometry.  With
-// t = (tx, ty, -1) the spot axis direction, a head-frame point h sits at
-// in-plane offset e = (hx - x + hz tx, hy - y + hz ty) from where the axis
-// crosses the plane z = hz, and its squared distance to the axis is the
-// quadratic form  d2 = m11 e_x^2 + 2 m12 e_x e_y + m22 e_y^2  of
-// M = I - t_xy t_xy^T / |t|^2.  Unlike |r|^2 - (t.r)^2/|t|^2 this has no
-// cancellation far from the isocentre plane (|r| ~ 150 mm vs d ~ 1 mm).
-struct SpotExt {
-    float x, y, tx, ty, mu, m11, m12x2, m22;
-};
-
-struct Model {
-    const float2 *idd_sigma;
-    const float *depths;
-    const float *dvp;
-    int n_energies, lut_len, dvp_len, uniform;
-    float depth0, inv_ddepth;
-};
-
-// Layer record (64 bytes): the layer's spot range and energy, the constants of
-// its sigma_air polynomial, and the reach of its spots: bounding box of the
-// positions at the isocentre plane, the smallest eigenvalue 1/(1 + |t_xy|^2)
-// of their distance forms and max |sigma_ms| of the energy's lookup row.
-struct LayerC {
-    int start, n, eid, ok;
-    float r80, A, B, C;
-    float xmin, xmax, ymin, ymax;
-    float lam_min, sms_max, lim, pad;
-};
 
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kTwoPi = 6.283185307179586f;
@@ -76,38 +52,19 @@ __device__ __forceinline__ float wave_min(float v) {
     return v;
 }
 
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_spot_prep(const dh_spot *__restrict__ in, SpotExt *__restrict__ out, int n,
-                                                  float inv_vsadx, float inv_vsady) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    dh_spot p = in[s];
-    SpotExt e;
-    e.x = p.x;
-    e.y = p.y;
-    e.mu = p.mu;
-    e.tx = p.x * inv_vsadx;
-    e.ty = p.y * inv_vsady;
-    const float itt = 1.0f / (e.tx * e.tx + e.ty * e.ty + 1.0f);
-    e.m11 = 1.0f - e.tx * e.tx * itt;
-    e.m22 = 1.0f - e.ty * e.ty * itt;
-    e.m12x2 = -2.0f * e.tx * e.ty * itt;
-    out[s] = e;
-}
-
 // Reach of spots [start, start + n) (lane-parallel) into c.
 __device__ __forceinline__ void reach_of(const SpotExt *__restrict__ sp, int start, int n, const Model &m, int eid,
                                          int lane, LayerC &c) {
     float xmin = 3.0e38f, xmax = -3.0e38f, ymin = 3.0e38f, ymax = -3.0e38f, t2max = 0.0f, smax = 0.0f;
     for (int s = lane; s < n; s += WAVE) {
-        const SpotExt q = sp[start + s];
+        const SpotExt q = gld(sp, start + s);
         xmin = fminf(xmin, q.x);
         xmax = fmaxf(xmax, q.x);
         ymin = fminf(ymin, q.y);
         ymax = fmaxf(ymax, q.y);
         t2max = fmaxf(t2max, q.tx * q.tx + q.ty * q.ty);
     }
-    const float2 *row = m.idd_sigma + (long long)eid * m.lut_len;
+    const float2 *row = gptr(m.idd_sigma, (long long)eid * m.lut_len);
     for (int t = lane; t < m.lut_len; t += WAVE) smax = fmaxf(smax, fabsf(row[t].y));
     c.xmin = wave_min(xmin);
     c.xmax = wave_max(xmax);
@@ -117,42 +74,10 @@ __device__ __forceinline__ void reach_of(const SpotExt *__restrict__ sp, int sta
     c.sms_max = wave_max(smax);
 }
 
-// One wave per layer.  A record the kernels cannot use safely (energy id
-// outside the tables, spot range outside the spot array) gets ok = 0 and is
-// skipped rather than read out of bounds.
-__global__ __launch_bounds__(64) void k_layer_prep(const SpotExt *__restrict__ sp, int n_spots,
-                                                  const dh_layer *__restrict__ layers, Model m,
-                                                  LayerC *__restrict__ out) {
-    const int l = blockIdx.x, lane = threadIdx.x;
-    const dh_layer L = layers[l];
-    LayerC c;
-    c.ok = (unsigned)L.energy_id < (unsigned)m.n_energies && L.spot_start >= 0 && L.n_spots > 0 &&
-           L.spot_start <= n_spots - L.n_spots;
-    c.start = c.ok ? L.spot_start : 0;
-    c.n = c.ok ? L.n_spots : 0;
-    c.eid = c.ok ? L.energy_id : 0;
-    c.r80 = L.r80;
-    c.lim = 1.1f * L.r80;
-    const float *d = m.dvp + (long long)c.eid * m.dvp_len;
-    c.A = d[2];
-    c.B = d[3];
-    c.C = d[4];
-    c.pad = 0.0f;
-    if (c.ok) {
-        reach_of(sp, c.start, c.n, m, c.eid, lane, c);
-    } else {
-        c.xmin = c.ymin = -3.0e38f;
-        c.xmax = c.ymax = 3.0e38f;
-        c.lam_min = 0.0f;
-        c.sms_max = 0.0f;
-    }
-    if (lane == 0) out[l] = c;
-}
-
 // LUT lookup (interpolateProtonLUT, IMPTClasses.cu:69-92): i = first depth >= wet;
 // i == 0 -> row 0, i == len -> last row, else lerp between i-1 and i.
 __device__ __forceinline__ void lut_lookup(const Model &m, int eid, float wet, float &idd, float &sig) {
-    const float2 *row = m.idd_sigma + (long long)eid * m.lut_len;
+    const float2 *row = gptr(m.idd_sigma, (long long)eid * m.lut_len);
     const int L = m.lut_len;
     int lo;
     float f;
@@ -173,7 +98,7 @@ __device__ __forceinline__ void lut_lookup(const Model &m, int eid, float wet, f
         lo = min((int)q, L - 2);
         f = q - (float)lo;
     } else {
-        const float *dep = m.depths + (long long)eid * L;
+        const float *dep = gptr(m.depths, (long long)eid * L);
         int l = 0, r = L;
         while (l < r) {
             const int mid = (l + r) >> 1;
@@ -299,6 +224,81 @@ __device__ __forceinline__ void brick_of(const BrickGrid &bg, long long brick, i
 }
 
 // ---------------------------------------------------------------------------
+// Preparation: spot records, layer records.
+__device__ __forceinline__ void spot_prep_one(const dh_spot *__restrict__ in, SpotExt *__restrict__ out, int s,
+                                              float inv_vsadx, float inv_vsady) {
+    const dh_spot p = gld(in, s);
+    SpotExt e;
+    e.x = p.x;
+    e.y = p.y;
+    e.mu = p.mu;
+    e.tx = p.x * inv_vsadx;
+    e.ty = p.y * inv_vsady;
+    const float itt = 1.0f / (e.tx * e.tx + e.ty * e.ty + 1.0f);
+    e.m11 = 1.0f - e.tx * e.tx * itt;
+    e.m22 = 1.0f - e.ty * e.ty * itt;
+    e.m12x2 = -2.0f * e.tx * e.ty * itt;
+    gst(out, s, e);
+}
+
+__global__ __launch_bounds__(256) void k_spot_prep(BeamCtx c) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= c.n_spots) return;
+    spot_prep_one(c.spots_in, c.sp, s, 1.0f / c.b.vsadx, 1.0f / c.b.vsady);
+}
+
+// One wave per layer.  A record the kernels cannot use safely (energy id
+// outside the tables, spot range outside the spot array) gets ok = 0 and is
+// skipped rather than read out of bounds.
+__device__ __forceinline__ void layer_prep_one(const BeamCtx &cx, int l, int lane) {
+    const dh_layer L = gld(cx.layers_in, l);
+    LayerC c;
+    c.ok = (unsigned)L.energy_id < (unsigned)cx.m.n_energies && L.spot_start >= 0 && L.n_spots > 0 &&
+           L.spot_start <= cx.n_spots - L.n_spots;
+    c.start = c.ok ? L.spot_start : 0;
+    c.n = c.ok ? L.n_spots : 0;
+    c.eid = c.ok ? L.energy_id : 0;
+    c.r80 = L.r80;
+    c.lim = 1.1f * L.r80;
+    const float *d = gptr(cx.m.dvp, (long long)c.eid * cx.m.dvp_len);
+    c.A = d[2];
+    c.B = d[3];
+    c.C = d[4];
+    c.pad = 0.0f;
+    if (c.ok) {
+        reach_of(cx.sp, c.start, c.n, cx.m, c.eid, lane, c);
+    } else {
+        c.xmin = c.ymin = -3.0e38f;
+        c.xmax = c.ymax = 3.0e38f;
+        c.lam_min = 0.0f;
+        c.sms_max = 0.0f;
+    }
+    if (lane == 0) gst(cx.lc, l, c);
+}
+
+__global__ __launch_bounds__(64) void k_layer_prep(BeamCtx c) { layer_prep_one(c, blockIdx.x, threadIdx.x); }
+
+// Batched preparation: one thread per spot of every beam, then one wave per
+// layer of every beam.
+__global__ __launch_bounds__(256) void k_spot_prep_batch(const BeamCtx *__restrict__ ctx,
+                                                        const long long *__restrict__ off, int n_beams,
+                                                        long long total) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int bm = find_owner(off, n_beams, t);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    spot_prep_one(c.spots_in, c.sp, (int)(t - off[bm]), 1.0f / c.b.vsadx, 1.0f / c.b.vsady);
+}
+
+__global__ __launch_bounds__(64) void k_layer_prep_batch(const BeamCtx *__restrict__ ctx,
+                                                        const long long *__restrict__ off, int n_beams) {
+    const long long item = blockIdx.x;
+    const int bm = find_owner(off, n_beams, item);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    layer_prep_one(c, (int)(item - off[bm]), threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
 // Forward.  One wave per 4x4x4 brick, one voxel per lane; a lane owns its
 // voxel's sum (deterministic, no atomics).  The lanes first judge 64 layers at
 // a time (reaches the brick's WET range, and with culling: some spot of the
@@ -308,8 +308,8 @@ __device__ __forceinline__ void brick_of(const BrickGrid &bg, long long brick, i
 // + brick radius) and the surviving spots are evaluated four at a time, their
 // records coming in through scalar loads.
 //
-// COUNT: no dose; counts[0] += in-range voxels x layer spots (the reference's
-// work), counts[1] += in-range voxels x spots evaluated here.
+// COUNT: no dose; *n_dense += in-range voxels x layer spots (the reference's
+// work), *n_exec += in-range voxels x spots evaluated here.
 __device__ __forceinline__ void spot_eval(const SpotExt &q, float hx, float hy, float hz, float a1, float a2,
                                           float &P, float &H) {
     const float d2 = axis_d2(hx, hy, hz, q);
@@ -317,59 +317,66 @@ __device__ __forceinline__ void spot_eval(const SpotExt &q, float hx, float hy, 
     H = fmaf(q.mu, __builtin_amdgcn_exp2f(a2 * d2), H);
 }
 
-template <bool CULL, bool COUNT = false>
-__global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ wetv, float *__restrict__ dose, Grid g,
-                                                   Geom b, BrickGrid bg, unsigned nblocks, Model m,
-                                                   const LayerC *__restrict__ lc, int n_layers,
-                                                   const SpotExt *__restrict__ sp, float cut, float rbrick,
-                                                   int accumulate, unsigned long long *__restrict__ counts = nullptr) {
-    // plain dispatch order: consecutive bricks go to different XCDs, which
-    // spreads the heavy bricks of the beam path over the whole chip
-    const long long brick = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (brick >= bg.count()) return;  // whole wave leaves together
-    const int lane = threadIdx.x & 63;
-    int bi, bj, bk;
-    brick_of(bg, brick, bi, bj, bk);
-    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
-    const bool valid = i < g.nx && j < g.ny && k < g.nz;
-    const long long idx = valid ? i + (long long)g.nx * (j + (long long)g.ny * k) : 0;
-    const float wet = valid ? wetv[idx] * 10.0f : 3.0e38f;
+struct VoxelOf {
+    bool valid;
+    long long idx;
+};
 
-    const float3 v = voxel_xyz(g, b, i, j, k);
-    const float3 h = image_to_head(b, v);
-    const float sx = b.src[0] - v.x, sy = b.src[1] - v.y, sz = b.src[2] - v.z;
+__device__ __forceinline__ VoxelOf brick_voxel(const Grid &g, const BrickGrid &bg, long long brick, int lane, int &bi,
+                                               int &bj, int &bk, int &i, int &j, int &k) {
+    brick_of(bg, brick, bi, bj, bk);
+    i = bi * BRICK + (lane & 3);
+    j = bj * BRICK + ((lane >> 2) & 3);
+    k = bk * BRICK + (lane >> 4);
+    VoxelOf r;
+    r.valid = i < g.nx && j < g.ny && k < g.nz;
+    r.idx = r.valid ? i + (long long)g.nx * (j + (long long)g.ny * k) : 0;
+    return r;
+}
+
+template <bool CULL, bool COUNT>
+__device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick, int lane, VoxelOf &vo,
+                                               unsigned long long &n_dense, unsigned long long &n_exec) {
+    int bi, bj, bk, i, j, k;
+    vo = brick_voxel(c.g, c.bg, brick, lane, bi, bj, bk, i, j, k);
+    const float wet = vo.valid ? gld(c.wet, vo.idx) * 10.0f : 3.0e38f;
+
+    const float3 v = voxel_xyz(c.g, c.b, i, j, k);
+    const float3 h = image_to_head(c.b, v);
+    const float sx = c.b.src[0] - v.x, sy = c.b.src[1] - v.y, sz = c.b.src[2] - v.z;
     const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
-    const float ivx = 1.0f / b.vsadx, ivy = 1.0f / b.vsady;
-    const BrickInfo B = brick_info_at(b, brick_centre(g, b, bi, bj, bk), wave_min(wet),
-                                      wave_max(valid ? wet : -3.0e38f));
+    const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
+    const BrickInfo B = brick_info_at(c.b, brick_centre(c.g, c.b, bi, bj, bk), wave_min(wet),
+                                      wave_max(vo.valid ? wet : -3.0e38f));
+    const LayerC *__restrict__ lc = c.lc;
+    const SpotExt *__restrict__ sp = c.sp;
 
     float acc = 0.0f;
-    unsigned long long n_dense = 0, n_exec = 0;
-    for (int l0 = 0; l0 < n_layers; l0 += WAVE) {
+    for (int l0 = 0; l0 < c.n_layers; l0 += WAVE) {
         bool lrel = false;
-        if (l0 + lane < n_layers) {
-            const LayerC c = lc[l0 + lane];
-            lrel = c.ok && B.wlo <= c.lim;
-            if (CULL && lrel) lrel = !reach_skip(c, B, ivx, ivy, cut, rbrick);
+        if (l0 + lane < c.n_layers) {
+            const LayerC L = gld(lc, l0 + lane);
+            lrel = L.ok && B.wlo <= L.lim;
+            if (CULL && lrel) lrel = !reach_skip(L, B, ivx, ivy, c.cut, c.rbrick);
         }
         unsigned long long lmask = __ballot(lrel);
         while (lmask) {
             const int l = l0 + __builtin_ctzll(lmask);
             lmask &= lmask - 1ull;
-            const LayerC c = lc[l];
-            const bool act = wet <= c.lim;
+            const LayerC L = gld(lc, l);
+            const bool act = wet <= L.lim;
             const unsigned long long amask = __ballot(act);
             if (!amask) continue;
-            const VoxTerms T = vox_terms(m, c, wet, dsrc);
+            const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
             float rc2 = 0.0f;
             if (CULL) {
-                const float rc = fmaf(cut, wave_max(act ? T.sh : 0.0f), rbrick);
+                const float rc = fmaf(c.cut, wave_max(act ? T.sh : 0.0f), c.rbrick);
                 rc2 = rc * rc;
             }
             float P = 0.0f, H = 0.0f;
-            const SpotExt *lsp = sp + c.start;
-            for (int c0 = 0; c0 < c.n; c0 += WAVE) {
-                const int nin = min(WAVE, c.n - c0);
+            const SpotExt *lsp = gptr(sp, L.start);
+            for (int c0 = 0; c0 < L.n; c0 += WAVE) {
+                const int nin = min(WAVE, L.n - c0);
                 unsigned long long mask;
                 if (CULL) {
                     bool rel = false;
@@ -408,49 +415,104 @@ __global__ __launch_bounds__(256) void k_pb_forward(const float *__restrict__ we
         }
     }
     if (COUNT) {
-        for (int l = 0; l < n_layers; ++l) {
-            const LayerC c = lc[l];
-            if (c.ok) n_dense += (unsigned long long)__popcll(__ballot(wet <= c.lim)) * (unsigned)c.n;
+        for (int l = 0; l < c.n_layers; ++l) {
+            const LayerC L = gld(lc, l);
+            if (L.ok) n_dense += (unsigned long long)__popcll(__ballot(wet <= L.lim)) * (unsigned)L.n;
         }
+    }
+    return acc;
+}
+
+template <bool CULL, bool COUNT = false>
+__global__ __launch_bounds__(256) void k_pb_forward(BeamCtx c, int accumulate,
+                                                   unsigned long long *__restrict__ counts = nullptr) {
+    // plain dispatch order: consecutive bricks go to different XCDs, which
+    // spreads the heavy bricks of the beam path over the whole chip
+    const long long brick = (long long)blockIdx.x * 4 + wave_in_block();
+    if (brick >= c.bg.count()) return;  // whole wave leaves together
+    const int lane = threadIdx.x & 63;
+    VoxelOf vo;
+    unsigned long long n_dense = 0, n_exec = 0;
+    const float acc = forward_brick<CULL, COUNT>(c, brick, lane, vo, n_dense, n_exec);
+    if (COUNT) {
         if (lane == 0) {
             atomicAdd(&counts[0], n_dense);
             atomicAdd(&counts[1], n_exec);
         }
         return;
     }
-    if (valid) dose[idx] = accumulate ? dose[idx] + acc : acc;
+    if (vo.valid) gst(c.dose, vo.idx, accumulate ? gld(c.dose, vo.idx) + acc : acc);
+}
+
+// Batched forward: one wave per brick of every beam; beam b owns items
+// [off[b], off[b + 1]) and writes its own dose volume (the plan's dose when the
+// plan has one beam, else a per-beam volume that k_plan_sum adds up).
+template <bool CULL>
+__global__ __launch_bounds__(256) void k_pb_forward_batch(const BeamCtx *__restrict__ ctx,
+                                                         const long long *__restrict__ off, int n_beams,
+                                                         long long base, long long total) {
+    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
+    if (item >= total) return;
+    const int bm = find_owner(off, n_beams, item);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    VoxelOf vo;
+    unsigned long long nd = 0, ne = 0;
+    const float acc = forward_brick<CULL, false>(c, item - off[bm], threadIdx.x & 63, vo, nd, ne);
+    if (vo.valid) gst(c.dose, vo.idx, acc);
+}
+
+// Dose of a plan with several beams: the sum of its beams' volumes, added in
+// beam order (the order of single-beam calls with accumulate = 1).  Plan p owns
+// blocks [off[p], off[p + 1]) of 256 voxels.
+__global__ __launch_bounds__(256) void k_plan_sum(const PlanCtx *__restrict__ plans, const BeamCtx *__restrict__ ctx,
+                                                 const long long *__restrict__ off, int n_plans) {
+    const long long blk = blockIdx.x;
+    const int p = find_owner(off, n_plans, blk);
+    const PlanCtx P = plans[p];
+    const long long t = (blk - off[p]) * 256 + threadIdx.x;
+    if (t >= P.nvox) return;
+    float d = gld(ctx[P.beam0].dose, t);
+    for (int b = P.beam0 + 1; b < P.beam0 + P.n_beams; ++b) d += gld(ctx[b].dose, t);
+    gst(P.dose, t, d);
 }
 
 // ---------------------------------------------------------------------------
 // One wave per brick: WET range (mm) of the voxels with dL/dD != 0; an empty
 // brick gets (+big, -big).
-__global__ __launch_bounds__(256) void k_brick_summary(const float *__restrict__ wetv,
-                                                      const float *__restrict__ gdose, Grid g, BrickGrid bg,
-                                                      float2 *__restrict__ out) {
-    const long long brick = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (brick >= bg.count()) return;
-    const int lane = threadIdx.x & 63;
-    int bi, bj, bk;
-    brick_of(bg, brick, bi, bj, bk);
-    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
+__device__ __forceinline__ void brick_summary_one(const BeamCtx &c, long long brick, int lane) {
+    int bi, bj, bk, i, j, k;
+    const VoxelOf vo = brick_voxel(c.g, c.bg, brick, lane, bi, bj, bk, i, j, k);
     float lo = 3.0e38f, hi = -3.0e38f;
-    if (i < g.nx && j < g.ny && k < g.nz) {
-        const long long idx = i + (long long)g.nx * (j + (long long)g.ny * k);
-        if (gdose[idx] != 0.0f) lo = hi = wetv[idx] * 10.0f;
-    }
+    if (vo.valid && gld(c.gdose, vo.idx) != 0.0f) lo = hi = gld(c.wet, vo.idx) * 10.0f;
     lo = wave_min(lo);
     hi = wave_max(hi);
-    if (lane == 0) out[brick] = make_float2(lo, hi);
+    if (lane == 0) gst(c.bsum, brick, make_float2(lo, hi));
+}
+
+__global__ __launch_bounds__(256) void k_brick_summary(BeamCtx c) {
+    const long long brick = (long long)blockIdx.x * 4 + wave_in_block();
+    if (brick >= c.bg.count()) return;
+    brick_summary_one(c, brick, threadIdx.x & 63);
+}
+
+__global__ __launch_bounds__(256) void k_brick_summary_batch(const BeamCtx *__restrict__ ctx,
+                                                            const long long *__restrict__ off, int n_beams,
+                                                            long long base, long long total) {
+    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
+    if (item >= total) return;
+    const int bm = find_owner(off, n_beams, item);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    brick_summary_one(c, item - off[bm], threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint.  Workgroup = one wave = one 64-spot chunk of one layer x one slab of
-// bricks.  Lane = spot: its three sums stay in registers across the whole
-// slab.  The lanes judge 64 bricks at a time (holds voxels with dL/dD != 0
-// within the layer's range, and with culling: within reach of the chunk);
-// per surviving brick the wave acts as lanes = voxels to compute the voxel
-// terms, compacts the voxels that matter into LDS, then every spot lane walks
-// them with broadcast LDS reads.
+// Adjoint.  Workgroup = one wave = one CS-spot chunk of one layer x one slab
+// of bricks.  Lane = (voxel subset, spot); a spot's three sums stay in
+// registers across the whole slab.  The lanes judge 64 bricks at a time
+// (holds voxels with dL/dD != 0 within the layer's range, and with culling:
+// within reach of the chunk); per surviving brick the wave acts as lanes =
+// voxels to compute the voxel terms, compacts the voxels that matter into LDS,
+// then every spot lane walks its subset of them with broadcast LDS reads.
 //   dL/dmu_s = sum_v g_v [A e1 + B e2]
 //   dL/dx_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADX) p_x
 //   dL/dy_s  = -2 mu_s sum_v g_v [A a1 e1 + B a2 e2] (1 + q/VSADY) p_y
@@ -458,95 +520,90 @@ __global__ __launch_bounds__(256) void k_brick_summary(const float *__restrict__
 // q = (t . r) / |t|^2, p = r - q t (its part perpendicular to the axis) and
 // a1, a2 = -1/(2 sigma^2) (natural-log scale).
 //
-// COUNT: no gradient; counts[2] += voxels (dL/dD != 0, in range) x chunk spots
-// (the reference's work), counts[3] += those voxels x spots evaluated here.
+// COUNT: no gradient; *n_dense += voxels (dL/dD != 0, in range) x chunk spots
+// (the reference's work), *n_exec += those voxels x spots evaluated here.
 struct LdsVox {
     float hx, hy, hz, a1;
     float a2, GA, GB, GAa;
     float GBb, pad0, pad1, pad2;
 };
 
-template <bool CULL, int CS, bool COUNT = false>
-__global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wetv, const float *__restrict__ gdose,
-                                                  Grid g, Geom b, BrickGrid bg, Model m,
-                                                  const LayerC *__restrict__ lc, int n_layers,
-                                                  const SpotExt *__restrict__ sp, int n_spots, float cut,
-                                                  float rbrick, int nslab, const float2 *__restrict__ bsum,
-                                                  float4 *__restrict__ partial,
-                                                  unsigned long long *__restrict__ counts = nullptr) {
-    __shared__ LdsVox s_vox[WAVE];
+// Number of CS-spot chunks the grid provides for a beam (an upper bound of
+// sum over layers of ceil(n_l / CS)).
+__host__ __device__ inline long long adjoint_chunks(int n_spots, int n_layers, int cs) {
+    return ((long long)n_spots + (cs - 1LL) * n_layers) / cs + 1;
+}
+
+template <bool CULL, int CS, bool COUNT>
+__device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab, int lane, LdsVox *s_vox,
+                                             unsigned long long &n_dense, unsigned long long &n_exec) {
     constexpr int G = WAVE / CS;  // voxel subsets per wave: lane = group * CS + slot
-    const int lane = threadIdx.x;
     const int slot = lane % CS, group = lane / CS;
+    const LayerC *__restrict__ lc = c.lc;
     // chunk -> (layer, first spot of the chunk within the layer)
-    int cid = blockIdx.x, l = 0, c0 = -1;
-    for (; l < n_layers; ++l) {
-        const int nch = (lc[l].n + CS - 1) / CS;
+    int l = 0, c0 = -1;
+    for (; l < c.n_layers; ++l) {
+        const int nch = (gld(lc, l).n + CS - 1) / CS;
         if (cid < nch) {
             c0 = cid * CS;
             break;
         }
         cid -= nch;
     }
-    if (c0 < 0) return;  // surplus workgroup of the upper-bound grid
-    const LayerC L = lc[l];
+    if (c0 < 0) return;  // surplus item of the upper-bound grid
+    const LayerC L = gld(lc, l);
     if (!L.ok) return;
-    const int slab = blockIdx.y;
     const bool sval = c0 + slot < L.n;
     const int spot = L.start + c0 + slot;
     SpotExt q;
-    if (sval) q = sp[spot];
+    if (sval) q = gld(c.sp, spot);
     else q = SpotExt{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f};
     const float itt = 1.0f / (q.tx * q.tx + q.ty * q.ty + 1.0f);
-    const float ivx = 1.0f / b.vsadx, ivy = 1.0f / b.vsady;
+    const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
     const unsigned long long chunk_n = (unsigned long long)min(CS, L.n - c0);
     LayerC R = L;  // reach of this chunk's spots only
-    if (CULL) reach_of(sp, L.start + c0, (int)chunk_n, m, L.eid, lane, R);
+    if (CULL) reach_of(c.sp, L.start + c0, (int)chunk_n, c.m, L.eid, lane, R);
 
-    const long long nb = bg.count();
-    const long long b0 = nb * slab / nslab, b1 = nb * (slab + 1) / nslab;
+    const long long nb = c.bg.count();
+    const long long b0 = nb * slab / c.nslab, b1 = nb * (slab + 1) / c.nslab;
     float K = 0.0f, FX = 0.0f, FY = 0.0f;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    unsigned long long n_dense = 0, n_exec = 0;
 
     for (long long base = b0; base < b1; base += WAVE) {
         bool brel = false;
         if (base + lane < b1) {
-            const float2 ws = bsum[base + lane];
+            const float2 ws = gld(c.bsum, base + lane);
             brel = ws.x <= ws.y && ws.x <= L.lim;
             if (CULL && brel && !COUNT) {
                 int ti, tj, tk;
-                brick_of(bg, base + lane, ti, tj, tk);
-                brel = !reach_skip(R, brick_info_at(b, brick_centre(g, b, ti, tj, tk), ws.x, ws.y), ivx, ivy, cut,
-                                   rbrick);
+                brick_of(c.bg, base + lane, ti, tj, tk);
+                brel = !reach_skip(R, brick_info_at(c.b, brick_centre(c.g, c.b, ti, tj, tk), ws.x, ws.y), ivx, ivy,
+                                   c.cut, c.rbrick);
             }
         }
         unsigned long long fm = __ballot(brel);
         while (fm) {
             const long long brick = base + __builtin_ctzll(fm);
             fm &= fm - 1ull;
-            int bi, bj, bk;
-            brick_of(bg, brick, bi, bj, bk);
-            const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
-            const bool valid = i < g.nx && j < g.ny && k < g.nz;
-            const long long idx = valid ? i + (long long)g.nx * (j + (long long)g.ny * k) : 0;
-            const float gv = valid ? gdose[idx] : 0.0f;
-            const float wet = valid ? wetv[idx] * 10.0f : 3.0e38f;
-            const bool act = valid && gv != 0.0f && wet <= L.lim;
+            int bi, bj, bk, i, j, k;
+            const VoxelOf vo = brick_voxel(c.g, c.bg, brick, lane, bi, bj, bk, i, j, k);
+            const float gv = vo.valid ? gld(c.gdose, vo.idx) : 0.0f;
+            const float wet = vo.valid ? gld(c.wet, vo.idx) * 10.0f : 3.0e38f;
+            const bool act = vo.valid && gv != 0.0f && wet <= L.lim;
             const unsigned long long am = __ballot(act);
             if (!am) continue;
             if (COUNT) n_dense += (unsigned long long)__popcll(am) * chunk_n;
 
-            const float3 v = voxel_xyz(g, b, i, j, k);
-            const float3 h = image_to_head(b, v);
-            const float sx = b.src[0] - v.x, sy = b.src[1] - v.y, sz = b.src[2] - v.z;
+            const float3 v = voxel_xyz(c.g, c.b, i, j, k);
+            const float3 h = image_to_head(c.b, v);
+            const float sx = c.b.src[0] - v.x, sy = c.b.src[1] - v.y, sz = c.b.src[2] - v.z;
             const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
-            const VoxTerms T = vox_terms(m, L, wet, dsrc);
+            const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
 
             bool rel = sval;
             if (CULL) {
-                const float rc = fmaf(cut, wave_max(act ? T.sh : 0.0f), rbrick);
-                const float3 hc = image_to_head(b, brick_centre(g, b, bi, bj, bk));
+                const float rc = fmaf(c.cut, wave_max(act ? T.sh : 0.0f), c.rbrick);
+                const float3 hc = image_to_head(c.b, brick_centre(c.g, c.b, bi, bj, bk));
                 rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
             }
             if (COUNT) {
@@ -596,13 +653,7 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
             __syncthreads();
         }
     }
-    if (COUNT) {
-        if (lane == 0) {
-            atomicAdd(&counts[2], n_dense);
-            atomicAdd(&counts[3], n_exec);
-        }
-        return;
-    }
+    if (COUNT) return;
     // fold the voxel subsets (fixed order: deterministic)
 #pragma unroll
     for (int off = CS; off < WAVE; off <<= 1) {
@@ -610,27 +661,61 @@ __global__ __launch_bounds__(64) void k_pb_adjoint(const float *__restrict__ wet
         FX += __shfl_xor(FX, off);
         FY += __shfl_xor(FY, off);
     }
-    if (sval && group == 0) partial[(long long)slab * n_spots + spot] = make_float4(K, FX, FY, 0.0f);
+    if (sval && group == 0) gst(c.partial, (long long)slab * c.n_spots + spot, make_float4(K, FX, FY, 0.0f));
 }
 
-// Spots that no chunk covers keep the zeroed partials (k_zero4).
+template <bool CULL, int CS, bool COUNT = false>
+__global__ __launch_bounds__(64) void k_pb_adjoint(BeamCtx c, unsigned long long *__restrict__ counts = nullptr) {
+    __shared__ LdsVox s_vox[WAVE];
+    unsigned long long nd = 0, ne = 0;
+    adjoint_item<CULL, CS, COUNT>(c, blockIdx.x, blockIdx.y, threadIdx.x, s_vox, nd, ne);
+    if (COUNT && threadIdx.x == 0) {
+        atomicAdd(&counts[2], nd);
+        atomicAdd(&counts[3], ne);
+    }
+}
+
+// Batched adjoint: beam b owns items [off[b], off[b + 1]) = chunk x slab.  One
+// launch per chunk size: beams of the other size own no items of it.
+template <bool CULL, int CS>
+__global__ __launch_bounds__(64) void k_pb_adjoint_batch(const BeamCtx *__restrict__ ctx,
+                                                        const long long *__restrict__ off, int n_beams,
+                                                        long long base, long long total) {
+    __shared__ LdsVox s_vox[WAVE];
+    const long long item = base + blockIdx.x;
+    if (item >= total) return;
+    const int bm = find_owner(off, n_beams, item);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    const long long local = item - off[bm];
+    const int cid = (int)(local / c.nslab), slab = (int)(local % c.nslab);
+    unsigned long long nd = 0, ne = 0;
+    adjoint_item<CULL, CS, false>(c, cid, slab, threadIdx.x, s_vox, nd, ne);
+}
+
+// Zero the adjoint partials (spots that no chunk covers keep zeros).
 __global__ __launch_bounds__(256) void k_zero4(float4 *__restrict__ p, long long n) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_zero4_batch(const BeamCtx *__restrict__ ctx,
+                                                    const long long *__restrict__ off, int n_beams, long long total) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int bm = find_owner(off, n_beams, t);
+    gst(ctx[bm].partial, t - off[bm], make_float4(0.f, 0.f, 0.f, 0.f));
 }
 
 // 256 threads = 64 spots x 4 slab groups; each group sums every 4th slab,
 // then the 4 group sums are added in a fixed order.
-__global__ __launch_bounds__(256) void k_pb_reduce(const float4 *__restrict__ partial, const SpotExt *__restrict__ sp,
-                                                  int n_spots, int nslab, float *__restrict__ grad_mu,
-                                                  float *__restrict__ grad_xy) {
+__device__ __forceinline__ void reduce_spots(const BeamCtx &c, long long block, int tid) {
     __shared__ float4 s_part[4][64];
-    const int sl = threadIdx.x & 63, grp = threadIdx.x >> 6;
-    const int s = blockIdx.x * 64 + sl;
+    const int sl = tid & 63, grp = tid >> 6;
+    const long long s = block * 64 + sl;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (s < n_spots) {
-        for (int d = grp; d < nslab; d += 4) {
-            const float4 p = partial[(long long)d * n_spots + s];
+    if (s < c.n_spots) {
+        for (int d = grp; d < c.nslab; d += 4) {
+            const float4 p = gld(c.partial, (long long)d * c.n_spots + s);
             acc.x += p.x;
             acc.y += p.y;
             acc.z += p.z;
@@ -638,65 +723,81 @@ __global__ __launch_bounds__(256) void k_pb_reduce(const float4 *__restrict__ pa
     }
     s_part[grp][sl] = acc;
     __syncthreads();
-    if (grp == 0 && s < n_spots) {
+    if (grp == 0 && s < c.n_spots) {
         float K = 0.f, FX = 0.f, FY = 0.f;
         for (int q = 0; q < 4; ++q) {
             K += s_part[q][sl].x;
             FX += s_part[q][sl].y;
             FY += s_part[q][sl].z;
         }
-        const float mu2 = -2.0f * sp[s].mu;
-        grad_mu[s] = K;
-        grad_xy[2 * s] = mu2 * FX;
-        grad_xy[2 * s + 1] = mu2 * FY;
+        const float mu2 = -2.0f * gld(c.sp, s).mu;
+        gst(c.grad_mu, s, K);
+        gst(c.grad_xy, 2 * s, mu2 * FX);
+        gst(c.grad_xy, 2 * s + 1, mu2 * FY);
     }
 }
 
+__global__ __launch_bounds__(256) void k_pb_reduce(BeamCtx c) { reduce_spots(c, blockIdx.x, threadIdx.x); }
+
+__global__ __launch_bounds__(256) void k_pb_reduce_batch(const BeamCtx *__restrict__ ctx,
+                                                        const long long *__restrict__ off, int n_beams) {
+    const long long blk = blockIdx.x;
+    const int bm = find_owner(off, n_beams, blk);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    reduce_spots(c, blk - off[bm], threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
+// Host side.
 static constexpr long long kAlign = 256;
 static long long align_up(long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-// Slabs of bricks per spot chunk: enough (chunk, slab) workgroups (~32k) that
-// the few heavy ones (chunks and slabs in the beam path) do not leave the chip
-// idle at the end, with at least 64 bricks per slab.
 // Spots per adjoint workgroup: 64, or 32 (with two voxel subsets per wave) when
 // the average layer is small enough that 64-spot chunks would leave many lanes
 // idle.  Scores: lane use / (1 + per-brick overhead share), the overhead share
 // growing as chunks shrink.
-static int adjoint_chunk(int n_spots, int n_layers) {
+int adjoint_chunk(int n_spots, int n_layers) {
     if (n_layers <= 0) return 64;
     const double n = (double)n_spots / n_layers;
     const double u64 = n / (ceil(n / 64.0) * 64.0), u32 = n / (ceil(n / 32.0) * 32.0);
     return (u32 / 1.38 > u64 / 1.23) ? 32 : 64;
 }
 
-static int adjoint_slabs(const dh_grid &g, int n_spots, int n_layers) {
-    const long long nb = (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
-    const int cs = adjoint_chunk(n_spots, n_layers);
-    const long long nch = ((long long)n_spots + (cs - 1LL) * n_layers) / cs + 1;
-    long long d = (32768 + nch - 1) / nch;
-    if (d > 1024) d = 1024;
-    if (d > nb / 64) d = nb / 64;
-    if (d > nb) d = nb;
+// Slabs of bricks per spot chunk: enough (chunk, slab) workgroups (wg_target)
+// that the few heavy ones (chunks and slabs in the beam path) do not leave the
+// chip idle at the end, with at least min_bricks bricks per slab and at most
+// max_slabs slabs.
+int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t) {
+    const long long nch = adjoint_chunks(n_spots, n_layers, cs);
+    long long d = (t.wg_target + nch - 1) / nch;
+    if (d > t.max_slabs) d = t.max_slabs;
+    if (d > nbricks / t.min_bricks) d = nbricks / t.min_bricks;
+    if (d > nbricks) d = nbricks;
     if (d < 1) d = 1;
     return (int)d;
 }
 
-// Workspace: [SpotExt x S][LayerC x L][adjoint partials float4 x nslab*S][brick summaries float2]
-struct Workspace {
-    SpotExt *ext;
-    LayerC *layer;
-    float4 *partial;
-    float2 *bsum;
-    int nslab;
-    long long bytes;
-};
+AdjTune default_tune() {
+    AdjTune t;
+    t.cs = 0;
+    t.wg_target = 32768;
+    t.min_bricks = 64;
+    t.max_slabs = 1024;
+    return t;
+}
 
-static Workspace carve(void *base, const dh_grid &g, int S, int L) {
-    Workspace w;
+bool tune_ok(const AdjTune &t) {
+    return (t.cs == 0 || t.cs == 32 || t.cs == 64) && t.wg_target >= 1 && t.min_bricks >= 1 && t.max_slabs >= 1 &&
+           t.max_slabs <= 65535;
+}
+
+// Workspace of one beam: [SpotExt x S][LayerC x L][adjoint partials float4 x
+// nslab*S][brick summaries float2 x bricks], each part 256-byte aligned.
+BeamWs carve_beam(void *base, long long nbricks, int S, int L, const AdjTune &t) {
+    BeamWs w;
     char *p = reinterpret_cast<char *>(base);
-    w.nslab = adjoint_slabs(g, S, L);
-    const long long nb = (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
+    w.cs = t.cs ? t.cs : adjoint_chunk(S, L);
+    w.nslab = adjoint_slabs(nbricks, S, L, w.cs, t);
     long long off = 0;
     w.ext = reinterpret_cast<SpotExt *>(p + off);
     off += align_up((long long)S * (long long)sizeof(SpotExt));
@@ -705,12 +806,16 @@ static Workspace carve(void *base, const dh_grid &g, int S, int L) {
     w.partial = reinterpret_cast<float4 *>(p + off);
     off += align_up((long long)w.nslab * S * 16LL);
     w.bsum = reinterpret_cast<float2 *>(p + off);
-    off += align_up(nb * 8LL);
+    off += align_up(nbricks * 8LL);
     w.bytes = off;
     return w;
 }
 
-static bool fill_model(const dh_beam_model *bm, Model &m) {
+static long long grid_bricks(const dh_grid &g) {
+    return (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
+}
+
+bool fill_model(const dh_beam_model *bm, Model &m) {
     if (!bm || !bm->idd_sigma || !bm->dvp || bm->n_energies <= 0 || bm->lut_len < 2 || bm->dvp_len < 5 ||
         (!bm->depth_uniform && !bm->depths) || (bm->depth_uniform && !(bm->ddepth > 0.0f))) {
         set_error("invalid beam model");
@@ -728,20 +833,185 @@ static bool fill_model(const dh_beam_model *bm, Model &m) {
     return true;
 }
 
-static bool common_args(const dh_grid &g, const dh_beam_geom *geom, const dh_spot_set *ss, void *ws,
-                        long long ws_bytes) {
-    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0.0f) || !geom || !ss) {
-        set_error("invalid grid, geometry or spot set");
+bool pencil_grid_ok(const dh_grid &g) {
+    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0.0f)) {
+        set_error("invalid grid %d x %d x %d, spacing %g", g.nx, g.ny, g.nz, (double)g.spacing);
         return false;
     }
     if ((double)g.nx * g.ny * g.nz >= 2147483647.0) {
         set_error("grid %d x %d x %d too large (below 2^31 voxels required)", g.nx, g.ny, g.nz);
         return false;
     }
-    if (ss->n_spots < 0 || ss->n_layers < 0 || (ss->n_spots > 0 && (!ss->spots || !ss->layers))) {
-        set_error("invalid spot set (n_spots=%d, n_layers=%d)", ss->n_spots, ss->n_layers);
+    return true;
+}
+
+bool spot_set_ok(const dh_spot_set *ss) {
+    if (!ss || ss->n_spots < 0 || ss->n_layers < 0 || (ss->n_spots > 0 && (!ss->spots || !ss->layers))) {
+        set_error("invalid spot set (n_spots=%d, n_layers=%d)", ss ? ss->n_spots : -1, ss ? ss->n_layers : -1);
         return false;
     }
+    return true;
+}
+
+float brick_radius(float spacing) { return 0.5f * (BRICK - 1) * 1.7320508075688772f * spacing * 1.0001f; }
+
+// Fill the pencil-beam part of a beam context.
+void fill_pencil_ctx(BeamCtx &c, const dh_grid &g, const dh_beam_geom &geom, const Model &m, const dh_spot_set &ss,
+                     const BeamWs &w, float cutoff_sigma) {
+    c.g = make_grid(g);
+    c.b = make_geom(geom);
+    c.bg = brick_grid(c.g);
+    c.m = m;
+    c.spots_in = ss.spots;
+    c.layers_in = ss.layers;
+    c.n_spots = ss.n_spots;
+    c.n_layers = ss.n_spots > 0 ? ss.n_layers : 0;
+    c.sp = w.ext;
+    c.lc = w.layer;
+    c.partial = w.partial;
+    c.bsum = w.bsum;
+    c.nslab = w.nslab;
+    c.cs = w.cs;
+    c.cut = cutoff_sigma > 0.0f ? cutoff_sigma : 0.0f;
+    c.rbrick = brick_radius(g.spacing);
+}
+
+// Launch helpers.  Item spaces larger than kMaxBlocks workgroups go out in
+// several launches (offset by `base`).
+static constexpr long long kMaxBlocks = 1LL << 22;
+
+static int prepare(hipStream_t st, const BeamCtx &c) {
+    if (c.n_spots == 0) return DH_OK;
+    hipLaunchKernelGGL(k_spot_prep, dim3((c.n_spots + 255) / 256), dim3(256), 0, st, c);
+    int rc = check_launch("k_spot_prep");
+    if (rc || c.n_layers == 0) return rc;
+    hipLaunchKernelGGL(k_layer_prep, dim3(c.n_layers), dim3(64), 0, st, c);
+    return check_launch("k_layer_prep");
+}
+
+template <bool COUNT>
+static int launch_forward(hipStream_t st, const BeamCtx &c, int accumulate, unsigned long long *counts) {
+    const unsigned nb = (unsigned)((c.bg.count() + 3) / 4);
+    if (c.cut > 0.0f)
+        hipLaunchKernelGGL((k_pb_forward<true, COUNT>), dim3(nb), dim3(256), 0, st, c, accumulate, counts);
+    else
+        hipLaunchKernelGGL((k_pb_forward<false, COUNT>), dim3(nb), dim3(256), 0, st, c, accumulate, counts);
+    return check_launch(COUNT ? "k_pb_forward(count)" : "k_pb_forward");
+}
+
+template <bool COUNT>
+static int launch_adjoint(hipStream_t st, const BeamCtx &c, unsigned long long *counts) {
+    hipLaunchKernelGGL(k_brick_summary, dim3((unsigned)((c.bg.count() + 3) / 4)), dim3(256), 0, st, c);
+    int rc = check_launch("k_brick_summary");
+    if (rc) return rc;
+    const dim3 grid((unsigned)adjoint_chunks(c.n_spots, c.n_layers, c.cs), c.nslab);
+    const bool cull = c.cut > 0.0f;
+    if (c.cs == 32) {
+        if (cull) hipLaunchKernelGGL((k_pb_adjoint<true, 32, COUNT>), grid, dim3(64), 0, st, c, counts);
+        else hipLaunchKernelGGL((k_pb_adjoint<false, 32, COUNT>), grid, dim3(64), 0, st, c, counts);
+    } else {
+        if (cull) hipLaunchKernelGGL((k_pb_adjoint<true, 64, COUNT>), grid, dim3(64), 0, st, c, counts);
+        else hipLaunchKernelGGL((k_pb_adjoint<false, 64, COUNT>), grid, dim3(64), 0, st, c, counts);
+    }
+    return check_launch(COUNT ? "k_pb_adjoint(count)" : "k_pb_adjoint");
+}
+
+// Batched launches: ctx is a device table of beam contexts, the offset tables
+// are device arrays of ascending item starts (first entry 0).
+int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long long *layer_off, int n_beams,
+                      long long total_spots, long long total_layers, hipStream_t st) {
+    if (total_spots > 0) {
+        hipLaunchKernelGGL(k_spot_prep_batch, dim3((unsigned)((total_spots + 255) / 256)), dim3(256), 0, st, ctx,
+                           spot_off, n_beams, total_spots);
+        const int rc = check_launch("k_spot_prep_batch");
+        if (rc) return rc;
+    }
+    if (total_layers > 0) {
+        hipLaunchKernelGGL(k_layer_prep_batch, dim3((unsigned)total_layers), dim3(64), 0, st, ctx, layer_off,
+                           n_beams);
+        return check_launch("k_layer_prep_batch");
+    }
+    return DH_OK;
+}
+
+int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
+                         const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
+                         bool cull, hipStream_t st) {
+    for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
+        const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
+        const dim3 grid((unsigned)((n + 3) / 4));
+        if (cull)
+            hipLaunchKernelGGL(k_pb_forward_batch<true>, grid, dim3(256), 0, st, ctx, brick_off, n_beams, base,
+                               total_bricks);
+        else
+            hipLaunchKernelGGL(k_pb_forward_batch<false>, grid, dim3(256), 0, st, ctx, brick_off, n_beams, base,
+                               total_bricks);
+        const int rc = check_launch("k_pb_forward_batch");
+        if (rc) return rc;
+    }
+    if (total_sum_blocks > 0) {
+        hipLaunchKernelGGL(k_plan_sum, dim3((unsigned)total_sum_blocks), dim3(256), 0, st, plans, ctx, sum_off,
+                           n_plans);
+        return check_launch("k_plan_sum");
+    }
+    return DH_OK;
+}
+
+template <int CS>
+static int launch_adjoint_items(const BeamCtx *ctx, int n_beams, const long long *item_off, long long total,
+                                bool cull, hipStream_t st) {
+    for (long long base = 0; base < total; base += kMaxBlocks) {
+        const long long n = total - base < kMaxBlocks ? total - base : kMaxBlocks;
+        if (cull)
+            hipLaunchKernelGGL((k_pb_adjoint_batch<true, CS>), dim3((unsigned)n), dim3(64), 0, st, ctx, item_off,
+                               n_beams, base, total);
+        else
+            hipLaunchKernelGGL((k_pb_adjoint_batch<false, CS>), dim3((unsigned)n), dim3(64), 0, st, ctx, item_off,
+                               n_beams, base, total);
+        const int rc = check_launch("k_pb_adjoint_batch");
+        if (rc) return rc;
+    }
+    return DH_OK;
+}
+
+int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
+                         const long long *part_off, long long total_part, const long long *item_off32,
+                         long long total_items32, const long long *item_off64, long long total_items64,
+                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st) {
+    int rc;
+    for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
+        const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
+        hipLaunchKernelGGL(k_brick_summary_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
+                           n_beams, base, total_bricks);
+        if ((rc = check_launch("k_brick_summary_batch"))) return rc;
+    }
+    if (total_part > 0) {
+        hipLaunchKernelGGL(k_zero4_batch, dim3((unsigned)((total_part + 255) / 256)), dim3(256), 0, st, ctx,
+                           part_off, n_beams, total_part);
+        if ((rc = check_launch("k_zero4_batch"))) return rc;
+    }
+    if ((rc = launch_adjoint_items<64>(ctx, n_beams, item_off64, total_items64, cull, st))) return rc;
+    if ((rc = launch_adjoint_items<32>(ctx, n_beams, item_off32, total_items32, cull, st))) return rc;
+    if (total_reduce > 0) {
+        hipLaunchKernelGGL(k_pb_reduce_batch, dim3((unsigned)total_reduce), dim3(256), 0, st, ctx, reduce_off,
+                           n_beams);
+        return check_launch("k_pb_reduce_batch");
+    }
+    return DH_OK;
+}
+
+}  // namespace dh
+
+using namespace dh;
+
+static bool common_args(const dh_grid &g, const dh_beam_geom *geom, const dh_spot_set *ss, void *ws,
+                        long long ws_bytes) {
+    if (!pencil_grid_ok(g)) return false;
+    if (!geom) {
+        set_error("null beam geometry");
+        return false;
+    }
+    if (!spot_set_ok(ss)) return false;
     const long long need = dh_pb_workspace_size(ss->n_spots, ss->n_layers, g);
     if (ss->n_spots > 0 && (!ws || ws_bytes < need || ((uintptr_t)ws % kAlign) != 0)) {
         set_error("workspace too small or misaligned (%lld bytes given, %lld needed, 256-byte alignment)",
@@ -751,83 +1021,17 @@ static bool common_args(const dh_grid &g, const dh_beam_geom *geom, const dh_spo
     return true;
 }
 
-// Spot records and per-layer reach into the workspace.
-static int prepare(hipStream_t st, const dh_spot_set *spots, const dh_beam_geom *geom, const Model &m,
-                   const Workspace &w) {
-    const int S = spots->n_spots, L = spots->n_layers;
-    hipLaunchKernelGGL(k_spot_prep, dim3((S + 255) / 256), dim3(256), 0, st, spots->spots, w.ext, S,
-                       1.0f / geom->vsadx, 1.0f / geom->vsady);
-    int rc = check_launch("k_spot_prep");
-    if (rc || L == 0) return rc;
-    hipLaunchKernelGGL(k_layer_prep, dim3(L), dim3(64), 0, st, w.ext, S, spots->layers, m, w.layer);
-    return check_launch("k_layer_prep");
+static BeamCtx single_ctx(const dh_grid &g, const dh_beam_geom *geom, const Model &m, const dh_spot_set *ss,
+                          void *ws, float cutoff_sigma) {
+    BeamCtx c = {};
+    const BeamWs w = carve_beam(ws, grid_bricks(g), ss->n_spots, ss->n_layers, default_tune());
+    fill_pencil_ctx(c, g, *geom, m, *ss, w, cutoff_sigma);
+    return c;
 }
-
-static float brick_radius(const dh_grid &g) { return 0.5f * (BRICK - 1) * 1.7320508075688772f * g.spacing * 1.0001f; }
-
-template <bool COUNT>
-static int launch_forward(hipStream_t st, const float *wet, float *dose, const dh_grid &g, const dh_beam_geom *geom,
-                          const Model &m, const dh_spot_set *spots, const Workspace &w, float cutoff_sigma,
-                          int accumulate, unsigned long long *counts) {
-    const Grid gg = make_grid(g);
-    const Geom gb = make_geom(*geom);
-    const BrickGrid bg = brick_grid(gg);
-    const unsigned nb = (unsigned)((bg.count() + 3) / 4);
-    const int S = spots->n_spots;
-    const int nl = S > 0 ? spots->n_layers : 0;
-    if (cutoff_sigma > 0.0f)
-        hipLaunchKernelGGL((k_pb_forward<true, COUNT>), dim3(nb), dim3(256), 0, st, wet, dose, gg, gb, bg, nb, m,
-                           w.layer, nl, w.ext, cutoff_sigma, brick_radius(g), accumulate, counts);
-    else
-        hipLaunchKernelGGL((k_pb_forward<false, COUNT>), dim3(nb), dim3(256), 0, st, wet, dose, gg, gb, bg, nb, m,
-                           w.layer, nl, w.ext, 0.0f, brick_radius(g), accumulate, counts);
-    return check_launch(COUNT ? "k_pb_forward(count)" : "k_pb_forward");
-}
-
-template <bool COUNT>
-static int launch_adjoint(hipStream_t st, const float *wet, const float *grad_dose, const dh_grid &g,
-                          const dh_beam_geom *geom, const Model &m, const dh_spot_set *spots, const Workspace &w,
-                          float cutoff_sigma, unsigned long long *counts) {
-    const Grid gg = make_grid(g);
-    const Geom gb = make_geom(*geom);
-    const BrickGrid bg = brick_grid(gg);
-    const int S = spots->n_spots;
-    hipLaunchKernelGGL(k_brick_summary, dim3((unsigned)((bg.count() + 3) / 4)), dim3(256), 0, st, wet, grad_dose,
-                       gg, bg, w.bsum);
-    int rc = check_launch("k_brick_summary");
-    if (rc) return rc;
-    const int cs = adjoint_chunk(S, spots->n_layers);
-    const unsigned nch = (unsigned)(((long long)S + (cs - 1LL) * spots->n_layers) / cs + 1);
-    const dim3 grid(nch, w.nslab);
-    const float rb = brick_radius(g);
-    const bool cull = cutoff_sigma > 0.0f;
-    if (cs == 32) {
-        if (cull)
-            hipLaunchKernelGGL((k_pb_adjoint<true, 32, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
-                               w.layer, spots->n_layers, w.ext, S, cutoff_sigma, rb, w.nslab, w.bsum, w.partial,
-                               counts);
-        else
-            hipLaunchKernelGGL((k_pb_adjoint<false, 32, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
-                               w.layer, spots->n_layers, w.ext, S, 0.0f, rb, w.nslab, w.bsum, w.partial, counts);
-    } else {
-        if (cull)
-            hipLaunchKernelGGL((k_pb_adjoint<true, 64, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
-                               w.layer, spots->n_layers, w.ext, S, cutoff_sigma, rb, w.nslab, w.bsum, w.partial,
-                               counts);
-        else
-            hipLaunchKernelGGL((k_pb_adjoint<false, 64, COUNT>), grid, dim3(64), 0, st, wet, grad_dose, gg, gb, bg, m,
-                               w.layer, spots->n_layers, w.ext, S, 0.0f, rb, w.nslab, w.bsum, w.partial, counts);
-    }
-    return check_launch(COUNT ? "k_pb_adjoint(count)" : "k_pb_adjoint");
-}
-
-}  // namespace dh
-
-using namespace dh;
 
 extern "C" int64_t dh_pb_workspace_size(int32_t n_spots, int32_t n_layers, dh_grid g) {
     if (n_spots <= 0) return 0;
-    return carve(nullptr, g, n_spots, n_layers).bytes;
+    return carve_beam(nullptr, grid_bricks(g), n_spots, n_layers, default_tune()).bytes;
 }
 
 extern "C" int dh_pb_forward(const float *wet, float *dose, dh_grid g, const dh_beam_geom *geom,
@@ -841,12 +1045,12 @@ extern "C" int dh_pb_forward(const float *wet, float *dose, dh_grid g, const dh_
     Model m;
     if (!fill_model(model, m)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    const Workspace w = carve(workspace, g, spots->n_spots, spots->n_layers);
-    if (spots->n_spots > 0) {
-        const int rc = prepare(st, spots, geom, m, w);
-        if (rc) return rc;
-    }
-    return launch_forward<false>(st, wet, dose, g, geom, m, spots, w, cutoff_sigma, accumulate, nullptr);
+    BeamCtx c = single_ctx(g, geom, m, spots, workspace, cutoff_sigma);
+    c.wet = const_cast<float *>(wet);
+    c.dose = dose;
+    const int rc = prepare(st, c);
+    if (rc) return rc;
+    return launch_forward<false>(st, c, accumulate, nullptr);
 }
 
 extern "C" int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g, const dh_beam_geom *geom,
@@ -863,17 +1067,18 @@ extern "C" int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g
     Model m;
     if (!fill_model(model, m)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    const Workspace w = carve(workspace, g, S, spots->n_layers);
-    int rc = prepare(st, spots, geom, m, w);
+    BeamCtx c = single_ctx(g, geom, m, spots, workspace, cutoff_sigma);
+    c.wet = const_cast<float *>(wet);
+    c.gdose = grad_dose;
+    c.grad_mu = grad_mu;
+    c.grad_xy = grad_xy;
+    int rc = prepare(st, c);
     if (rc) return rc;
-    const long long np = (long long)w.nslab * S;
-    hipLaunchKernelGGL(k_zero4, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, w.partial, np);
-    rc = check_launch("k_zero4");
-    if (rc) return rc;
-    rc = launch_adjoint<false>(st, wet, grad_dose, g, geom, m, spots, w, cutoff_sigma, nullptr);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_pb_reduce, dim3((S + 63) / 64), dim3(256), 0, st, w.partial, w.ext, S, w.nslab, grad_mu,
-                       grad_xy);
+    const long long np = (long long)c.nslab * S;
+    hipLaunchKernelGGL(k_zero4, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, c.partial, np);
+    if ((rc = check_launch("k_zero4"))) return rc;
+    if ((rc = launch_adjoint<false>(st, c, nullptr))) return rc;
+    hipLaunchKernelGGL(k_pb_reduce, dim3((S + 63) / 64), dim3(256), 0, st, c);
     return check_launch("k_pb_reduce");
 }
 
@@ -886,15 +1091,16 @@ extern "C" int dh_pb_count_pairs(const float *wet, const float *grad_dose, dh_gr
         set_error("dh_pb_count_pairs: null pointer");
         return DH_ERR_ARG;
     }
-    const int S = spots->n_spots;
-    if (S == 0) return DH_OK;  // counts are accumulated into; nothing to add
+    if (spots->n_spots == 0) return DH_OK;  // counts are accumulated into; nothing to add
     Model m;
     if (!fill_model(model, m)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    const Workspace w = carve(workspace, g, S, spots->n_layers);
-    int rc = prepare(st, spots, geom, m, w);
+    BeamCtx c = single_ctx(g, geom, m, spots, workspace, cutoff_sigma);
+    c.wet = const_cast<float *>(wet);
+    c.gdose = grad_dose;
+    int rc = prepare(st, c);
     if (rc) return rc;
-    rc = launch_forward<true>(st, wet, nullptr, g, geom, m, spots, w, cutoff_sigma, 0, counts);
-    if (rc || !grad_dose) return rc;
-    return launch_adjoint<true>(st, wet, grad_dose, g, geom, m, spots, w, cutoff_sigma, counts);
+    if ((rc = launch_forward<true>(st, c, 0, counts))) return rc;
+    if (!grad_dose) return DH_OK;
+    return launch_adjoint<true>(st, c, counts);
 }

@@ -104,7 +104,7 @@ __device__ __forceinline__ float trilinear_quad(const float4 *__restrict__ qv, i
                                                 float cz) {
     const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
     const long long base = (long long)(int)fx + (long long)nx * (int)fy + (long long)nxy * (int)fz;
-    return blend_quads(qv[base], qv[base + nxy], cx - fx, cy - fy, cz - fz);
+    return blend_quads(gld(qv, base), gld(qv, base + nxy), cx - fx, cy - fy, cz - fz);
 }
 
 // Steps until the ray leaves [0, n) along one axis (real-valued).
@@ -133,18 +133,9 @@ __device__ __forceinline__ float trilinear_buf(__amdgpu_buffer_rsrc_t rs, float 
     return blend_quads(q0, q1, ax, ay, az);
 }
 
-__global__ __launch_bounds__(256) void k_wet_raymarch(const float4 *__restrict__ qv, float *__restrict__ wet, Grid g,
-                                                     Geom b, BrickGrid bg, unsigned nblocks) {
-    const unsigned blk = xcd_remap(blockIdx.x, nblocks);
-    const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
-    if (brick >= bg.count()) return;
-    const int lane = threadIdx.x & 63;
-    const int bi = (int)(brick % bg.bx);
-    const long long bt = brick / bg.bx;
-    const int bj = (int)(bt % bg.by), bk = (int)(bt / bg.by);
-    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
-    if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-
+// Raw WET of voxel (i, j, k) of a beam; the caller guarantees (i, j, k) inside.
+__device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, const Grid &g, const Geom &b, int i,
+                                                int j, int k) {
     const float3 v = voxel_xyz(g, b, i, j, k);
     const float dx = b.src[0] - v.x, dy = b.src[1] - v.y, dz = b.src[2] - v.z;
     // correctly rounded sqrt and divisions: an axis-aligned ray gets an exactly
@@ -190,7 +181,36 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(const float4 *__restrict__
                             cz < (float)g.nz;
         if (inside) acc = fmaf(fmaxf(trilinear_quad(qv, g.nx, nxy, cx, cy, cz), 0.0f), 0.1f, acc);
     }
-    wet[i + (long long)g.nx * (j + (long long)g.ny * k)] = acc;
+    return acc;
+}
+
+// One wave per 4x4x4 brick, lane = voxel.
+__device__ __forceinline__ void raymarch_brick(const BeamCtx &c, long long brick, int lane) {
+    const int bi = (int)(brick % c.bg.bx);
+    const long long bt = brick / c.bg.bx;
+    const int bj = (int)(bt % c.bg.by), bk = (int)(bt / c.bg.by);
+    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
+    if (i >= c.g.nx || j >= c.g.ny || k >= c.g.nz) return;
+    gst(c.wet_raw, i + (long long)c.g.nx * (j + (long long)c.g.ny * k), raymarch_voxel(c.rlsp_q, c.g, c.b, i, j, k));
+}
+
+__global__ __launch_bounds__(256) void k_wet_raymarch(BeamCtx c, unsigned nblocks) {
+    const unsigned blk = xcd_remap(blockIdx.x, nblocks);
+    const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
+    if (brick >= c.bg.count()) return;
+    raymarch_brick(c, brick, threadIdx.x & 63);
+}
+
+// Batched: bricks of every beam in one launch; beam b owns bricks
+// [off[b], off[b + 1]) of the item space.
+__global__ __launch_bounds__(256) void k_wet_raymarch_batch(const BeamCtx *__restrict__ ctx,
+                                                           const long long *__restrict__ off, int n_beams,
+                                                           long long base, long long total) {
+    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
+    if (item >= total) return;
+    const int bm = find_owner(off, n_beams, item);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    raymarch_brick(c, item - off[bm], threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -209,51 +229,75 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(const float4 *__restrict__
 // float64) evaluation of the rotations would break exact half-voxel ties at
 // random, and such ties are common (gantry angles that are multiples of 30 deg
 // put the 60-deg directions on rational offsets).
-constexpr int N_STENCIL = 54;
-
-struct Stencil {
-    // per entry: (dx, dy, dz) integer base offsets, tie bits (1 = x, 2 = y, 4 = z), dr
-    signed char dx[N_STENCIL], dy[N_STENCIL], dz[N_STENCIL];
-    unsigned char tie[N_STENCIL], dr[N_STENCIL];
-};
-
 __device__ __forceinline__ int stencil_axis(int i, int base, bool tie) {
     const int n = i + base;
     return tie ? n + (n & 1) : n;
 }
 
-__global__ __launch_bounds__(256) void k_wet_smooth(const float *__restrict__ raw, float *__restrict__ out, Grid g,
-                                                   Stencil st, long long nvox) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nvox) return;
+// Stencil entry e, from a kernel-argument copy or from a table in global memory.
+struct StencilEntry {
+    int dx, dy, dz;
+    unsigned tie;
+    float dr;
+};
+
+__device__ __forceinline__ StencilEntry stencil_entry(const Stencil &st, int e) {
+    return StencilEntry{st.dx[e], st.dy[e], st.dz[e], st.tie[e], (float)st.dr[e]};
+}
+
+__device__ __forceinline__ StencilEntry stencil_entry(const Stencil *st, int e) {
+    return StencilEntry{gld(st->dx, e), gld(st->dy, e), gld(st->dz, e), gld(st->tie, e), (float)gld(st->dr, e)};
+}
+
+template <class ST>
+__device__ __forceinline__ float smooth_voxel(const float *__restrict__ raw, const Grid &g, const ST &st, long long t) {
     const int i = (int)(t % g.nx);
     const long long r = t / g.nx;
     const int j = (int)(r % g.ny), k = (int)(r / g.ny);
-
-    const float wc = raw[t];
+    const float wc = gld(raw, t);
     const float max_dr = fminf(wc * 10.0f, 10.0f);
     float sum = wc;
     int cnt = 1;
     // entries are ordered direction-major, dr = 1..9 within a direction
     for (int a = 0; a < 6; ++a) {
         for (int d = 0; d < 9; ++d) {
-            const int e = a * 9 + d;
-            if (!((float)st.dr[e] < max_dr)) break;
-            const unsigned tb = st.tie[e];
-            const int ni = stencil_axis(i, st.dx[e], tb & 1u);
-            const int nj = stencil_axis(j, st.dy[e], tb & 2u);
-            const int nk = stencil_axis(k, st.dz[e], tb & 4u);
+            const StencilEntry se = stencil_entry(st, a * 9 + d);
+            if (!(se.dr < max_dr)) break;
+            const int ni = stencil_axis(i, se.dx, se.tie & 1u);
+            const int nj = stencil_axis(j, se.dy, se.tie & 2u);
+            const int nk = stencil_axis(k, se.dz, se.tie & 4u);
             if ((unsigned)ni < (unsigned)g.nx && (unsigned)nj < (unsigned)g.ny && (unsigned)nk < (unsigned)g.nz) {
-                sum += raw[ni + (long long)g.nx * (nj + (long long)g.ny * nk)];
+                sum += gld(raw, ni + (long long)g.nx * (nj + (long long)g.ny * nk));
                 ++cnt;
             }
         }
     }
-    out[t] = sum / (float)cnt;
+    return sum / (float)cnt;
+}
+
+__global__ __launch_bounds__(256) void k_wet_smooth(const float *__restrict__ raw, float *__restrict__ out, Grid g,
+                                                   Stencil st, long long nvox) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvox) return;
+    out[t] = smooth_voxel(raw, g, st, t);
+}
+
+// Batched: beam b owns blocks [boff[b], boff[b + 1]) of 256 voxels each.
+__global__ __launch_bounds__(256) void k_wet_smooth_batch(const BeamCtx *__restrict__ ctx,
+                                                         const long long *__restrict__ boff, int n_beams,
+                                                         long long base, long long total_blocks) {
+    const long long blk = base + blockIdx.x;
+    if (blk >= total_blocks) return;
+    const int bm = find_owner(boff, n_beams, blk);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    const long long t = (blk - boff[bm]) * 256 + threadIdx.x;
+    const long long nvox = (long long)c.g.nx * c.g.ny * c.g.nz;
+    if (t >= nvox) return;
+    gst(c.wet, t, smooth_voxel(c.wet_raw, c.g, c.stencil, t));
 }
 
 // Host: tabulate the smoothing stencil of a beam.
-static void make_stencil(const dh_beam_geom &b, double s, Stencil &st) {
+void make_stencil(const dh_beam_geom &b, double s, Stencil &st) {
     static const double kc[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
     static const double ks[6] = {0.0, 0.86602540378443864676, 0.86602540378443864676,
                                  0.0, -0.86602540378443864676, -0.86602540378443864676};
@@ -287,7 +331,7 @@ static void make_stencil(const dh_beam_geom &b, double s, Stencil &st) {
     }
 }
 
-static bool grid_ok(const dh_grid &g) {
+bool grid_ok(const dh_grid &g) {
     if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0.0f)) {
         set_error("invalid grid %d x %d x %d, spacing %g", g.nx, g.ny, g.nz, (double)g.spacing);
         return false;
@@ -349,17 +393,44 @@ extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid
         set_error("dh_wet_raymarch: null pointer");
         return DH_ERR_ARG;
     }
-    const Grid gg = make_grid(g);
-    const BrickGrid bg = brick_grid(gg);
-    const unsigned nb = (unsigned)((bg.count() + 3) / 4);
     if (((uintptr_t)rlsp_padded & 15u) != 0) {
         set_error("dh_wet_raymarch: the quad RLSP volume must be 16-byte aligned");
         return DH_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_wet_raymarch, dim3(nb), dim3(256), 0, (hipStream_t)stream,
-                       reinterpret_cast<const float4 *>(rlsp_padded), wet_raw, gg, make_geom(*geom), bg, nb);
+    BeamCtx c = {};
+    c.rlsp_q = reinterpret_cast<const float4 *>(rlsp_padded);
+    c.wet_raw = wet_raw;
+    c.g = make_grid(g);
+    c.b = make_geom(*geom);
+    c.bg = brick_grid(c.g);
+    const unsigned nb = (unsigned)((c.bg.count() + 3) / 4);
+    hipLaunchKernelGGL(k_wet_raymarch, dim3(nb), dim3(256), 0, (hipStream_t)stream, c, nb);
     return check_launch("k_wet_raymarch");
 }
+
+// Batched launches: ctx is a device table of n_beams entries.
+namespace dh {
+int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
+                     long long total_bricks, long long total_blocks, hipStream_t st) {
+    // item spaces above 2^24 bricks go out in several launches
+    constexpr long long kMaxItems = 1LL << 24;
+    for (long long base = 0; base < total_bricks; base += kMaxItems) {
+        const long long n = total_bricks - base < kMaxItems ? total_bricks - base : kMaxItems;
+        hipLaunchKernelGGL(k_wet_raymarch_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
+                           n_beams, base, total_bricks);
+        const int rc = check_launch("k_wet_raymarch_batch");
+        if (rc) return rc;
+    }
+    for (long long base = 0; base < total_blocks; base += kMaxItems / 4) {
+        const long long n = total_blocks - base < kMaxItems / 4 ? total_blocks - base : kMaxItems / 4;
+        hipLaunchKernelGGL(k_wet_smooth_batch, dim3((unsigned)n), dim3(256), 0, st, ctx, block_off, n_beams, base,
+                           total_blocks);
+        const int rc = check_launch("k_wet_smooth_batch");
+        if (rc) return rc;
+    }
+    return DH_OK;
+}
+}  // namespace dh
 
 extern "C" int dh_wet_smooth(const float *wet_raw, float *wet, dh_grid g, const dh_beam_geom *geom, void *stream) {
     if (!grid_ok(g)) return DH_ERR_ARG;

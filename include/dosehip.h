@@ -177,6 +177,103 @@ int dh_pb_count_pairs(const float *wet, const float *grad_dose, dh_grid g,
                       unsigned long long *counts,
                       void *workspace, int64_t workspace_bytes, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Batched plans.  Many plans (each its own dose grid, 1..n beams, any spot
+ * counts) computed together: every phase is one launch over the work items of
+ * all beams (bricks, spot chunks x brick slabs), so small and large plans fill
+ * the GPU together.  The reference computes one beam per call
+ * (computeIMPTPlan, plan_impt.py:170-206; proton_spot_cuda, IMPTClasses.cu:301-340);
+ * a batch gives the same per-plan dose (beam doses added in beam order), and
+ * the same gradients as dh_pb_adjoint per beam.
+ *
+ * Use: dh_batch_workspace_size -> allocate -> dh_batch_setup (uploads the
+ * descriptor tables, synchronises `stream`, not capturable) -> any number of
+ * dh_batch_wet / dh_batch_forward / dh_batch_adjoint calls.  Those three only
+ * enqueue kernels that read the tables in the workspace: no allocation, copy,
+ * memset or synchronisation, so they may be captured in a HIP graph and
+ * replayed.  Spot weights and positions are read from the device spot arrays
+ * by every forward / adjoint call, so updating them between replays is
+ * enough; sizes, pointers and geometry are fixed at setup.
+ */
+
+/* A plan of a batch. */
+typedef struct dh_batch_plan {
+    dh_grid grid;
+    const float *rlsp_padded;  /* quad RLSP (dh_padded_len floats, 16-byte aligned); dh_batch_wet input */
+    float *dose;               /* [nvox] plan dose, written by dh_batch_forward */
+    const float *grad_dose;    /* [nvox] dL/dDose, read by dh_batch_adjoint (may be NULL if unused) */
+    int32_t beam_start;        /* beams [beam_start, beam_start + n_beams) of the beam array; */
+    int32_t n_beams;           /* plans list their beams in order, without gaps */
+} dh_batch_plan;
+
+/* A beam of a batch. */
+typedef struct dh_batch_beam {
+    dh_beam_geom geom;
+    dh_beam_model model;       /* device lookup tables (may be shared by beams) */
+    dh_spot_set spots;         /* device spot and layer records, as for dh_pb_forward */
+    float *wet;                /* [nvox] smoothed WET (cm): written by dh_batch_wet, read by forward/adjoint */
+    float *wet_raw;            /* [nvox] raw WET, or NULL for workspace scratch */
+    float *dose;               /* [nvox] this beam's dose, or NULL: the plan dose for a one-beam plan,
+                                  workspace scratch otherwise */
+    float *grad_mu;            /* [n_spots] dL/dmu (sorted spot order) */
+    float *grad_xy;            /* [2 n_spots] dL/dx, dL/dy */
+} dh_batch_beam;
+
+/* Adjoint schedule, chosen per batch (e.g. by an autotuner):
+ *   adj_cs          spots per adjoint workgroup: 32, 64, or 0 (chosen per beam from its layer sizes)
+ *   adj_wg_target   workgroups per beam to aim for (bricks are cut into slabs to reach it)
+ *   adj_min_bricks  fewest bricks per slab
+ *   adj_max_slabs   most slabs per beam (<= 65535)
+ * Zero fields take the defaults (0, 32768, 64, 1024). */
+typedef struct dh_batch_tune {
+    int32_t adj_cs;
+    int32_t adj_wg_target;
+    int32_t adj_min_bricks;
+    int32_t adj_max_slabs;
+} dh_batch_tune;
+
+/* Filled by dh_batch_setup; plain data owned by the caller, read-only to it.
+ * The fields describe the workspace layout and the item counts of each phase. */
+typedef struct dh_batch {
+    void *ws;
+    int64_t ws_bytes;
+    int32_t n_plans, n_beams;
+    float cutoff_sigma;
+    int32_t magic;
+    int64_t n_bricks;          /* ray-march / forward / brick-summary items (bricks of all beams) */
+    int64_t n_smooth_blocks;   /* 256-voxel smoothing blocks of all beams */
+    int64_t n_spots, n_layers; /* prepared spot and layer records */
+    int64_t n_sum_blocks;      /* 256-voxel blocks of plans with several beams */
+    int64_t n_partials;        /* adjoint partial records */
+    int64_t n_adj32, n_adj64;  /* adjoint workgroups of 32- and 64-spot chunks */
+    int64_t n_reduce_blocks;   /* 64-spot reduction blocks */
+    int64_t tab[16];           /* byte offsets of the device tables in ws */
+} dh_batch;
+
+/* Workspace bytes for a batch (negative on invalid descriptors; see dh_last_error). */
+int64_t dh_batch_workspace_size(const dh_batch_plan *plans, int32_t n_plans,
+                                const dh_batch_beam *beams, int32_t n_beams,
+                                const dh_batch_tune *tune);
+
+/* Validate the descriptors, lay out the workspace and upload the per-beam
+ * tables (ray-march, smoothing stencil, pencil-beam and adjoint contexts).
+ * cutoff_sigma as for dh_pb_forward.  tune may be NULL (defaults). */
+int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32_t n_plans,
+                   const dh_batch_beam *beams, int32_t n_beams, const dh_batch_tune *tune,
+                   float cutoff_sigma, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* WET of every beam: ray-march then smoothing (dh_wet_raymarch + dh_wet_smooth per beam). */
+int dh_batch_wet(const dh_batch *batch, void *stream);
+
+/* Dose of every plan (dh_pb_forward per beam, beam doses added in beam order). */
+int dh_batch_forward(const dh_batch *batch, void *stream);
+
+/* Spot gradients of every beam from its plan's grad_dose (dh_pb_adjoint per beam).
+ * flags & DH_BATCH_REUSE_PREP: reuse the spot and layer records the last
+ * dh_batch_forward prepared (spot arrays unchanged since). */
+#define DH_BATCH_REUSE_PREP 1
+int dh_batch_adjoint(const dh_batch *batch, int32_t flags, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
