@@ -52,6 +52,35 @@ class dh_spot_set(ctypes.Structure):
                 ("n_spots", ctypes.c_int32), ("n_layers", ctypes.c_int32)]
 
 
+class dh_batch_plan(ctypes.Structure):
+    _fields_ = [("grid", dh_grid), ("rlsp_padded", ctypes.c_void_p), ("dose", ctypes.c_void_p),
+                ("grad_dose", ctypes.c_void_p), ("beam_start", ctypes.c_int32), ("n_beams", ctypes.c_int32)]
+
+
+class dh_batch_beam(ctypes.Structure):
+    _fields_ = [("geom", dh_beam_geom), ("model", dh_beam_model), ("spots", dh_spot_set),
+                ("wet", ctypes.c_void_p), ("wet_raw", ctypes.c_void_p), ("dose", ctypes.c_void_p),
+                ("grad_mu", ctypes.c_void_p), ("grad_xy", ctypes.c_void_p)]
+
+
+class dh_batch_tune(ctypes.Structure):
+    _fields_ = [("adj_cs", ctypes.c_int32), ("adj_wg_target", ctypes.c_int32),
+                ("adj_min_bricks", ctypes.c_int32), ("adj_max_slabs", ctypes.c_int32)]
+
+
+class dh_batch(ctypes.Structure):
+    _fields_ = [("ws", ctypes.c_void_p), ("ws_bytes", ctypes.c_int64),
+                ("n_plans", ctypes.c_int32), ("n_beams", ctypes.c_int32),
+                ("cutoff_sigma", ctypes.c_float), ("magic", ctypes.c_int32),
+                ("n_bricks", ctypes.c_int64), ("n_smooth_blocks", ctypes.c_int64),
+                ("n_spots", ctypes.c_int64), ("n_layers", ctypes.c_int64),
+                ("n_sum_blocks", ctypes.c_int64), ("n_partials", ctypes.c_int64),
+                ("n_adj32", ctypes.c_int64), ("n_adj64", ctypes.c_int64),
+                ("n_reduce_blocks", ctypes.c_int64), ("tab", ctypes.c_int64 * 16)]
+
+
+DH_BATCH_REUSE_PREP = 1
+
 # exported symbol -> (restype, argtypes); every declaration of include/dosehip.h
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -73,6 +102,15 @@ SIGNATURES = {
     "dh_pb_count_pairs": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom),
                                          ctypes.POINTER(dh_beam_model), ctypes.POINTER(dh_spot_set),
                                          ctypes.c_float, _P, _P, ctypes.c_int64, _P]),
+    "dh_batch_workspace_size": (ctypes.c_int64, [ctypes.POINTER(dh_batch_plan), ctypes.c_int32,
+                                                 ctypes.POINTER(dh_batch_beam), ctypes.c_int32,
+                                                 ctypes.POINTER(dh_batch_tune)]),
+    "dh_batch_setup": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.POINTER(dh_batch_plan), ctypes.c_int32,
+                                      ctypes.POINTER(dh_batch_beam), ctypes.c_int32, ctypes.POINTER(dh_batch_tune),
+                                      ctypes.c_float, _P, ctypes.c_int64, _P]),
+    "dh_batch_wet": (ctypes.c_int, [ctypes.POINTER(dh_batch), _P]),
+    "dh_batch_forward": (ctypes.c_int, [ctypes.POINTER(dh_batch), _P]),
+    "dh_batch_adjoint": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
 }
 
 _lib = None

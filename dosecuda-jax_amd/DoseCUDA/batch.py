@@ -1,0 +1,210 @@
+"""Batched plans: many (dose grid, plan) pairs computed together.
+
+``BatchedDose([engine_0, engine_1, ...])`` gathers the device state of several
+``IMPTDoseEngine`` objects (plans of any grid size, beam count and spot count)
+into one batch of the C ABI (dh_batch_setup).  Each phase then runs as a
+single launch over the work items of every beam of every plan:
+
+* ``compute_wet()``    ray-march + smoothing of every beam (dh_batch_wet)
+* ``forward()``        dose of every plan (dh_batch_forward)
+* ``adjoint(grads)``   dL/dweight and dL/d(x, y) of every beam (dh_batch_adjoint)
+
+The reference computes one beam per extension call and one plan per
+``computeIMPTPlan`` (plan_impt.py:170-206); per plan the results here are the
+same as the engine's (beam doses added in beam order, times n_fractions).
+
+The phase calls only enqueue kernels that read tables prepared at set-up, so
+``GraphedStep`` can capture a whole optimisation step (forward, a loss
+gradient written with torch element-wise ops, adjoint) in one HIP graph and
+replay it; spot weights and positions are read from device tensors at every
+replay.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import device as D
+
+
+def make_tune(adj_cs=0, adj_wg_target=0, adj_min_bricks=0, adj_max_slabs=0):
+    """dh_batch_tune; zero fields take the library defaults (0, 32768, 64, 1024)."""
+    return N.dh_batch_tune(int(adj_cs), int(adj_wg_target), int(adj_min_bricks), int(adj_max_slabs))
+
+
+def tune_dict(t):
+    return {"adj_cs": t.adj_cs, "adj_wg_target": t.adj_wg_target, "adj_min_bricks": t.adj_min_bricks,
+            "adj_max_slabs": t.adj_max_slabs}
+
+
+class BatchedDose:
+    """Device state of a batch of plans and the three batched phases.
+
+    engines: IMPTDoseEngine objects on one device (their HU -> RLSP volumes,
+    beam models and sorted spots are used as they are).
+    keep_beam_doses: give every beam a dose volume of its own (``beam_doses``),
+    as the reference's ``IMPTDoseGrid.beam_doses``; otherwise beams of
+    multi-beam plans write workspace scratch.
+    tune: dh_batch_tune (adjoint schedule) or None for the defaults.
+    """
+
+    def __init__(self, engines, cutoff_sigma=None, tune=None, keep_beam_doses=False):
+        if not engines:
+            raise ValueError("BatchedDose needs at least one plan")
+        self.lib = N.load()
+        self.engines = list(engines)
+        self.device = self.engines[0].device
+        for e in self.engines:
+            if e.device != self.device:
+                raise ValueError("all plans of a batch must live on one device")
+        self.cutoff_sigma = float(self.engines[0].cutoff_sigma if cutoff_sigma is None else cutoff_sigma)
+        self.tune = tune if tune is not None else make_tune()
+        dev = self.device
+        self.doses, self.grad_doses, self.beam_doses = [], [], []
+        self.beam_grads = []   # per beam (grad_mu [S], grad_xy [S, 2]) in sorted spot order
+        plans = (N.dh_batch_plan * len(self.engines))()
+        n_beams = sum(len(e.beams) for e in self.engines)
+        beams = (N.dh_batch_beam * max(n_beams, 1))()
+        k = 0
+        with torch.cuda.device(dev):
+            for p, e in enumerate(self.engines):
+                dose = torch.zeros(e.shape, dtype=torch.float32, device=dev)
+                gdose = torch.zeros(e.shape, dtype=torch.float32, device=dev)
+                self.doses.append(dose)
+                self.grad_doses.append(gdose)
+                plans[p].grid = e.grid
+                plans[p].rlsp_padded = N.ptr(e.rlsp_padded).value
+                plans[p].dose = N.ptr(dose).value
+                plans[p].grad_dose = N.ptr(gdose).value
+                plans[p].beam_start = k
+                plans[p].n_beams = len(e.beams)
+                for b in e.beams:
+                    if b.wet is None:
+                        b.wet = torch.zeros(e.shape, dtype=torch.float32, device=dev)
+                    S = b.spots.n_spots
+                    gmu = torch.zeros(max(S, 1), dtype=torch.float32, device=dev)[:S]
+                    gxy = torch.zeros((max(S, 1), 2), dtype=torch.float32, device=dev)[:S]
+                    self.beam_grads.append((gmu, gxy))
+                    bd = None
+                    if keep_beam_doses:
+                        bd = torch.zeros(e.shape, dtype=torch.float32, device=dev)
+                    self.beam_doses.append(bd)
+                    B = beams[k]
+                    B.geom = b.geo
+                    B.model = b.dmodel.c
+                    B.spots = b.spots.c
+                    B.wet = N.ptr(b.wet).value
+                    B.wet_raw = N.ptr(b.wet_raw).value if b.wet_raw is not None else None
+                    B.dose = N.ptr(bd).value if bd is not None else None
+                    B.grad_mu = N.ptr(gmu).value if S else None
+                    B.grad_xy = N.ptr(gxy).value if S else None
+                    k += 1
+            self._plans, self._beams = plans, beams
+            nbytes = int(self.lib.dh_batch_workspace_size(plans, len(self.engines), beams, n_beams,
+                                                          ctypes.byref(self.tune)))
+            if nbytes < 0:
+                N.check(nbytes, "dh_batch_workspace_size")
+            self.ws_bytes = nbytes
+            self.ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+            self.batch = N.dh_batch()
+            rc = self.lib.dh_batch_setup(ctypes.byref(self.batch), plans, len(self.engines), beams, n_beams,
+                                         ctypes.byref(self.tune), self.cutoff_sigma, N.ptr(self.ws), nbytes,
+                                         N.stream_handle(dev))
+            N.check(rc, "dh_batch_setup")
+        self.n_beams = n_beams
+        self._wet_done = False
+
+    # -- phases -------------------------------------------------------------------
+    def _stream(self):
+        return N.stream_handle(self.device)
+
+    def compute_wet(self, force=False):
+        """WET of every beam (written into each engine beam's ``wet``)."""
+        if self._wet_done and not force:
+            return
+        N.check(self.lib.dh_batch_wet(ctypes.byref(self.batch), self._stream()), "dh_batch_wet")
+        self._wet_done = True
+        for e in self.engines:
+            for b in e.beams:
+                b.wet_valid = True
+
+    def set_spots(self, mus=None, xys=None):
+        """Per-beam weights / positions (lists over all beams of the batch, in
+        each beam's spot_list row order); None entries keep the current values."""
+        k = 0
+        for e in self.engines:
+            for b in e.beams:
+                mu = None if mus is None else mus[k]
+                xy = None if xys is None else xys[k]
+                if mu is not None or xy is not None:
+                    b.spots.set_weights_positions(mu, xy)
+                k += 1
+
+    def forward(self, mus=None, xys=None):
+        """Plan doses (list of (z, y, x) tensors, times n_fractions)."""
+        self.compute_wet()
+        self.set_spots(mus, xys)
+        N.check(self.lib.dh_batch_forward(ctypes.byref(self.batch), self._stream()), "dh_batch_forward")
+        for e, d in zip(self.engines, self.doses):
+            if e.n_fractions != 1:
+                d.mul_(float(e.n_fractions))
+        return self.doses
+
+    def adjoint(self, grad_doses=None, reuse_prep=False, input_order=True):
+        """Spot gradients of every beam for dL/dD = grad_doses[p] of plan p
+        (None: the batch's own ``grad_doses`` tensors, filled anew before each
+        call; plans with n_fractions != 1 scale them in place).
+        Returns [(dL/dmu [S], dL/d(x, y) [S, 2])] per beam."""
+        self.compute_wet()
+        for p, e in enumerate(self.engines):
+            g = self.grad_doses[p]
+            if grad_doses is not None:
+                g.copy_(grad_doses[p])
+            if e.n_fractions != 1:
+                g.mul_(float(e.n_fractions))
+        flags = N.DH_BATCH_REUSE_PREP if reuse_prep else 0
+        N.check(self.lib.dh_batch_adjoint(ctypes.byref(self.batch), flags, self._stream()), "dh_batch_adjoint")
+        if not input_order:
+            return self.beam_grads
+        out = []
+        k = 0
+        for e in self.engines:
+            for b in e.beams:
+                gmu, gxy = self.beam_grads[k]
+                out.append((b.spots.to_input_order(gmu), b.spots.to_input_order(gxy)))
+                k += 1
+        return out
+
+    def item_counts(self):
+        """Work items of each phase (from the set-up tables)."""
+        b = self.batch
+        return {"bricks": b.n_bricks, "smooth_blocks": b.n_smooth_blocks, "spots": b.n_spots,
+                "layers": b.n_layers, "sum_blocks": b.n_sum_blocks, "partials": b.n_partials,
+                "adjoint_wg_32": b.n_adj32, "adjoint_wg_64": b.n_adj64, "reduce_blocks": b.n_reduce_blocks,
+                "workspace_bytes": self.ws_bytes}
+
+
+class GraphedStep:
+    """Capture ``fn()`` (device work only: batched phases and torch ops, no
+    host synchronisation) in a HIP graph; ``replay()`` reruns it and returns
+    what ``fn`` returned at capture (tensors refreshed in place)."""
+
+    def __init__(self, fn, device, warmup=2):
+        self.device = device
+        side = torch.cuda.Stream(device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                fn()
+        torch.cuda.current_stream(device).wait_stream(side)
+        torch.cuda.synchronize(device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = fn()
+        torch.cuda.synchronize(device)
+
+    def replay(self):
+        self.graph.replay()
+        return self.out

@@ -44,12 +44,27 @@ class BeamState:
                                                beam.gantry_angle, beam.couch_angle, model.VSADX, model.VSADY))
         sl = np.asarray(beam.spot_list, dtype=np.float32).reshape(-1, 4)
         self.spots = D.DeviceSpots(sl, model.divergence_params, engine.device, spatial=engine.spatial_sort)
-        self.ws, self.ws_bytes = D.alloc_workspace(self.spots.n_spots, self.spots.n_layers, engine.grid,
-                                                   engine.device)
+        self._grid = engine.grid
+        self._ws = None
         self.mu0 = torch.from_numpy(sl[:, 2].copy()).to(engine.device)
         self.xy0 = torch.from_numpy(sl[:, :2].copy()).to(engine.device)
         self.wet_raw = None
         self.wet = None
+        self.wet_valid = False   # wet holds this beam's WET (set by whoever computed it)
+
+    def _workspace(self):
+        # single-beam pencil-beam workspace, allocated on first use (batches bring their own)
+        if self._ws is None:
+            self._ws = D.alloc_workspace(self.spots.n_spots, self.spots.n_layers, self._grid, self.spots.device)
+        return self._ws
+
+    @property
+    def ws(self):
+        return self._workspace()[0]
+
+    @property
+    def ws_bytes(self):
+        return self._workspace()[1]
 
 
 class IMPTDoseEngine:
@@ -81,9 +96,10 @@ class IMPTDoseEngine:
     # -- WET -----------------------------------------------------------------
     def compute_wet(self, force=False):
         for b in self.beams:
-            if b.wet is None or force:
+            if not b.wet_valid or force:
                 b.wet_raw, b.wet = D.wet_from_padded(self.rlsp_padded, self.grid, b.geo, self.device,
                                                      raw_out=b.wet_raw, out=b.wet)
+                b.wet_valid = True
         return [b.wet for b in self.beams]
 
     # -- forward / adjoint on explicit tensors --------------------------------
@@ -103,6 +119,8 @@ class IMPTDoseEngine:
     def dose(self, mus=None, xys=None, out=None):
         """Plan dose: sum of beams times n_fractions."""
         out = out if out is not None else torch.empty(self.shape, dtype=torch.float32, device=self.device)
+        if not self.beams:
+            return out.zero_()
         for k in range(len(self.beams)):
             self.beam_dose(k, None if mus is None else mus[k], None if xys is None else xys[k], out=out,
                            accumulate=k > 0)

@@ -187,6 +187,12 @@ struct AdjTune {
     int cs, wg_target, min_bricks, max_slabs;
 };
 
+// Number of cs-spot adjoint chunks provided for a beam: an upper bound of the
+// sum over layers of ceil(n_l / cs).
+__host__ __device__ inline long long adjoint_chunks(int n_spots, int n_layers, int cs) {
+    return ((long long)n_spots + (cs - 1LL) * n_layers) / cs + 1;
+}
+
 // Pencil-beam workspace of one beam (device memory).
 struct BeamWs {
     SpotExt *ext;

@@ -461,9 +461,10 @@ __global__ __launch_bounds__(256) void k_pb_forward_batch(const BeamCtx *__restr
     if (vo.valid) gst(c.dose, vo.idx, acc);
 }
 
-// Dose of a plan with several beams: the sum of its beams' volumes, added in
-// beam order (the order of single-beam calls with accumulate = 1).  Plan p owns
-// blocks [off[p], off[p + 1]) of 256 voxels.
+// Dose of a plan whose beams write volumes of their own: the sum of the beam
+// volumes, added in beam order (the order of single-beam calls with
+// accumulate = 1); zero for a plan without beams.  Plan p owns blocks
+// [off[p], off[p + 1]) of 256 voxels.
 __global__ __launch_bounds__(256) void k_plan_sum(const PlanCtx *__restrict__ plans, const BeamCtx *__restrict__ ctx,
                                                  const long long *__restrict__ off, int n_plans) {
     const long long blk = blockIdx.x;
@@ -471,8 +472,8 @@ __global__ __launch_bounds__(256) void k_plan_sum(const PlanCtx *__restrict__ pl
     const PlanCtx P = plans[p];
     const long long t = (blk - off[p]) * 256 + threadIdx.x;
     if (t >= P.nvox) return;
-    float d = gld(ctx[P.beam0].dose, t);
-    for (int b = P.beam0 + 1; b < P.beam0 + P.n_beams; ++b) d += gld(ctx[b].dose, t);
+    float d = 0.0f;  // 0 + D_0 == D_0: the same sums as beam-by-beam accumulation
+    for (int b = P.beam0; b < P.beam0 + P.n_beams; ++b) d += gld(ctx[b].dose, t);
     gst(P.dose, t, d);
 }
 
@@ -527,12 +528,6 @@ struct LdsVox {
     float a2, GA, GB, GAa;
     float GBb, pad0, pad1, pad2;
 };
-
-// Number of CS-spot chunks the grid provides for a beam (an upper bound of
-// sum over layers of ceil(n_l / CS)).
-__host__ __device__ inline long long adjoint_chunks(int n_spots, int n_layers, int cs) {
-    return ((long long)n_spots + (cs - 1LL) * n_layers) / cs + 1;
-}
 
 template <bool CULL, int CS, bool COUNT>
 __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab, int lane, LdsVox *s_vox,

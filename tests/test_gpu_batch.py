@@ -1,0 +1,191 @@
+"""Batched plans (dh_batch_*) vs the single-beam path, and HIP-graph replay (GPU).
+
+The batched kernels run the same per-item device code as the single-beam ones
+and add beam doses in the same order, so WET, dose and gradients must agree
+BIT FOR BIT with the single-beam path, which the oracle tests pin
+(test_gpu_parity.py).  Other adjoint schedules (chunk size, slab count) change
+only the order of the partial sums: <= 1e-5 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[gpu-batch] " + msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda", 0)
+
+
+def _phantom(shape, spacing, seed):
+    from DoseCUDA.workloads import _grid
+    g = _grid(shape, spacing)
+    rng = np.random.default_rng(seed)
+    hu = rng.uniform(-200.0, 300.0, size=shape).astype(np.float32)
+    hu[:, shape[1] // 5: shape[1] // 5 + 4, :] = 1200.0
+    hu[3:6, :, 4:7] = -1000.0
+    g.HU = hu
+    return g
+
+
+def _plan(gantries, couches, layers, radius=12.0, pitch=3.0, seed=5, n_fractions=1):
+    from DoseCUDA import IMPTPlan
+    from DoseCUDA.workloads import _beam, grid_spots
+    rng = np.random.default_rng(seed)
+    plan = IMPTPlan()
+    plan.n_fractions = n_fractions
+    c = np.arange(-radius, radius + 1e-6, pitch)
+    for ga, ca in zip(gantries, couches):
+        sl = grid_spots(layers, c, c, disk_radius=radius)
+        sl[:, 2] = rng.uniform(0.2, 1.0, size=sl.shape[0]).astype(np.float32)
+        plan.addBeam(_beam(ga, sl, couch=ca))
+    return plan
+
+
+def _engines(dev, cutoff=6.0):
+    """Four plans of different grids: one beam (water slab), two oblique beams
+    with n_fractions = 2, three beams on a small grid, and a plan without beams."""
+    from DoseCUDA import IMPTPlan
+    from DoseCUDA.engine import IMPTDoseEngine
+    from DoseCUDA.workloads import water_slab_plan
+    out = []
+    g0, p0 = water_slab_plan(n=40, spacing=2.5)
+    out.append(IMPTDoseEngine(g0, p0, cutoff_sigma=cutoff))
+    g1 = _phantom((40, 56, 48), 2.5, 3)
+    out.append(IMPTDoseEngine(g1, _plan([120.0, 240.0], [30.0, 0.0], [22, 27, 31], n_fractions=2),
+                              cutoff_sigma=cutoff))
+    g2 = _phantom((24, 28, 20), 2.0, 7)
+    out.append(IMPTDoseEngine(g2, _plan([0.0, 90.0, 200.0], [0.0, 0.0, 15.0], [20, 24], radius=6.0, seed=9),
+                              cutoff_sigma=cutoff))
+    g3 = _phantom((16, 20, 24), 2.0, 1)
+    out.append(IMPTDoseEngine(g3, IMPTPlan(), cutoff_sigma=cutoff))
+    return out
+
+
+def _grads(engines, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for e in engines:
+        g = rng.normal(size=e.shape).astype(np.float32)
+        g[rng.uniform(size=e.shape) < 0.3] = 0.0
+        out.append(torch.from_numpy(g).to(e.device))
+    return out
+
+
+def _single_results(engines, grads):
+    wets, doses, adj = [], [], []
+    for e, g in zip(engines, grads):
+        e.compute_wet(force=True)
+        wets.append([b.wet.clone() for b in e.beams])
+        doses.append(e.dose().clone())
+        adj.append([(m.clone(), x.clone()) for m, x in e.adjoint(g)])
+    torch.cuda.synchronize()
+    return wets, doses, adj
+
+
+@pytest.mark.parametrize("cutoff", [6.0, 0.0])
+def test_batch_matches_single_path(dev, cutoff):
+    from DoseCUDA.batch import BatchedDose
+    _say("batch vs single path, cutoff %g" % cutoff)
+    engines = _engines(dev, cutoff)
+    grads = _grads(engines)
+    wets, doses, adj = _single_results(engines, grads)
+    bd = BatchedDose(engines)
+    _say("items %s" % bd.item_counts())
+    bd.compute_wet(force=True)
+    got = bd.forward()
+    gadj = bd.adjoint(grads)
+    torch.cuda.synchronize()
+    k = 0
+    for p, e in enumerate(engines):
+        for j, b in enumerate(e.beams):
+            assert torch.equal(b.wet, wets[p][j]), "WET plan %d beam %d" % (p, j)
+            assert torch.equal(gadj[k][0], adj[p][j][0]), "dL/dmu plan %d beam %d" % (p, j)
+            assert torch.equal(gadj[k][1], adj[p][j][1]), "dL/dxy plan %d beam %d" % (p, j)
+            k += 1
+        assert torch.equal(got[p], doses[p]), "dose plan %d" % p
+    assert float(got[3].abs().max()) == 0.0      # plan without beams
+    assert float(got[1].max()) > 0.0
+
+
+def test_batch_beam_doses(dev):
+    from DoseCUDA.batch import BatchedDose
+    _say("per-beam dose volumes")
+    engines = _engines(dev)[:3]
+    bd = BatchedDose(engines, keep_beam_doses=True)
+    doses = bd.forward()
+    torch.cuda.synchronize()
+    k = 0
+    for p, e in enumerate(engines):
+        acc = None
+        for j in range(len(e.beams)):
+            single = e.beam_dose(j)
+            assert torch.equal(bd.beam_doses[k], single), "beam dose plan %d beam %d" % (p, j)
+            acc = single.clone() if acc is None else acc + single
+            k += 1
+        assert torch.equal(doses[p], acc * float(e.n_fractions))
+
+
+def test_batch_graph_replay(dev):
+    """One HIP graph: forward -> element-wise loss gradient -> adjoint.  Replays
+    with new weights between them (and eager work in between) match eager
+    batched steps bit for bit."""
+    from DoseCUDA.batch import BatchedDose, GraphedStep
+    _say("graph capture / replay")
+    engines = _engines(dev)[:3]
+    bd = BatchedDose(engines)
+    bd.compute_wet()
+    mus = [b.mu0.clone() for e in engines for b in e.beams]
+    xys = [b.xy0.clone() for e in engines for b in e.beams]
+
+    def step():
+        doses = bd.forward(mus, xys)
+        for d, gd in zip(doses, bd.grad_doses):
+            torch.mul(d - 1.0, 2.0 / d.numel(), out=gd)
+        return doses, bd.adjoint(None)
+
+    gs = GraphedStep(step, dev)
+    rng = np.random.default_rng(11)
+    for rep in range(3):
+        for m, x in zip(mus, xys):
+            m.copy_(torch.from_numpy(rng.uniform(0.1, 1.0, size=m.shape[0]).astype(np.float32)))
+            x.add_(torch.from_numpy(rng.normal(scale=0.3, size=tuple(x.shape)).astype(np.float32)).to(dev))
+        doses_g, adj_g = gs.replay()
+        doses_g = [d.clone() for d in doses_g]
+        adj_g = [(a.clone(), b.clone()) for a, b in adj_g]
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device=dev).sum().item()   # eager work between replays
+        doses_e, adj_e = step()
+        torch.cuda.synchronize()
+        for a, b in zip(doses_g, doses_e):
+            assert torch.equal(a, b), "replay %d dose" % rep
+        for (a0, a1), (b0, b1) in zip(adj_g, adj_e):
+            assert torch.equal(a0, b0) and torch.equal(a1, b1), "replay %d gradients" % rep
+        _say("replay %d ok" % rep)
+
+
+@pytest.mark.parametrize("tune", [(32, 0, 0, 0), (64, 0, 0, 0), (0, 512, 16, 8), (32, 200000, 8, 4000)])
+def test_batch_adjoint_schedules(dev, tune):
+    from DoseCUDA.batch import BatchedDose, make_tune
+    _say("adjoint schedule %s" % (tune,))
+    engines = _engines(dev)[:3]
+    grads = _grads(engines, seed=3)
+    ref = BatchedDose(engines)
+    ref.compute_wet()
+    want = [(a.clone(), b.clone()) for a, b in ref.adjoint(grads)]
+    bd = BatchedDose(engines, tune=make_tune(*tune))
+    bd.compute_wet()
+    got = bd.adjoint(grads)
+    torch.cuda.synchronize()
+    for (a0, a1), (b0, b1) in zip(got, want):
+        if b0.numel() == 0:
+            continue
+        assert float((a0 - b0).abs().max()) <= 1e-5 * float(b0.abs().max())
+        assert float((a1 - b1).abs().max()) <= 1e-5 * float(b1.abs().max())

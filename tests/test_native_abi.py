@@ -82,6 +82,93 @@ def test_argument_errors_without_gpu():
     assert rc == N.DH_ERR_ARG
 
 
+def test_batch_struct_layout_matches_c(tmp_path):
+    from DoseCUDA import _native as N
+    src = tmp_path / "blayout.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "dosehip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(dh_batch_plan), sizeof(dh_batch_beam), sizeof(dh_batch_tune),
+         sizeof(dh_batch));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(dh_batch_plan, beam_start), offsetof(dh_batch_beam, spots),
+         offsetof(dh_batch_beam, wet), offsetof(dh_batch_beam, grad_xy), offsetof(dh_batch, n_bricks),
+         offsetof(dh_batch, tab));
+  return 0;
+}
+''')
+    exe = tmp_path / "blayout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    sizes = [int(v) for v in out[0].split()]
+    offs = [int(v) for v in out[1].split()]
+    assert sizes == [ctypes.sizeof(c) for c in (N.dh_batch_plan, N.dh_batch_beam, N.dh_batch_tune, N.dh_batch)]
+    assert offs == [N.dh_batch_plan.beam_start.offset, N.dh_batch_beam.spots.offset, N.dh_batch_beam.wet.offset,
+                    N.dh_batch_beam.grad_xy.offset, N.dh_batch.n_bricks.offset, N.dh_batch.tab.offset]
+
+
+def _fake_batch(N, n_spots=(100, 0, 7), grid=(16, 12, 10)):
+    """Descriptors with non-null fake device pointers (nothing is dereferenced
+    by the sizing / validation code)."""
+    fake = 4096
+    nb = len(n_spots)
+    plans = (N.dh_batch_plan * 2)()
+    plans[0].grid = N.dh_grid(grid[0], grid[1], grid[2], 2.0)
+    plans[0].rlsp_padded = fake
+    plans[0].dose = fake
+    plans[0].grad_dose = fake
+    plans[0].beam_start = 0
+    plans[0].n_beams = nb - 1
+    plans[1].grid = N.dh_grid(8, 8, 8, 2.0)
+    plans[1].rlsp_padded = fake
+    plans[1].dose = fake
+    plans[1].grad_dose = fake
+    plans[1].beam_start = nb - 1
+    plans[1].n_beams = 1
+    beams = (N.dh_batch_beam * nb)()
+    for k, S in enumerate(n_spots):
+        B = beams[k]
+        B.model = N.dh_beam_model(fake, fake, fake, 98, 400, 5, 1, 0.0, 1.0)
+        B.spots = N.dh_spot_set(fake if S else None, fake if S else None, S, 3 if S else 0)
+        B.wet = fake
+        B.grad_mu = fake
+        B.grad_xy = fake
+    return plans, beams
+
+
+def test_batch_sizes_and_errors_without_gpu():
+    from DoseCUDA import _native as N
+    lib = N.load()
+    plans, beams = _fake_batch(N)
+    tune = N.dh_batch_tune(0, 0, 0, 0)
+    n = lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(tune))
+    assert n > 0 and n % 256 == 0
+    # more slabs -> more adjoint partials -> larger workspace
+    big = N.dh_batch_tune(0, 1 << 20, 1, 70000)
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(big)) == N.DH_ERR_ARG  # over 65535
+    big = N.dh_batch_tune(0, 1 << 20, 1, 60000)
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(big)) > n
+    bad = N.dh_batch_tune(48, 0, 0, 0)
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(bad)) == N.DH_ERR_ARG
+    assert "schedule" in N.last_error()
+    # beams must be listed in order without gaps
+    plans[1].beam_start = 1
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, None) == N.DH_ERR_ARG
+    assert "without gaps" in N.last_error()
+    plans[1].beam_start = 2
+    beams[0].wet = None
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, None) == N.DH_ERR_ARG
+    beams[0].wet = 4096
+    # setup rejects a small workspace before touching the device
+    b = N.dh_batch()
+    rc = lib.dh_batch_setup(ctypes.byref(b), plans, 2, beams, 3, None, 6.0, ctypes.c_void_p(4096), 256, None)
+    assert rc == N.DH_ERR_ARG and "workspace too small" in N.last_error()
+    # phases refuse a batch that was never set up
+    assert lib.dh_batch_forward(ctypes.byref(b), None) == N.DH_ERR_ARG
+    assert "not set up" in N.last_error()
+
+
 def test_geometry_matches_oracle_restatement():
     from DoseCUDA.geometry import beam_geometry
     from oracle import impt_oracle as O
