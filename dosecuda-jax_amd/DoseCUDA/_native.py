@@ -80,6 +80,7 @@ class dh_batch(ctypes.Structure):
 
 
 DH_BATCH_REUSE_PREP = 1
+DH_BATCH_WET_REACH = 2
 
 # exported symbol -> (restype, argtypes); every declaration of include/dosehip.h
 _P = ctypes.c_void_p
@@ -108,8 +109,8 @@ SIGNATURES = {
     "dh_batch_setup": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.POINTER(dh_batch_plan), ctypes.c_int32,
                                       ctypes.POINTER(dh_batch_beam), ctypes.c_int32, ctypes.POINTER(dh_batch_tune),
                                       ctypes.c_float, _P, ctypes.c_int64, _P]),
-    "dh_batch_wet": (ctypes.c_int, [ctypes.POINTER(dh_batch), _P]),
-    "dh_batch_forward": (ctypes.c_int, [ctypes.POINTER(dh_batch), _P]),
+    "dh_batch_wet": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
+    "dh_batch_forward": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
     "dh_batch_adjoint": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
 }
 

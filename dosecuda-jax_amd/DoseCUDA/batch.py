@@ -9,6 +9,12 @@ single launch over the work items of every beam of every plan:
 * ``forward()``        dose of every plan (dh_batch_forward)
 * ``adjoint(grads)``   dL/dweight and dL/d(x, y) of every beam (dh_batch_adjoint)
 
+WET modes: ``"full"`` computes every voxel's WET once and keeps it (positions
+may then move freely); ``"reach"`` recomputes WET inside every ``forward()``,
+only in the bricks the current spots can reach (dh_batch_wet with
+DH_BATCH_WET_REACH), which gives the same dose and gradients bit for bit at a
+fraction of the ray-march work.
+
 The reference computes one beam per extension call and one plan per
 ``computeIMPTPlan`` (plan_impt.py:170-206); per plan the results here are the
 same as the engine's (beam doses added in beam order, times n_fractions).
@@ -48,9 +54,13 @@ class BatchedDose:
     as the reference's ``IMPTDoseGrid.beam_doses``; otherwise beams of
     multi-beam plans write workspace scratch.
     tune: dh_batch_tune (adjoint schedule) or None for the defaults.
+    wet_mode: "full" or "reach" (see the module notes).
     """
 
-    def __init__(self, engines, cutoff_sigma=None, tune=None, keep_beam_doses=False):
+    def __init__(self, engines, cutoff_sigma=None, tune=None, keep_beam_doses=False, wet_mode="full"):
+        if wet_mode not in ("full", "reach"):
+            raise ValueError("wet_mode must be 'full' or 'reach'")
+        self.wet_mode = wet_mode
         if not engines:
             raise ValueError("BatchedDose needs at least one plan")
         self.lib = N.load()
@@ -121,14 +131,23 @@ class BatchedDose:
         return N.stream_handle(self.device)
 
     def compute_wet(self, force=False):
-        """WET of every beam (written into each engine beam's ``wet``)."""
+        """Full WET of every beam (written into each engine beam's ``wet``)."""
         if self._wet_done and not force:
             return
-        N.check(self.lib.dh_batch_wet(ctypes.byref(self.batch), self._stream()), "dh_batch_wet")
+        N.check(self.lib.dh_batch_wet(ctypes.byref(self.batch), 0, self._stream()), "dh_batch_wet")
         self._wet_done = True
         for e in self.engines:
             for b in e.beams:
                 b.wet_valid = True
+
+    def _reach_wet(self):
+        # prepares the spot / layer records, then WET only where they reach
+        N.check(self.lib.dh_batch_wet(ctypes.byref(self.batch), N.DH_BATCH_WET_REACH, self._stream()),
+                "dh_batch_wet")
+        self._wet_done = False
+        for e in self.engines:
+            for b in e.beams:
+                b.wet_valid = False   # not the full WET any more
 
     def set_spots(self, mus=None, xys=None):
         """Per-beam weights / positions (lists over all beams of the batch, in
@@ -144,9 +163,14 @@ class BatchedDose:
 
     def forward(self, mus=None, xys=None):
         """Plan doses (list of (z, y, x) tensors, times n_fractions)."""
-        self.compute_wet()
         self.set_spots(mus, xys)
-        N.check(self.lib.dh_batch_forward(ctypes.byref(self.batch), self._stream()), "dh_batch_forward")
+        if self.wet_mode == "reach":
+            self._reach_wet()
+            flags = N.DH_BATCH_REUSE_PREP
+        else:
+            self.compute_wet()
+            flags = 0
+        N.check(self.lib.dh_batch_forward(ctypes.byref(self.batch), flags, self._stream()), "dh_batch_forward")
         for e, d in zip(self.engines, self.doses):
             if e.n_fractions != 1:
                 d.mul_(float(e.n_fractions))
@@ -156,8 +180,11 @@ class BatchedDose:
         """Spot gradients of every beam for dL/dD = grad_doses[p] of plan p
         (None: the batch's own ``grad_doses`` tensors, filled anew before each
         call; plans with n_fractions != 1 scale them in place).
-        Returns [(dL/dmu [S], dL/d(x, y) [S, 2])] per beam."""
-        self.compute_wet()
+        Returns [(dL/dmu [S], dL/d(x, y) [S, 2])] per beam.  In "reach" mode
+        the WET is the one the last forward() computed for its spot positions:
+        call adjoint after the forward of the same spots."""
+        if self.wet_mode == "full":
+            self.compute_wet()
         for p, e in enumerate(self.engines):
             g = self.grad_doses[p]
             if grad_doses is not None:

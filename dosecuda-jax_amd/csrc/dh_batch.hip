@@ -192,6 +192,8 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
                 c.wet_raw = reinterpret_cast<float *>(at(off));
                 off += align_up(nvox * 4);
             }
+            c.need = reinterpret_cast<unsigned char *>(at(off));
+            off += align_up(nb);
             L.brick.push_back(L.n_bricks);
             L.n_bricks += nb;
             L.smooth.push_back(L.n_smooth);
@@ -308,27 +310,34 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     return DH_OK;
 }
 
-extern "C" int dh_batch_wet(const dh_batch *b, void *stream) {
+static int prep(const dh_batch *b, hipStream_t st) {
+    return launch_prep_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_SPOT_OFF),
+                             table<long long>(b, T_LAYER_OFF), b->n_beams, b->n_spots, b->n_layers, st);
+}
+
+extern "C" int dh_batch_wet(const dh_batch *b, int32_t flags, void *stream) {
     if (!batch_ok(b)) return DH_ERR_ARG;
     if (!(b->tab[T_FLAGS] & F_RLSP)) {
         set_error("dh_batch_wet: a plan of the batch has no quad RLSP volume");
         return DH_ERR_ARG;
     }
     if (b->n_beams == 0) return DH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const bool reach = (flags & DH_BATCH_WET_REACH) != 0;
+    if (reach) {
+        int rc = (flags & DH_BATCH_REUSE_PREP) ? DH_OK : prep(b, st);
+        if (!rc) rc = launch_wet_need_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF), b->n_beams,
+                                            b->n_bricks, st);
+        if (rc) return rc;
+    }
     return launch_wet_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF),
-                            table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks,
-                            (hipStream_t)stream);
+                            table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks, reach, st);
 }
 
-static int prep(const dh_batch *b, hipStream_t st) {
-    return launch_prep_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_SPOT_OFF),
-                             table<long long>(b, T_LAYER_OFF), b->n_beams, b->n_spots, b->n_layers, st);
-}
-
-extern "C" int dh_batch_forward(const dh_batch *b, void *stream) {
+extern "C" int dh_batch_forward(const dh_batch *b, int32_t flags, void *stream) {
     if (!batch_ok(b)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    int rc = b->n_beams ? prep(b, st) : DH_OK;
+    int rc = (b->n_beams && !(flags & DH_BATCH_REUSE_PREP)) ? prep(b, st) : DH_OK;
     if (rc) return rc;
     return launch_forward_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF), b->n_beams,
                                 b->n_bricks, table<PlanCtx>(b, T_PLANS), table<long long>(b, T_SUM_OFF), b->n_plans,

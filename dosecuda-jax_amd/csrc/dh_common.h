@@ -162,6 +162,7 @@ struct BeamCtx {
     float2 *bsum;           // per-brick WET range of voxels with dL/dD != 0
     float *grad_mu, *grad_xy;
     const Stencil *stencil; // smoothing stencil (batched smoothing)
+    unsigned char *need;    // per brick: bit 0 smoothed WET needed, bit 1 raw WET needed
     Model m;
     Grid g;
     Geom b;
@@ -170,6 +171,10 @@ struct BeamCtx {
     float cut, rbrick;
     int pad0, pad1;
 };
+
+// WET (cm) written where the reach-limited WET pass skips a brick: beyond any
+// layer's range (1.1 R80), so the dose kernels skip those voxels.
+constexpr float kWetOutside = 1.0e30f;
 
 // A plan of a batch: its dose volume and its beams [beam0, beam0 + n_beams) of
 // the beam-context table.
@@ -209,7 +214,7 @@ struct BeamWs {
 void make_stencil(const dh_beam_geom &b, double s, Stencil &st);
 bool grid_ok(const dh_grid &g);
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
-                     long long total_bricks, long long total_blocks, hipStream_t st);
+                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st);
 // dh_pencil.hip
 int adjoint_chunk(int n_spots, int n_layers);
 int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t);
@@ -222,6 +227,8 @@ bool spot_set_ok(const dh_spot_set *ss);
 float brick_radius(float spacing);
 void fill_pencil_ctx(BeamCtx &c, const dh_grid &g, const dh_beam_geom &geom, const Model &m, const dh_spot_set &ss,
                      const BeamWs &w, float cutoff_sigma);
+int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
+                          hipStream_t st);
 int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long long *layer_off, int n_beams,
                       long long total_spots, long long total_layers, hipStream_t st);
 int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,

@@ -205,12 +205,15 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(BeamCtx c, unsigned nblock
 // [off[b], off[b + 1]) of the item space.
 __global__ __launch_bounds__(256) void k_wet_raymarch_batch(const BeamCtx *__restrict__ ctx,
                                                            const long long *__restrict__ off, int n_beams,
-                                                           long long base, long long total) {
+                                                           long long base, long long total, int use_need) {
     const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
     if (item >= total) return;
     const int bm = find_owner(off, n_beams, item);
     const BeamCtx &c = load_ctx(ctx, bm);
-    raymarch_brick(c, item - off[bm], threadIdx.x & 63);
+    const long long brick = item - off[bm];
+    // reach-limited pass: bricks no smoothed voxel reads stay unwritten
+    if (use_need && !(gld(c.need, brick) & 2u)) return;
+    raymarch_brick(c, brick, threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(256) void k_wet_smooth(const float *__restrict__ ra
 // Batched: beam b owns blocks [boff[b], boff[b + 1]) of 256 voxels each.
 __global__ __launch_bounds__(256) void k_wet_smooth_batch(const BeamCtx *__restrict__ ctx,
                                                          const long long *__restrict__ boff, int n_beams,
-                                                         long long base, long long total_blocks) {
+                                                         long long base, long long total_blocks, int use_need) {
     const long long blk = base + blockIdx.x;
     if (blk >= total_blocks) return;
     const int bm = find_owner(boff, n_beams, blk);
@@ -293,6 +296,16 @@ __global__ __launch_bounds__(256) void k_wet_smooth_batch(const BeamCtx *__restr
     const long long t = (blk - boff[bm]) * 256 + threadIdx.x;
     const long long nvox = (long long)c.g.nx * c.g.ny * c.g.nz;
     if (t >= nvox) return;
+    if (use_need) {
+        const int i = (int)(t % c.g.nx);
+        const long long r = t / c.g.nx;
+        const int j = (int)(r % c.g.ny), k = (int)(r / c.g.ny);
+        const long long brick = (i >> 2) + (long long)c.bg.bx * ((j >> 2) + (long long)c.bg.by * (k >> 2));
+        if (!(gld(c.need, brick) & 1u)) {
+            gst(c.wet, t, kWetOutside);
+            return;
+        }
+    }
     gst(c.wet, t, smooth_voxel(c.wet_raw, c.g, c.stencil, t));
 }
 
@@ -411,20 +424,20 @@ extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid
 // Batched launches: ctx is a device table of n_beams entries.
 namespace dh {
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
-                     long long total_bricks, long long total_blocks, hipStream_t st) {
+                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st) {
     // item spaces above 2^24 bricks go out in several launches
     constexpr long long kMaxItems = 1LL << 24;
     for (long long base = 0; base < total_bricks; base += kMaxItems) {
         const long long n = total_bricks - base < kMaxItems ? total_bricks - base : kMaxItems;
         hipLaunchKernelGGL(k_wet_raymarch_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
-                           n_beams, base, total_bricks);
+                           n_beams, base, total_bricks, use_need ? 1 : 0);
         const int rc = check_launch("k_wet_raymarch_batch");
         if (rc) return rc;
     }
     for (long long base = 0; base < total_blocks; base += kMaxItems / 4) {
         const long long n = total_blocks - base < kMaxItems / 4 ? total_blocks - base : kMaxItems / 4;
         hipLaunchKernelGGL(k_wet_smooth_batch, dim3((unsigned)n), dim3(256), 0, st, ctx, block_off, n_beams, base,
-                           total_blocks);
+                           total_blocks, use_need ? 1 : 0);
         const int rc = check_launch("k_wet_smooth_batch");
         if (rc) return rc;
     }

@@ -262,16 +262,29 @@ int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32_t n_plans,
                    const dh_batch_beam *beams, int32_t n_beams, const dh_batch_tune *tune,
                    float cutoff_sigma, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* Flags of the phase calls.
+ * DH_BATCH_REUSE_PREP  reuse the spot and layer records the previous phase call
+ *                      prepared (spot arrays unchanged since) instead of
+ *                      preparing them again.
+ * DH_BATCH_WET_REACH   (dh_batch_wet, cutoff_sigma > 0) compute WET only in
+ *                      bricks that some layer of the beam can reach with the
+ *                      CURRENT spot positions, and where smoothing reads raw
+ *                      WET for them; elsewhere WET = 1e30 (beyond every
+ *                      range).  Dose and gradients of forward / adjoint calls
+ *                      with the same spot positions equal those from the full
+ *                      WET bit for bit; after spot positions move, call
+ *                      dh_batch_wet again.  The volume is not the reference's
+ *                      WET output (use the full pass for that). */
+#define DH_BATCH_REUSE_PREP 1
+#define DH_BATCH_WET_REACH  2
+
 /* WET of every beam: ray-march then smoothing (dh_wet_raymarch + dh_wet_smooth per beam). */
-int dh_batch_wet(const dh_batch *batch, void *stream);
+int dh_batch_wet(const dh_batch *batch, int32_t flags, void *stream);
 
 /* Dose of every plan (dh_pb_forward per beam, beam doses added in beam order). */
-int dh_batch_forward(const dh_batch *batch, void *stream);
+int dh_batch_forward(const dh_batch *batch, int32_t flags, void *stream);
 
-/* Spot gradients of every beam from its plan's grad_dose (dh_pb_adjoint per beam).
- * flags & DH_BATCH_REUSE_PREP: reuse the spot and layer records the last
- * dh_batch_forward prepared (spot arrays unchanged since). */
-#define DH_BATCH_REUSE_PREP 1
+/* Spot gradients of every beam from its plan's grad_dose (dh_pb_adjoint per beam). */
 int dh_batch_adjoint(const dh_batch *batch, int32_t flags, void *stream);
 
 #ifdef __cplusplus

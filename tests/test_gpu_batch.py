@@ -189,3 +189,52 @@ def test_batch_adjoint_schedules(dev, tune):
             continue
         assert float((a0 - b0).abs().max()) <= 1e-5 * float(b0.abs().max())
         assert float((a1 - b1).abs().max()) <= 1e-5 * float(b1.abs().max())
+
+
+@pytest.mark.parametrize("cutoff", [6.0, 0.0])
+def test_batch_reach_limited_wet(dev, cutoff):
+    """WET only where the current spots reach: dose and gradients equal the
+    full-WET batch bit for bit, also after spot positions move (graph replays
+    recompute the reach-limited WET), and the skipped bricks hold the marker."""
+    from DoseCUDA.batch import BatchedDose, GraphedStep
+    _say("reach-limited WET, cutoff %g" % cutoff)
+    engines = _engines(dev, cutoff)[:3]
+    full = BatchedDose(engines)
+    full.compute_wet()
+    mus = [b.mu0.clone() for e in engines for b in e.beams]
+    xys = [b.xy0.clone() for e in engines for b in e.beams]
+    grads = _grads(engines, seed=5)
+
+    def run_full():
+        d = [x.clone() for x in full.forward(mus, xys)]
+        a = [(m.clone(), x.clone()) for m, x in full.adjoint(grads)]
+        return d, a
+
+    # separate engine objects so the two batches keep separate WET volumes
+    engines_r = _engines(dev, cutoff)[:3]
+    reach = BatchedDose(engines_r, wet_mode="reach")
+
+    def step_reach():
+        d = reach.forward(mus, xys)
+        for gd, g in zip(reach.grad_doses, grads):
+            gd.copy_(g)
+        return d, reach.adjoint(None)
+
+    gs = GraphedStep(step_reach, dev)
+    rng = np.random.default_rng(21)
+    for rep in range(3):
+        if rep:
+            for x in xys:
+                x.add_(torch.from_numpy(rng.normal(scale=4.0, size=tuple(x.shape)).astype(np.float32)).to(dev))
+        want_d, want_a = run_full()
+        got_d, got_a = gs.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(got_d, want_d):
+            assert torch.equal(a, b), "replay %d dose" % rep
+        for (a0, a1), (b0, b1) in zip(got_a, want_a):
+            assert torch.equal(a0, b0) and torch.equal(a1, b1), "replay %d gradients" % rep
+        skipped = sum(int((b.wet >= 1e29).sum()) for e in engines_r for b in e.beams)
+        total = sum(b.wet.numel() for e in engines_r for b in e.beams)
+        _say("replay %d ok, WET skipped in %d of %d voxels" % (rep, skipped, total))
+        if cutoff == 0.0:
+            assert skipped == 0

@@ -165,7 +165,7 @@ def test_batch_sizes_and_errors_without_gpu():
     rc = lib.dh_batch_setup(ctypes.byref(b), plans, 2, beams, 3, None, 6.0, ctypes.c_void_p(4096), 256, None)
     assert rc == N.DH_ERR_ARG and "workspace too small" in N.last_error()
     # phases refuse a batch that was never set up
-    assert lib.dh_batch_forward(ctypes.byref(b), None) == N.DH_ERR_ARG
+    assert lib.dh_batch_forward(ctypes.byref(b), 0, None) == N.DH_ERR_ARG
     assert "not set up" in N.last_error()
 
 
