@@ -79,6 +79,7 @@ class dh_batch(ctypes.Structure):
                 ("n_reduce_blocks", ctypes.c_int64), ("tab", ctypes.c_int64 * 16)]
 
 
+DH_BATCH_TAB_HEADER_END = 13   # dh_batch.tab index: end of the tables uploaded at setup
 DH_BATCH_REUSE_PREP = 1
 DH_BATCH_WET_REACH = 2
 

@@ -204,6 +204,13 @@ class BatchedDose:
                 k += 1
         return out
 
+    def poison_scratch(self):
+        """Fill the workspace scratch (everything after the set-up tables) with
+        NaN bit patterns: a phase that read scratch it had not written would
+        then show up as NaN results (test aid)."""
+        start = int(self.batch.tab[N.DH_BATCH_TAB_HEADER_END])
+        self.ws[start:].fill_(255)
+
     def item_counts(self):
         """Work items of each phase (from the set-up tables)."""
         b = self.batch

@@ -41,11 +41,13 @@ enum Table {
     T_REDUCE_OFF,   // reduction blocks
     T_SUM_OFF,      // long long [n_plans]: plan-sum blocks
     T_STENCIL,      // Stencil [n_beams]
+    T_TOUCH,        // int [n_beams][MAX_TOUCH]: brick offsets smoothing reads
     T_HEADER_END,   // first byte after the tables (uploaded at setup)
     T_FLAGS,        // not an offset: capability bits below
     N_TABLES
 };
 static_assert(N_TABLES <= 16, "dh_batch.tab holds 16 entries");
+static_assert(T_HEADER_END == 13, "dh_batch.tab[13] is documented as the end of the uploaded tables");
 
 enum Flags { F_RLSP = 1, F_GRAD_DOSE = 2, F_GRAD_OUT = 4 };
 
@@ -71,6 +73,7 @@ struct Layout {
     std::vector<PlanCtx> plans;
     std::vector<long long> brick, smooth, spot, layer, part, adj32, adj64, reduce, sum;
     std::vector<Stencil> stencil;
+    std::vector<int> touch;
     long long tab[N_TABLES];
     long long n_bricks = 0, n_smooth = 0, n_spots = 0, n_layers = 0, n_sum = 0, n_part = 0, n_adj32 = 0,
               n_adj64 = 0, n_reduce = 0;
@@ -141,7 +144,10 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
     off += align_up((long long)n_plans * 8);
     L.tab[T_STENCIL] = off;
     off += align_up((long long)n_beams * sizeof(Stencil));
+    L.tab[T_TOUCH] = off;
+    off += align_up((long long)n_beams * MAX_TOUCH * 4);
     L.tab[T_HEADER_END] = off;
+    L.touch.assign((size_t)n_beams * MAX_TOUCH, 0);
 
     for (int p = 0; p < n_plans; ++p) {
         const dh_batch_plan &P = plans[p];
@@ -180,6 +186,8 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
             c.grad_xy = B.grad_xy;
             make_stencil(B.geom, (double)P.grid.spacing, L.stencil[b]);
             c.stencil = reinterpret_cast<const Stencil *>(at(L.tab[T_STENCIL] + (long long)b * sizeof(Stencil)));
+            c.n_touch = make_touch(L.stencil[b], L.touch.data() + (size_t)b * MAX_TOUCH, MAX_TOUCH);
+            c.touch = reinterpret_cast<const int *>(at(L.tab[T_TOUCH] + (long long)b * MAX_TOUCH * 4));
             c.dose = B.dose;
             if (c.dose && c.dose != P.dose) direct = false;
             if (!c.dose && P.n_beams == 1) c.dose = P.dose;
@@ -282,6 +290,7 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     put(T_REDUCE_OFF, D.reduce.data(), D.reduce.size() * 8);
     put(T_SUM_OFF, D.sum.data(), D.sum.size() * 8);
     put(T_STENCIL, D.stencil.data(), D.stencil.size() * sizeof(Stencil));
+    put(T_TOUCH, D.touch.data(), D.touch.size() * 4);
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(workspace, img.data(), img.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -163,14 +163,18 @@ struct BeamCtx {
     float *grad_mu, *grad_xy;
     const Stencil *stencil; // smoothing stencil (batched smoothing)
     unsigned char *need;    // per brick: bit 0 smoothed WET needed, bit 1 raw WET needed
+    const int *touch;       // brick offsets smoothing reads raw WET from (make_touch)
     Model m;
     Grid g;
     Geom b;
     BrickGrid bg;
     int n_spots, n_layers, nslab, cs;
     float cut, rbrick;
-    int pad0, pad1;
+    int n_touch, pad1;
 };
+
+// Most brick offsets a smoothing stencil reaches (54 entries x 2 x 2 x 2).
+constexpr int MAX_TOUCH = 432;
 
 // WET (cm) written where the reach-limited WET pass skips a brick: beyond any
 // layer's range (1.1 R80), so the dose kernels skip those voxels.
@@ -212,6 +216,7 @@ struct BeamWs {
 // host helpers shared by the single-beam and batched entry points
 // dh_wet.hip
 void make_stencil(const dh_beam_geom &b, double s, Stencil &st);
+int make_touch(const Stencil &st, int *out, int cap);
 bool grid_ok(const dh_grid &g);
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
                      long long total_bricks, long long total_blocks, bool use_need, hipStream_t st);
@@ -228,7 +233,7 @@ float brick_radius(float spacing);
 void fill_pencil_ctx(BeamCtx &c, const dh_grid &g, const dh_beam_geom &geom, const Model &m, const dh_spot_set &ss,
                      const BeamWs &w, float cutoff_sigma);
 int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
-                          hipStream_t st);
+                          hipStream_t st);  // two launches: reach test, then dilation
 int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long long *layer_off, int n_beams,
                       long long total_spots, long long total_layers, hipStream_t st);
 int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,

@@ -509,40 +509,49 @@ __global__ __launch_bounds__(256) void k_brick_summary_batch(const BeamCtx *__re
 // ---------------------------------------------------------------------------
 // Reach-limited WET.  With culling, a brick that every layer skips for the
 // widest WET range possible ([-1 mm, 1.1 R80]) gets no dose and no gradient
-// term whatever its WET, so its WET need not be computed: bit 0 of need[brick]
-// (smoothed WET) is set when some layer is not skipped.  Smoothing reads raw
-// WET up to 9 mm (plus rounding to a voxel) away, so bit 1 (raw WET) is the
-// same test with the brick radius grown to cover every brick that a voxel of
-// a bit-0 brick can read: rbrick + Lf (2 rbrick + 9 mm + s), Lf bounding how
-// fast the reach test's distance changes as the point moves (the spot
-// positions scale with depth by 1 - z / VSAD).  reach_skip is monotone in the
-// radius and in the WET range, so every brick the dose kernels evaluate has
-// bit 0 and the WET they read equals the full computation's.
+// term whatever its WET, so its WET need not be computed.  reach_skip is
+// monotone in the WET range, so every brick the dose kernels evaluate (and
+// every brick whose voxels the adjoint visits) passes this test.
+//   pass 1 (k_wet_need_batch): bit 0 of need[brick] = some layer reaches it
+//           (smoothed WET needed);
+//   pass 2 (k_wet_dilate_batch): bit 1 = some brick that smoothing reads raw
+//           WET for, i.e. b - o for an offset o of the beam's touch set, has
+//           bit 0 (raw WET needed).
+// Every WET value the dose kernels then read equals the full computation's.
 __device__ __forceinline__ void wet_need_one(const BeamCtx &c, long long brick, int lane) {
-    bool need_s = !(c.cut > 0.0f), need_r = need_s;
-    if (!need_s) {
+    bool need = !(c.cut > 0.0f);
+    if (!need) {
         int bi, bj, bk;
         brick_of(c.bg, brick, bi, bj, bk);
         const float3 vc = brick_centre(c.g, c.b, bi, bj, bk);
         const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
-        const float reach = fmaf(2.0f, c.rbrick, 9.0f + c.g.s);
         for (int l0 = 0; l0 < c.n_layers; l0 += WAVE) {
             if (l0 + lane < c.n_layers) {
                 const LayerC L = gld(c.lc, l0 + lane);
-                if (L.ok) {
-                    const BrickInfo B = brick_info_at(c.b, vc, -1.0f, L.lim);
-                    need_s |= !reach_skip(L, B, ivx, ivy, c.cut, c.rbrick);
-                    const float ax = fmaxf(fabsf(L.xmin), fabsf(L.xmax)) * ivx;
-                    const float ay = fmaxf(fabsf(L.ymin), fabsf(L.ymax)) * ivy;
-                    const float lf = 1.001f * sqrtf(1.0f + ax * ax + ay * ay);
-                    need_r |= !reach_skip(L, B, ivx, ivy, c.cut, fmaf(lf, reach, c.rbrick));
-                }
+                if (L.ok) need |= !reach_skip(L, brick_info_at(c.b, vc, -1.0f, L.lim), ivx, ivy, c.cut, c.rbrick);
             }
         }
-        need_s = __ballot(need_s) != 0ull;
-        need_r = need_s || __ballot(need_r) != 0ull;
+        need = __ballot(need) != 0ull;
     }
-    if (lane == 0) gst(c.need, brick, (unsigned char)((need_s ? 1u : 0u) | (need_r ? 2u : 0u)));
+    if (lane == 0) gst(c.need, brick, (unsigned char)(need ? 1u : 0u));
+}
+
+__device__ __forceinline__ void wet_dilate_one(const BeamCtx &c, long long brick, int lane) {
+    int bi, bj, bk;
+    brick_of(c.bg, brick, bi, bj, bk);
+    const unsigned char own = gld(c.need, brick);
+    bool raw = (own & 1u) != 0;
+    for (int t0 = 0; t0 < c.n_touch && !__ballot(raw); t0 += WAVE) {
+        if (t0 + lane < c.n_touch) {
+            const int key = gld(c.touch, t0 + lane);
+            const int si = bi - ((key & 255) - 128), sj = bj - (((key >> 8) & 255) - 128),
+                      sk = bk - (((key >> 16) & 255) - 128);
+            if ((unsigned)si < (unsigned)c.bg.bx && (unsigned)sj < (unsigned)c.bg.by && (unsigned)sk < (unsigned)c.bg.bz)
+                raw |= (gld(c.need, si + (long long)c.bg.bx * (sj + (long long)c.bg.by * sk)) & 1u) != 0;
+        }
+    }
+    // only bit 1 of this brick's own byte changes; other waves read bit 0 only
+    if (lane == 0 && __ballot(raw)) gst(c.need, brick, (unsigned char)(own | 2u));
 }
 
 __global__ __launch_bounds__(256) void k_wet_need_batch(const BeamCtx *__restrict__ ctx,
@@ -553,6 +562,16 @@ __global__ __launch_bounds__(256) void k_wet_need_batch(const BeamCtx *__restric
     const int bm = find_owner(off, n_beams, item);
     const BeamCtx &c = load_ctx(ctx, bm);
     wet_need_one(c, item - off[bm], threadIdx.x & 63);
+}
+
+__global__ __launch_bounds__(256) void k_wet_dilate_batch(const BeamCtx *__restrict__ ctx,
+                                                         const long long *__restrict__ off, int n_beams,
+                                                         long long base, long long total) {
+    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
+    if (item >= total) return;
+    const int bm = find_owner(off, n_beams, item);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    wet_dilate_one(c, item - off[bm], threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -985,6 +1004,13 @@ int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_
         hipLaunchKernelGGL(k_wet_need_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
                            n_beams, base, total_bricks);
         const int rc = check_launch("k_wet_need_batch");
+        if (rc) return rc;
+    }
+    for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
+        const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
+        hipLaunchKernelGGL(k_wet_dilate_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
+                           n_beams, base, total_bricks);
+        const int rc = check_launch("k_wet_dilate_batch");
         if (rc) return rc;
     }
     return DH_OK;

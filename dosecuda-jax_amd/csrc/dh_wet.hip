@@ -344,6 +344,35 @@ void make_stencil(const dh_beam_geom &b, double s, Stencil &st) {
     }
 }
 
+// Brick offsets o such that smoothing a voxel of brick b reads raw WET in
+// brick b + o: per stencil entry and axis the neighbour index is l + base
+// (+1 on a tie) for l = 0..3 inside the brick, so the brick offset is
+// floor((l + base [+ 1]) / 4).  Packed as (ox + 128) | (oy + 128) << 8 |
+// (oz + 128) << 16; returns the count (at most 54 * 8).
+static int floor4(int n) { return n >= 0 ? n / 4 : -((-n + 3) / 4); }
+
+int make_touch(const Stencil &st, int *out, int cap) {
+    int n = 0;
+    for (int e = 0; e < N_STENCIL; ++e) {
+        const int base[3] = {st.dx[e], st.dy[e], st.dz[e]};
+        int lo[3], hi[3];
+        for (int ax = 0; ax < 3; ++ax) {
+            const int t = (st.tie[e] >> ax) & 1;
+            lo[ax] = floor4(base[ax]);
+            hi[ax] = floor4(base[ax] + 3 + t);
+        }
+        for (int oz = lo[2]; oz <= hi[2]; ++oz)
+            for (int oy = lo[1]; oy <= hi[1]; ++oy)
+                for (int ox = lo[0]; ox <= hi[0]; ++ox) {
+                    const int key = (ox + 128) | (oy + 128) << 8 | (oz + 128) << 16;
+                    bool seen = false;
+                    for (int q = 0; q < n && !seen; ++q) seen = out[q] == key;
+                    if (!seen && n < cap) out[n++] = key;
+                }
+    }
+    return n;
+}
+
 bool grid_ok(const dh_grid &g) {
     if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0.0f)) {
         set_error("invalid grid %d x %d x %d, spacing %g", g.nx, g.ny, g.nz, (double)g.spacing);

@@ -247,7 +247,8 @@ typedef struct dh_batch {
     int64_t n_partials;        /* adjoint partial records */
     int64_t n_adj32, n_adj64;  /* adjoint workgroups of 32- and 64-spot chunks */
     int64_t n_reduce_blocks;   /* 64-spot reduction blocks */
-    int64_t tab[16];           /* byte offsets of the device tables in ws */
+    int64_t tab[16];           /* byte offsets of the device tables in ws; tab[13] = end of the tables
+                                  uploaded at setup (the rest of ws is scratch the phases rewrite) */
 } dh_batch;
 
 /* Workspace bytes for a batch (negative on invalid descriptors; see dh_last_error). */

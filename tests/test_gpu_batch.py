@@ -213,6 +213,10 @@ def test_batch_reach_limited_wet(dev, cutoff):
     # separate engine objects so the two batches keep separate WET volumes
     engines_r = _engines(dev, cutoff)[:3]
     reach = BatchedDose(engines_r, wet_mode="reach")
+    reach.poison_scratch()     # raw WET the reach pass skips must never be read
+    for e in engines_r:
+        for b in e.beams:
+            b.wet.fill_(float("nan"))
 
     def step_reach():
         d = reach.forward(mus, xys)
