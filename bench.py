@@ -1,24 +1,33 @@
 #!/usr/bin/env python3
-"""Forward + adjoint of the 10k-spot head phantom on one MI355X.
+"""Forward + adjoint of the 10k-spot head phantom on one MI355X: batched vs plain.
 
 Workload (synthetic, seed 2): 160 x 200 x 200 (z, y, x) HU grid at 2 mm --
 ellipsoidal soft tissue N(40, 20) clipped to [-100, 100], 8 mm bone shell
 U(700, 1200), two 15 mm air spheres, air outside; 3 beams at gantry 0/120/240;
 40 energy layers (RS0 ids 20..59) x 81 spots (3 mm grid in a 15 mm disk) per
-beam = 9720 spots, weights U(0, 1).
+beam = 9720 spots.  Plan copy 0 has the seed-2 weights U(0, 1); copies 1..K-1
+draw their own weights U(0, 1).
 
-One step = per beam: WET ray-march + smoothing; plan dose (forward); dL/dD of
-L = mean_target (D - 2)^2 + 0.1 mean_body D^2 (target: 25 mm sphere at the
-isocentre, body: HU > -500); adjoint (dL/dweight, dL/d(x, y)) per beam.
+One step (batched path) = ONE HIP-graph replay over a batch of K such plans:
+reach-limited WET (ray-march + smoothing where the spots can reach), plan dose
+(forward), dL/dD of L = mean_target (D - 2)^2 + 0.1 mean_body D^2 (target:
+25 mm sphere at the isocentre, body: HU > -500), adjoint (dL/dweight,
+dL/d(x, y)) for every beam of every plan.
 
-value = spot.voxel evaluations per second, counted as the reference does
-(every spot of a layer for every voxel with WET <= 1.1 R80, forward plus
-adjoint over voxels with dL/dD != 0), per step, over the timed steps.
+Plain path (per plan, for the ratio): per beam eager calls of full WET
+(ray-march + smoothing), dense forward (every spot for every in-range voxel,
+cutoff 0) summed over beams, the same dL/dD, dense adjoint per beam.
+
+value = spot.voxel evaluations per second counted as the reference does: per
+plan, every spot of a layer for every voxel with WET <= 1.1 R80 (forward) plus
+the same over voxels with dL/dD != 0 of the plain path (adjoint), times K, per
+second of the timed batched steps.
 
 Prints ONE JSON line.
 """
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -35,33 +44,48 @@ FWD_FLOP_PER_PAIR = 20.0   # r_x, r_y 2; |r|^2 4; t.r 4; d2 3; 2 scales + factor
 ADJ_FLOP_PER_PAIR = 41.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def build_problem(engine_cls, cutoff):
+def masks(g, dev):
     import numpy as np
     import torch
-    from DoseCUDA.workloads import head_plan, voxel_centres
-    g, plan = head_plan(seed=2)
-    eng = engine_cls(g, plan, gpu_id=0, cutoff_sigma=cutoff)
+    from DoseCUDA.workloads import voxel_centres
     z, y, x = voxel_centres(g)
     Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
     target = (X ** 2 + Y ** 2 + Z ** 2) <= 25.0 ** 2
     body = g.HU > -500.0
-    dev = eng.device
-    tmask = torch.from_numpy(target).to(dev)
-    bmask = torch.from_numpy(body).to(dev)
-    return g, plan, eng, tmask, bmask
+    return torch.from_numpy(target).to(dev), torch.from_numpy(body).to(dev)
 
 
-def loss_grad(dose, tmask, bmask, n_t, n_b):
-    """dL/dD of mean_target (D - 2)^2 + 0.1 mean_body D^2."""
+def loss_grad(dose, tmask, bmask, n_t, n_b, out=None):
+    """dL/dD of mean_target (D - 2)^2 + 0.1 mean_body D^2 (element-wise ops only)."""
     import torch
     g = torch.where(tmask, (dose - 2.0) * (2.0 / n_t), torch.zeros_like(dose))
-    return g + torch.where(bmask, dose * (0.2 / n_b), torch.zeros_like(dose))
+    g = g + torch.where(bmask, dose * (0.2 / n_b), torch.zeros_like(dose))
+    if out is not None:
+        out.copy_(g)
+        return out
+    return g
+
+
+def plan_copies(K):
+    """K head-phantom plans: copy 0 with the seed-2 weights, the others with weights of their own."""
+    import numpy as np
+    from DoseCUDA.workloads import head_plan
+    g, plan = head_plan(seed=2)
+    plans = [plan]
+    for k in range(1, K):
+        _, p = head_plan(seed=2)
+        rng = np.random.default_rng(100 + k)
+        for b in p.beam_list:
+            b.spot_list[:, 2] = rng.uniform(0.0, 1.0, size=b.spot_list.shape[0]).astype(np.float32)
+        plans.append(p)
+    return g, plans
 
 
 def cpu_baseline(g, plan, eng, gvol_np, n_sample, seed=0):
@@ -104,9 +128,20 @@ def cpu_baseline(g, plan, eng, gvol_np, n_sample, seed=0):
             pairs += int(inr.sum()) * int(ns) + int((inr & gnz).sum()) * int(ns)
         log("cpu baseline beam %d: fwd %.2fs adj %.2fs" % (k, t1 - t0, t2 - t1))
     return {"value": pairs / t_total, "unit": "spot·voxel evals/s", "cores": threads, "kind": "port",
-            "sample": "%d random voxels of the same phantom, 3 beams: ray-march of the sampled voxels + "
-                      "forward dose + adjoint at them (float32 oracle build, OpenMP); smoothing omitted; "
-                      "%d spot.voxel pairs in %.1f s" % (len(idx), pairs, t_total)}
+            "sample": "%d random voxels of the head phantom (plan copy 0), 3 beams: ray-march of the sampled "
+                      "voxels + forward dose + adjoint at them (float32 oracle build, OpenMP, dense: every "
+                      "spot); smoothing omitted; %d spot.voxel pairs in %.1f s" % (len(idx), pairs, t_total)}
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
 
 
 def main():
@@ -114,9 +149,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plans", type=int, default=4, help="head-phantom plans per batch (K)")
     ap.add_argument("--cutoff-sigma", type=float, default=6.0)
-    ap.add_argument("--plain-steps", type=int, default=2, help="steps of the dense (cutoff 0) path; 0 = skip")
+    ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain (dense, per-beam) path; 0 = skip")
     ap.add_argument("--cpu-sample", type=int, default=60000, help="voxels in the CPU-baseline sample; 0 = skip")
+    ap.add_argument("--eager-only", action="store_true",
+                    help="time eager batched steps only (no graph), e.g. under a profiler")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE "
                          "counters against a copy of known size) and exit")
@@ -138,118 +176,165 @@ def main():
 
     import numpy as np
     import torch
-    from DoseCUDA.engine import IMPTDoseEngine
+    from DoseCUDA import _native as N
     from DoseCUDA import device as D
+    from DoseCUDA.batch import BatchedDose, GraphedStep
+    from DoseCUDA.engine import IMPTDoseEngine
 
     assert torch.cuda.is_available(), "bench.py needs a HIP GPU"
-    log("building workload")
-    g, plan, eng, tmask, bmask = build_problem(IMPTDoseEngine, args.cutoff_sigma)
+    K = args.plans
+    log("building workload: %d head-phantom plans" % K)
+    g, plans = plan_copies(K)
+    engines = [IMPTDoseEngine(g, p, gpu_id=0, cutoff_sigma=args.cutoff_sigma) for p in plans]
+    dev = engines[0].device
+    tmask, bmask = masks(g, dev)
     n_t = float(tmask.sum().item())
     n_b = float(bmask.sum().item())
-    dev = eng.device
-    mus = [b.mu0.clone() for b in eng.beams]
-    xys = [b.xy0.clone() for b in eng.beams]
-    nb = len(eng.beams)
-    n_spots = sum(b.spots.n_spots for b in eng.beams)
-    nvox = int(np.prod(eng.shape))
+    nvox = int(np.prod(engines[0].shape))
+    nb = len(engines[0].beams)
+    n_spots = sum(b.spots.n_spots for b in engines[0].beams)
 
-    ev = {k: [] for k in ("wet", "fwd", "adj")}
+    bd = BatchedDose(engines, wet_mode="reach")
+    mus = [b.mu0.clone() for e in engines for b in e.beams]
+    xys = [b.xy0.clone() for e in engines for b in e.beams]
 
-    def step(record):
-        def rec():
-            e = torch.cuda.Event(enable_timing=True)
-            e.record()
-            return e
-        for b in eng.beams:
-            e0 = rec() if record else None
-            b.wet_raw, b.wet = D.wet_from_padded(eng.rlsp_padded, eng.grid, b.geo, dev, raw_out=b.wet_raw,
-                                                 out=b.wet)
-            if record:
-                ev["wet"].append((e0, rec()))
-        dose = torch.empty(eng.shape, dtype=torch.float32, device=dev)
-        for k, b in enumerate(eng.beams):
-            b.spots.set_weights_positions(mus[k], xys[k])
-            e0 = rec() if record else None
-            D.pb_forward(b.wet, dose, eng.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes,
-                         cutoff_sigma=eng.cutoff_sigma, accumulate=k > 0)
-            if record:
-                ev["fwd"].append((e0, rec()))
-        gd = loss_grad(dose, tmask, bmask, n_t, n_b)
-        out = []
-        for k, b in enumerate(eng.beams):
-            e0 = rec() if record else None
-            gm, gx = D.pb_adjoint(b.wet, gd, eng.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes,
-                                  cutoff_sigma=eng.cutoff_sigma)
-            if record:
-                ev["adj"].append((e0, rec()))
-            out.append((gm, gx))
-        return dose, gd, out
+    def step():
+        doses = bd.forward(mus, xys)
+        for d, gd in zip(doses, bd.grad_doses):
+            loss_grad(d, tmask, bmask, n_t, n_b, out=gd)
+        return doses, bd.adjoint(None, reuse_prep=True, input_order=False)
 
-    log("warmup %d" % args.warmup)
-    for _ in range(args.warmup):
-        dose, gd, _ = step(False)
+    log("batch items %s" % bd.item_counts())
+    ms_graph = None
+    if not args.eager_only:
+        log("capturing the batched step")
+        gs = GraphedStep(step, dev, warmup=max(1, args.warmup))
+        for _ in range(args.warmup):
+            gs.replay()
+        torch.cuda.synchronize()
+        log("timing %d graph replays" % args.steps)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            gs.replay()
+        torch.cuda.synchronize()
+        ms_graph = (time.perf_counter() - t0) / args.steps * 1e3
+
+    # eager batched steps with per-kernel HIP events (same kernels, same stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * N.DH_TIME_N)]
+    for e in ev:
+        e.record()
     torch.cuda.synchronize()
-
-    # work counts (reference-equivalent and evaluated pairs) for this step
-    counts = torch.zeros(4, dtype=torch.int64, device=dev)
-    for b in eng.beams:
-        D.pb_count_pairs(b.wet, gd, eng.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes,
-                         cutoff_sigma=eng.cutoff_sigma, counts=counts)
-    counts = [int(v) for v in counts.cpu().tolist()]
-    dense_pairs = counts[0] + counts[2]
-
-    log("timing %d steps" % args.steps)
+    handles = (ctypes.c_void_p * len(ev))(*[e.cuda_event for e in ev])
+    N.check(bd.lib.dh_batch_timing(ctypes.byref(bd.batch), handles), "dh_batch_timing")
+    for _ in range(args.warmup if args.eager_only else 1):
+        step()
     torch.cuda.synchronize()
+    kt = {k: [] for k in range(N.DH_TIME_N)}
+    n_eager = args.steps if args.eager_only else max(3, min(args.steps, 10))
+    log("timing %d eager batched steps with kernel events" % n_eager)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dose, gd, grads = step(True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_step = elapsed / args.steps * 1e3
+    for _ in range(n_eager):
+        step()
+        for k in range(N.DH_TIME_N):
+            kt[k].append((ev[2 * k], ev[2 * k + 1]))
+        torch.cuda.synchronize()   # events are re-recorded by the next step
+        for k in range(N.DH_TIME_N):
+            a, b = kt[k][-1]
+            kt[k][-1] = a.elapsed_time(b)
+    ms_eager = (time.perf_counter() - t0) / n_eager * 1e3
+    N.check(bd.lib.dh_batch_timing(ctypes.byref(bd.batch), None), "dh_batch_timing")
+    kms = {N.TIMED_KERNELS[k]: float(np.mean(v)) for k, v in kt.items()}
+    ms_step = ms_graph if ms_graph is not None else ms_eager
 
-    def avg_ms(key):
-        return float(np.mean([a.elapsed_time(b) for a, b in ev[key]]))
+    # work counts per plan (plan copy 0): evaluated pairs with the culled path, reference pairs with the
+    # plain path's dL/dD; both on the full WET (the dose kernels see the same values where they evaluate)
+    e0 = engines[0]
+    step()
+    gd_cull = bd.grad_doses[0].clone()
+    e0.compute_wet(force=True)      # full WET into plan 0's volumes (the batch's next step redoes its own)
+    counts_cull = torch.zeros(4, dtype=torch.int64, device=dev)
+    for b in e0.beams:
+        D.pb_count_pairs(b.wet, gd_cull, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes,
+                         cutoff_sigma=args.cutoff_sigma, counts=counts_cull)
 
-    t_wet, t_fwd, t_adj = avg_ms("wet"), avg_ms("fwd"), avg_ms("adj")
-    fwd_tflops = counts[1] / nb * FWD_FLOP_PER_PAIR / (t_fwd * 1e-3) / 1e12
-    adj_tflops = counts[3] / nb * ADJ_FLOP_PER_PAIR / (t_adj * 1e-3) / 1e12
-    if t_fwd >= t_adj:
-        dom = {"kernel": "k_pb_forward", "bound": "valu", "achieved": fwd_tflops, "peak": PEAK_FP32_TFLOPS,
-               "unit": "TFLOP/s", "frac": fwd_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-               "avg_launch_ms": t_fwd, "flop_per_pair": FWD_FLOP_PER_PAIR, "pairs_per_launch": counts[1] / nb}
-    else:
-        dom = {"kernel": "k_pb_adjoint", "bound": "valu", "achieved": adj_tflops, "peak": PEAK_FP32_TFLOPS,
-               "unit": "TFLOP/s", "frac": adj_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-               "avg_launch_ms": t_adj, "flop_per_pair": ADJ_FLOP_PER_PAIR, "pairs_per_launch": counts[3] / nb}
-    # ray-march + smoothing: compulsory bytes per voxel per beam: RLSP read 4 + raw WET write 4 + read 4 +
-    # smoothed write 4
-    wet_gbs = 16.0 * nvox / (t_wet * 1e-3) / 1e9
-    roof_wet = {"kernel": "k_wet_raymarch+k_wet_smooth", "bound": "hbm", "achieved": wet_gbs,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": wet_gbs / PEAK_HBM_GBS, "traffic": None,
-                "avg_launch_ms": t_wet}
-
-    # plain path: every spot for every in-range voxel (cutoff 0), same step
+    # plain path: one plan, eager per-beam calls, full WET every step, dense forward + separate adjoint
     plain_ms = None
+    gd_dense = None
+
+    def plain_step():
+        for b in e0.beams:
+            b.wet_raw, b.wet = D.wet_from_padded(e0.rlsp_padded, e0.grid, b.geo, dev, raw_out=b.wet_raw, out=b.wet)
+        dose = torch.empty(e0.shape, dtype=torch.float32, device=dev)
+        for k, b in enumerate(e0.beams):
+            b.spots.set_weights_positions(b.mu0, b.xy0)
+            D.pb_forward(b.wet, dose, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0,
+                         accumulate=k > 0)
+        gd = loss_grad(dose, tmask, bmask, n_t, n_b)
+        for b in e0.beams:
+            D.pb_adjoint(b.wet, gd, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0)
+        return dose, gd
+
     if args.plain_steps > 0:
-        log("plain (dense) path %d steps" % args.plain_steps)
-        eng.cutoff_sigma = 0.0
-        step(False)
+        log("plain path %d steps" % args.plain_steps)
+        _, gd_dense = plain_step()
         torch.cuda.synchronize()
         tp = time.perf_counter()
         for _ in range(args.plain_steps):
-            step(False)
+            _, gd_dense = plain_step()
         torch.cuda.synchronize()
         plain_ms = (time.perf_counter() - tp) / args.plain_steps * 1e3
-        eng.cutoff_sigma = args.cutoff_sigma
+    else:
+        e0.compute_wet(force=True)
+        dose_d = e0.dose()
+        gd_dense = loss_grad(dose_d, tmask, bmask, n_t, n_b)
+    counts_ref = torch.zeros(4, dtype=torch.int64, device=dev)
+    for b in e0.beams:
+        D.pb_count_pairs(b.wet, gd_dense, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes,
+                         cutoff_sigma=args.cutoff_sigma, counts=counts_ref)
+    cc = [int(v) for v in counts_cull.cpu().tolist()]
+    cr = [int(v) for v in counts_ref.cpu().tolist()]
+    ref_pairs_plan = cr[0] + cr[2]           # reference-equivalent pairs per plan
+    fwd_eval, adj_eval = cc[1] * K, cc[3] * K  # evaluated per batched step
+
+    # roofline of the dominant kernel (largest time per batched step)
+    rows = {
+        "k_pb_adjoint_batch": ("valu", adj_eval * ADJ_FLOP_PER_PAIR, ADJ_FLOP_PER_PAIR, adj_eval),
+        "k_pb_forward_batch": ("valu", fwd_eval * FWD_FLOP_PER_PAIR, FWD_FLOP_PER_PAIR, fwd_eval),
+    }
+    dom_name = max(kms, key=lambda k: kms[k])
+    if dom_name in rows:
+        bound, flop, fpp, pairs = rows[dom_name]
+        achieved = flop / (kms[dom_name] * 1e-3) / 1e12
+        roofline = {"kernel": dom_name, "bound": bound, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": load_traffic(dom_name),
+                    "avg_launch_ms": kms[dom_name], "flop_per_pair": fpp, "pairs_per_launch": pairs}
+    else:
+        # WET kernels: compulsory HBM bytes = quad RLSP read (16 B per voxel) + raw WET write (4 B) per beam
+        # for the ray-march; raw read + smoothed write (8 B) per beam for smoothing; all voxels as bound
+        per_vox = 20.0 if dom_name == "k_wet_raymarch_batch" else 8.0
+        byts = per_vox * nvox * nb * K
+        achieved = byts / (kms[dom_name] * 1e-3) / 1e9
+        roofline = {"kernel": dom_name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(dom_name),
+                    "avg_launch_ms": kms[dom_name], "bytes_per_launch": byts}
+    kernels = {}
+    for name, ms in kms.items():
+        r = {"avg_ms_per_step": ms}
+        if name in rows:
+            _, flop, _, pairs = rows[name]
+            r.update({"pairs": pairs, "tflops": flop / (ms * 1e-3) / 1e12,
+                      "frac_fp32_peak": flop / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS})
+        kernels[name] = r
 
     cpu = None
     if args.cpu_sample > 0:
         log("cpu baseline on %d voxels" % args.cpu_sample)
-        cpu = cpu_baseline(g, plan, eng, gd.detach().cpu().numpy(), args.cpu_sample)
+        cpu = cpu_baseline(g, plans[0], e0, gd_dense.detach().cpu().numpy(), args.cpu_sample)
 
+    ms_plan = ms_step / K
     rec = {
         "metric": METRIC,
-        "value": dense_pairs / (ms_step * 1e-3),
+        "value": ref_pairs_plan * K / (ms_step * 1e-3),
         "unit": "spot·voxel evals/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -261,18 +346,20 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "head phantom 160x200x200 @ 2 mm, 3 beams (gantry 0/120/240), 40 layers x 81 spots = "
-                        "9720 spots; step = WET ray-march + smoothing, plan dose, dL/dD, adjoint (weights and "
-                        "positions), all beams",
-            "voxels": nvox, "spots": n_spots, "beams": nb, "cutoff_sigma": args.cutoff_sigma,
-            "path": "plain per-plan (one plan per launch sequence; no batching yet)",
+            "workload": "batch of %d head-phantom plans (160x200x200 @ 2 mm, 3 beams at gantry 0/120/240, "
+                        "40 layers x 81 spots = 9720 spots each); step = one HIP-graph replay: reach-limited "
+                        "WET, plan doses, dL/dD, adjoint (weights and positions) of every plan" % K,
+            "plans": K, "voxels_per_plan": nvox, "spots_per_plan": n_spots, "beams_per_plan": nb,
+            "cutoff_sigma": args.cutoff_sigma, "path": "graph" if ms_graph is not None else "eager",
         },
-        "breakdown_ms": {"wet_per_beam": t_wet, "forward_per_beam": t_fwd, "adjoint_per_beam": t_adj},
-        "pairs_per_step": {"forward_reference": counts[0], "forward_evaluated": counts[1],
-                           "adjoint_reference": counts[2], "adjoint_evaluated": counts[3]},
-        "dense_path_ms_per_step": plain_ms,
-        "roofline": dom,
-        "roofline_wet": roof_wet,
+        "ms_per_plan": ms_plan,
+        "plain_ms_per_plan": plain_ms,
+        "speedup_vs_plain_per_plan": (plain_ms / ms_plan) if plain_ms else None,
+        "eager_ms_per_step": ms_eager,
+        "kernels": kernels,
+        "pairs_per_plan": {"forward_reference": cr[0], "forward_evaluated": cc[1],
+                           "adjoint_reference": cr[2], "adjoint_evaluated": cc[3]},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

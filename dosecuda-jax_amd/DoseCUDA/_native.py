@@ -76,7 +76,15 @@ class dh_batch(ctypes.Structure):
                 ("n_spots", ctypes.c_int64), ("n_layers", ctypes.c_int64),
                 ("n_sum_blocks", ctypes.c_int64), ("n_partials", ctypes.c_int64),
                 ("n_adj32", ctypes.c_int64), ("n_adj64", ctypes.c_int64),
-                ("n_reduce_blocks", ctypes.c_int64), ("tab", ctypes.c_int64 * 16)]
+                ("n_reduce_blocks", ctypes.c_int64), ("tab", ctypes.c_int64 * 16),
+                ("events", ctypes.c_void_p)]
+
+
+DH_TIME_RAYMARCH, DH_TIME_SMOOTH, DH_TIME_FORWARD, DH_TIME_ADJOINT, DH_TIME_NEED = range(5)
+DH_TIME_N = 5
+TIMED_KERNELS = {DH_TIME_RAYMARCH: "k_wet_raymarch_batch", DH_TIME_SMOOTH: "k_wet_smooth_batch",
+                 DH_TIME_FORWARD: "k_pb_forward_batch", DH_TIME_ADJOINT: "k_pb_adjoint_batch",
+                 DH_TIME_NEED: "k_wet_need_batch+k_wet_dilate_batch"}
 
 
 DH_BATCH_TAB_HEADER_END = 13   # dh_batch.tab index: end of the tables uploaded at setup
@@ -113,6 +121,7 @@ SIGNATURES = {
     "dh_batch_wet": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
     "dh_batch_forward": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
     "dh_batch_adjoint": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
+    "dh_batch_timing": (ctypes.c_int, [ctypes.POINTER(dh_batch), _P]),
 }
 
 _lib = None

@@ -336,11 +336,12 @@ extern "C" int dh_batch_wet(const dh_batch *b, int32_t flags, void *stream) {
     if (reach) {
         int rc = (flags & DH_BATCH_REUSE_PREP) ? DH_OK : prep(b, st);
         if (!rc) rc = launch_wet_need_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF), b->n_beams,
-                                            b->n_bricks, st);
+                                            b->n_bricks, st, b->events);
         if (rc) return rc;
     }
     return launch_wet_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF),
-                            table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks, reach, st);
+                            table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks, reach, st,
+                            b->events);
 }
 
 extern "C" int dh_batch_forward(const dh_batch *b, int32_t flags, void *stream) {
@@ -350,7 +351,7 @@ extern "C" int dh_batch_forward(const dh_batch *b, int32_t flags, void *stream) 
     if (rc) return rc;
     return launch_forward_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF), b->n_beams,
                                 b->n_bricks, table<PlanCtx>(b, T_PLANS), table<long long>(b, T_SUM_OFF), b->n_plans,
-                                b->n_sum_blocks, b->cutoff_sigma > 0.0f, st);
+                                b->n_sum_blocks, b->cutoff_sigma > 0.0f, st, b->events);
 }
 
 extern "C" int dh_batch_adjoint(const dh_batch *b, int32_t flags, void *stream) {
@@ -366,5 +367,12 @@ extern "C" int dh_batch_adjoint(const dh_batch *b, int32_t flags, void *stream) 
     return launch_adjoint_batch(table<BeamCtx>(b, T_CTX), b->n_beams, table<long long>(b, T_BRICK_OFF), b->n_bricks,
                                 table<long long>(b, T_PART_OFF), b->n_partials, table<long long>(b, T_ADJ32_OFF),
                                 b->n_adj32, table<long long>(b, T_ADJ64_OFF), b->n_adj64,
-                                table<long long>(b, T_REDUCE_OFF), b->n_reduce_blocks, b->cutoff_sigma > 0.0f, st);
+                                table<long long>(b, T_REDUCE_OFF), b->n_reduce_blocks, b->cutoff_sigma > 0.0f, st,
+                                b->events);
+}
+
+extern "C" int dh_batch_timing(dh_batch *b, void *const *events) {
+    if (!batch_ok(b)) return DH_ERR_ARG;
+    b->events = events;
+    return DH_OK;
 }

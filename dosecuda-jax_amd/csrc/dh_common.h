@@ -212,6 +212,11 @@ struct BeamWs {
     long long bytes;
 };
 
+// Record timing event `slot` (dh_batch_timing) if timing is on.
+inline void mark(void *const *ev, int slot, hipStream_t st) {
+    if (ev && ev[slot]) (void)hipEventRecord((hipEvent_t)ev[slot], st);
+}
+
 // ---------------------------------------------------------------------------
 // host helpers shared by the single-beam and batched entry points
 // dh_wet.hip
@@ -219,7 +224,8 @@ void make_stencil(const dh_beam_geom &b, double s, Stencil &st);
 int make_touch(const Stencil &st, int *out, int cap);
 bool grid_ok(const dh_grid &g);
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
-                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st);
+                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st,
+                     void *const *ev = nullptr);
 // dh_pencil.hip
 int adjoint_chunk(int n_spots, int n_layers);
 int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t);
@@ -233,16 +239,17 @@ float brick_radius(float spacing);
 void fill_pencil_ctx(BeamCtx &c, const dh_grid &g, const dh_beam_geom &geom, const Model &m, const dh_spot_set &ss,
                      const BeamWs &w, float cutoff_sigma);
 int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
-                          hipStream_t st);  // two launches: reach test, then dilation
+                          hipStream_t st, void *const *ev = nullptr);  // two launches: reach test, dilation
 int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long long *layer_off, int n_beams,
                       long long total_spots, long long total_layers, hipStream_t st);
 int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
                          const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
-                         bool cull, hipStream_t st);
+                         bool cull, hipStream_t st, void *const *ev = nullptr);
 int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
                          const long long *part_off, long long total_part, const long long *item_off32,
                          long long total_items32, const long long *item_off64, long long total_items64,
-                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st);
+                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st,
+                         void *const *ev = nullptr);
 
 // ---------------------------------------------------------------------------
 // wave64 helpers

@@ -998,7 +998,8 @@ int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long 
 }
 
 int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
-                          hipStream_t st) {
+                          hipStream_t st, void *const *ev) {
+    mark(ev, 2 * DH_TIME_NEED, st);
     for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
         const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
         hipLaunchKernelGGL(k_wet_need_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
@@ -1013,12 +1014,14 @@ int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_
         const int rc = check_launch("k_wet_dilate_batch");
         if (rc) return rc;
     }
+    mark(ev, 2 * DH_TIME_NEED + 1, st);
     return DH_OK;
 }
 
 int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
                          const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
-                         bool cull, hipStream_t st) {
+                         bool cull, hipStream_t st, void *const *ev) {
+    mark(ev, 2 * DH_TIME_FORWARD, st);
     for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
         const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
         const dim3 grid((unsigned)((n + 3) / 4));
@@ -1031,6 +1034,7 @@ int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_b
         const int rc = check_launch("k_pb_forward_batch");
         if (rc) return rc;
     }
+    mark(ev, 2 * DH_TIME_FORWARD + 1, st);
     if (total_sum_blocks > 0) {
         hipLaunchKernelGGL(k_plan_sum, dim3((unsigned)total_sum_blocks), dim3(256), 0, st, plans, ctx, sum_off,
                            n_plans);
@@ -1059,7 +1063,8 @@ static int launch_adjoint_items(const BeamCtx *ctx, int n_beams, const long long
 int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
                          const long long *part_off, long long total_part, const long long *item_off32,
                          long long total_items32, const long long *item_off64, long long total_items64,
-                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st) {
+                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st,
+                         void *const *ev) {
     int rc;
     for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
         const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
@@ -1072,8 +1077,10 @@ int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick
                            part_off, n_beams, total_part);
         if ((rc = check_launch("k_zero4_batch"))) return rc;
     }
+    mark(ev, 2 * DH_TIME_ADJOINT, st);
     if ((rc = launch_adjoint_items<64>(ctx, n_beams, item_off64, total_items64, cull, st))) return rc;
     if ((rc = launch_adjoint_items<32>(ctx, n_beams, item_off32, total_items32, cull, st))) return rc;
+    mark(ev, 2 * DH_TIME_ADJOINT + 1, st);
     if (total_reduce > 0) {
         hipLaunchKernelGGL(k_pb_reduce_batch, dim3((unsigned)total_reduce), dim3(256), 0, st, ctx, reduce_off,
                            n_beams);

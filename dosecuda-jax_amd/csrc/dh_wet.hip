@@ -453,9 +453,11 @@ extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid
 // Batched launches: ctx is a device table of n_beams entries.
 namespace dh {
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
-                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st) {
+                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st,
+                     void *const *ev) {
     // item spaces above 2^24 bricks go out in several launches
     constexpr long long kMaxItems = 1LL << 24;
+    mark(ev, 2 * DH_TIME_RAYMARCH, st);
     for (long long base = 0; base < total_bricks; base += kMaxItems) {
         const long long n = total_bricks - base < kMaxItems ? total_bricks - base : kMaxItems;
         hipLaunchKernelGGL(k_wet_raymarch_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
@@ -463,6 +465,8 @@ int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long 
         const int rc = check_launch("k_wet_raymarch_batch");
         if (rc) return rc;
     }
+    mark(ev, 2 * DH_TIME_RAYMARCH + 1, st);
+    mark(ev, 2 * DH_TIME_SMOOTH, st);
     for (long long base = 0; base < total_blocks; base += kMaxItems / 4) {
         const long long n = total_blocks - base < kMaxItems / 4 ? total_blocks - base : kMaxItems / 4;
         hipLaunchKernelGGL(k_wet_smooth_batch, dim3((unsigned)n), dim3(256), 0, st, ctx, block_off, n_beams, base,
@@ -470,6 +474,7 @@ int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long 
         const int rc = check_launch("k_wet_smooth_batch");
         if (rc) return rc;
     }
+    mark(ev, 2 * DH_TIME_SMOOTH + 1, st);
     return DH_OK;
 }
 }  // namespace dh

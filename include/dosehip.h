@@ -249,6 +249,7 @@ typedef struct dh_batch {
     int64_t n_reduce_blocks;   /* 64-spot reduction blocks */
     int64_t tab[16];           /* byte offsets of the device tables in ws; tab[13] = end of the tables
                                   uploaded at setup (the rest of ws is scratch the phases rewrite) */
+    void *const *events;       /* kernel timing events (dh_batch_timing), NULL = off */
 } dh_batch;
 
 /* Workspace bytes for a batch (negative on invalid descriptors; see dh_last_error). */
@@ -281,6 +282,19 @@ int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32_t n_plans,
 
 /* WET of every beam: ray-march then smoothing (dh_wet_raymarch + dh_wet_smooth per beam). */
 int dh_batch_wet(const dh_batch *batch, int32_t flags, void *stream);
+
+/* Kernel timing for eager calls (not for graph capture).  events is a HOST
+ * array of 2*DH_TIME_N hipEvent_t (created by the caller; NULL entries are
+ * skipped) that stays valid while timing is on: events[2k] and events[2k+1]
+ * are recorded on the stream right before and right after the launches of
+ * timed kernel k.  events == NULL turns timing off. */
+#define DH_TIME_RAYMARCH 0  /* k_wet_raymarch_batch */
+#define DH_TIME_SMOOTH   1  /* k_wet_smooth_batch */
+#define DH_TIME_FORWARD  2  /* k_pb_forward_batch */
+#define DH_TIME_ADJOINT  3  /* k_pb_adjoint_batch (both chunk sizes) */
+#define DH_TIME_NEED     4  /* k_wet_need_batch + k_wet_dilate_batch */
+#define DH_TIME_N        5
+int dh_batch_timing(dh_batch *batch, void *const *events);
 
 /* Dose of every plan (dh_pb_forward per beam, beam doses added in beam order). */
 int dh_batch_forward(const dh_batch *batch, int32_t flags, void *stream);

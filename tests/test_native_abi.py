@@ -94,7 +94,7 @@ int main(void) {
          sizeof(dh_batch));
   printf("%zu %zu %zu %zu %zu %zu\n", offsetof(dh_batch_plan, beam_start), offsetof(dh_batch_beam, spots),
          offsetof(dh_batch_beam, wet), offsetof(dh_batch_beam, grad_xy), offsetof(dh_batch, n_bricks),
-         offsetof(dh_batch, tab));
+         offsetof(dh_batch, events));
   return 0;
 }
 ''')
@@ -105,7 +105,7 @@ int main(void) {
     offs = [int(v) for v in out[1].split()]
     assert sizes == [ctypes.sizeof(c) for c in (N.dh_batch_plan, N.dh_batch_beam, N.dh_batch_tune, N.dh_batch)]
     assert offs == [N.dh_batch_plan.beam_start.offset, N.dh_batch_beam.spots.offset, N.dh_batch_beam.wet.offset,
-                    N.dh_batch_beam.grad_xy.offset, N.dh_batch.n_bricks.offset, N.dh_batch.tab.offset]
+                    N.dh_batch_beam.grad_xy.offset, N.dh_batch.n_bricks.offset, N.dh_batch.events.offset]
 
 
 def _fake_batch(N, n_spots=(100, 0, 7), grid=(16, 12, 10)):
@@ -167,6 +167,7 @@ def test_batch_sizes_and_errors_without_gpu():
     # phases refuse a batch that was never set up
     assert lib.dh_batch_forward(ctypes.byref(b), 0, None) == N.DH_ERR_ARG
     assert "not set up" in N.last_error()
+    assert lib.dh_batch_timing(ctypes.byref(b), None) == N.DH_ERR_ARG
 
 
 def test_geometry_matches_oracle_restatement():
