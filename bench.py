@@ -153,6 +153,7 @@ def main():
     ap.add_argument("--cutoff-sigma", type=float, default=6.0)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain (dense, per-beam) path; 0 = skip")
     ap.add_argument("--cpu-sample", type=int, default=60000, help="voxels in the CPU-baseline sample; 0 = skip")
+    ap.add_argument("--no-tune", action="store_true", help="library default schedules instead of the autotuner's")
     ap.add_argument("--eager-only", action="store_true",
                     help="time eager batched steps only (no graph), e.g. under a profiler")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -194,7 +195,11 @@ def main():
     nb = len(engines[0].beams)
     n_spots = sum(b.spots.n_spots for b in engines[0].beams)
 
-    bd = BatchedDose(engines, wet_mode="reach")
+    t_tune = time.perf_counter()
+    bd = BatchedDose(engines, wet_mode="reach", tune=None if args.no_tune else "auto")
+    t_tune = time.perf_counter() - t_tune
+    from DoseCUDA.batch import tune_dict
+    log("schedule %s (set-up incl. autotuning %.1f s)" % (tune_dict(bd.tune), t_tune))
     mus = [b.mu0.clone() for e in engines for b in e.beams]
     xys = [b.xy0.clone() for e in engines for b in e.beams]
 
@@ -351,6 +356,7 @@ def main():
                         "WET, plan doses, dL/dD, adjoint (weights and positions) of every plan" % K,
             "plans": K, "voxels_per_plan": nvox, "spots_per_plan": n_spots, "beams_per_plan": nb,
             "cutoff_sigma": args.cutoff_sigma, "path": "graph" if ms_graph is not None else "eager",
+            "schedule": tune_dict(bd.tune), "setup_and_autotune_s": t_tune,
         },
         "ms_per_plan": ms_plan,
         "plain_ms_per_plan": plain_ms,

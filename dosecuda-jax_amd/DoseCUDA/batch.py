@@ -35,14 +35,17 @@ from . import _native as N
 from . import device as D
 
 
-def make_tune(adj_cs=0, adj_wg_target=0, adj_min_bricks=0, adj_max_slabs=0):
-    """dh_batch_tune; zero fields take the library defaults (0, 32768, 64, 1024)."""
-    return N.dh_batch_tune(int(adj_cs), int(adj_wg_target), int(adj_min_bricks), int(adj_max_slabs))
+TUNE_FIELDS = ("adj_cs", "adj_wg_target", "adj_min_bricks", "adj_max_slabs", "fwd_blocks", "ray_blocks")
+
+
+def make_tune(adj_cs=0, adj_wg_target=0, adj_min_bricks=0, adj_max_slabs=0, fwd_blocks=0, ray_blocks=0):
+    """dh_batch_tune; zero fields take the library defaults (0, 32768, 64, 1024, 0, 0)."""
+    return N.dh_batch_tune(int(adj_cs), int(adj_wg_target), int(adj_min_bricks), int(adj_max_slabs),
+                           int(fwd_blocks), int(ray_blocks))
 
 
 def tune_dict(t):
-    return {"adj_cs": t.adj_cs, "adj_wg_target": t.adj_wg_target, "adj_min_bricks": t.adj_min_bricks,
-            "adj_max_slabs": t.adj_max_slabs}
+    return {f: int(getattr(t, f)) for f in TUNE_FIELDS}
 
 
 class BatchedDose:
@@ -53,7 +56,8 @@ class BatchedDose:
     keep_beam_doses: give every beam a dose volume of its own (``beam_doses``),
     as the reference's ``IMPTDoseGrid.beam_doses``; otherwise beams of
     multi-beam plans write workspace scratch.
-    tune: dh_batch_tune (adjoint schedule) or None for the defaults.
+    tune: dh_batch_tune, None for the defaults, or "auto" for the autotuner's
+    schedule for this batch shape (DoseCUDA.autotune; tuned on a cache miss).
     wet_mode: "full" or "reach" (see the module notes).
     """
 
@@ -70,6 +74,11 @@ class BatchedDose:
             if e.device != self.device:
                 raise ValueError("all plans of a batch must live on one device")
         self.cutoff_sigma = float(self.engines[0].cutoff_sigma if cutoff_sigma is None else cutoff_sigma)
+        if isinstance(tune, str):
+            if tune != "auto":
+                raise ValueError("tune must be a dh_batch_tune, None or 'auto'")
+            from .autotune import get_tune
+            tune, _ = get_tune(self.engines, self.cutoff_sigma, wet_mode)
         self.tune = tune if tune is not None else make_tune()
         dev = self.device
         self.doses, self.grad_doses, self.beam_doses = [], [], []

@@ -261,6 +261,12 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
         return DH_ERR_ARG;
     }
     memset(batch, 0, sizeof *batch);
+    if (tune && (tune->fwd_blocks < 0 || tune->ray_blocks < 0 || tune->fwd_blocks > (1 << 20) ||
+                 tune->ray_blocks > (1 << 20))) {
+        set_error("dh_batch_setup: invalid brick schedule (fwd_blocks=%d, ray_blocks=%d)", tune->fwd_blocks,
+                  tune->ray_blocks);
+        return DH_ERR_ARG;
+    }
     const AdjTune t = tune_from(tune);
     Layout L;
     if (!plan_layout(plans, n_plans, beams, n_beams, t, cutoff_sigma, nullptr, L)) return DH_ERR_ARG;
@@ -315,6 +321,8 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     batch->n_reduce_blocks = D.n_reduce;
     for (int i = 0; i < T_FLAGS; ++i) batch->tab[i] = D.tab[i];
     batch->tab[T_FLAGS] = D.flags;
+    batch->fwd_blocks = tune && tune->fwd_blocks > 0 ? tune->fwd_blocks : 0;
+    batch->ray_blocks = tune && tune->ray_blocks > 0 ? tune->ray_blocks : 0;
     batch->magic = kMagic;
     return DH_OK;
 }
@@ -340,8 +348,8 @@ extern "C" int dh_batch_wet(const dh_batch *b, int32_t flags, void *stream) {
         if (rc) return rc;
     }
     return launch_wet_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF),
-                            table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks, reach, st,
-                            b->events);
+                            table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks, reach,
+                            b->ray_blocks, st, b->events);
 }
 
 extern "C" int dh_batch_forward(const dh_batch *b, int32_t flags, void *stream) {
@@ -351,7 +359,7 @@ extern "C" int dh_batch_forward(const dh_batch *b, int32_t flags, void *stream) 
     if (rc) return rc;
     return launch_forward_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF), b->n_beams,
                                 b->n_bricks, table<PlanCtx>(b, T_PLANS), table<long long>(b, T_SUM_OFF), b->n_plans,
-                                b->n_sum_blocks, b->cutoff_sigma > 0.0f, st, b->events);
+                                b->n_sum_blocks, b->cutoff_sigma > 0.0f, b->fwd_blocks, st, b->events);
 }
 
 extern "C" int dh_batch_adjoint(const dh_batch *b, int32_t flags, void *stream) {

@@ -196,6 +196,12 @@ struct AdjTune {
     int cs, wg_target, min_bricks, max_slabs;
 };
 
+// Schedules of the batched brick kernels: 0 = one wave per brick (hardware
+// dispatch), n > 0 = a persistent grid of n 256-thread blocks.
+struct BrickTune {
+    int fwd_blocks, ray_blocks;
+};
+
 // Number of cs-spot adjoint chunks provided for a beam: an upper bound of the
 // sum over layers of ceil(n_l / cs).
 __host__ __device__ inline long long adjoint_chunks(int n_spots, int n_layers, int cs) {
@@ -224,8 +230,8 @@ void make_stencil(const dh_beam_geom &b, double s, Stencil &st);
 int make_touch(const Stencil &st, int *out, int cap);
 bool grid_ok(const dh_grid &g);
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
-                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st,
-                     void *const *ev = nullptr);
+                     long long total_bricks, long long total_blocks, bool use_need, int persist_blocks,
+                     hipStream_t st, void *const *ev = nullptr);
 // dh_pencil.hip
 int adjoint_chunk(int n_spots, int n_layers);
 int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t);
@@ -244,7 +250,7 @@ int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long 
                       long long total_spots, long long total_layers, hipStream_t st);
 int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
                          const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
-                         bool cull, hipStream_t st, void *const *ev = nullptr);
+                         bool cull, int persist_blocks, hipStream_t st, void *const *ev = nullptr);
 int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
                          const long long *part_off, long long total_part, const long long *item_off32,
                          long long total_items32, const long long *item_off64, long long total_items64,

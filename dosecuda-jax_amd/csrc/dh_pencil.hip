@@ -446,19 +446,39 @@ __global__ __launch_bounds__(256) void k_pb_forward(BeamCtx c, int accumulate,
 
 // Batched forward: one wave per brick of every beam; beam b owns items
 // [off[b], off[b + 1]) and writes its own dose volume (the plan's dose when the
-// plan has one beam, else a per-beam volume that k_plan_sum adds up).
+// plan has one beam, else a per-beam volume that k_plan_sum adds up).  A wave
+// takes item base + its wave index, then steps by `stride`: stride >= total
+// gives one brick per wave (hardware dispatch), stride = 4 * gridDim.x a
+// persistent grid that strides over all bricks.  Either way each brick is
+// computed by exactly one wave, so the results are the same.
+// The persistent form is its own instantiation: the loop around the brick
+// body costs registers (80 VGPRs instead of 62), which the one-shot form
+// should not pay.
+__device__ __forceinline__ long long first_item(long long base) {
+    return base + (long long)blockIdx.x * 4 + wave_in_block();
+}
+
 template <bool CULL>
-__global__ __launch_bounds__(256) void k_pb_forward_batch(const BeamCtx *__restrict__ ctx,
-                                                         const long long *__restrict__ off, int n_beams,
-                                                         long long base, long long total) {
-    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
-    if (item >= total) return;
+__device__ __forceinline__ void forward_item(const BeamCtx *__restrict__ ctx, const long long *__restrict__ off,
+                                             int n_beams, long long item) {
     const int bm = find_owner(off, n_beams, item);
     const BeamCtx &c = load_ctx(ctx, bm);
     VoxelOf vo;
     unsigned long long nd = 0, ne = 0;
     const float acc = forward_brick<CULL, false>(c, item - off[bm], threadIdx.x & 63, vo, nd, ne);
     if (vo.valid) gst(c.dose, vo.idx, acc);
+}
+
+template <bool CULL, bool PERSIST>
+__global__ __launch_bounds__(256) void k_pb_forward_batch(const BeamCtx *__restrict__ ctx,
+                                                         const long long *__restrict__ off, int n_beams,
+                                                         long long base, long long total, long long stride) {
+    if (PERSIST) {
+        for (long long item = first_item(base); item < total; item += stride) forward_item<CULL>(ctx, off, n_beams, item);
+    } else {
+        const long long item = first_item(base);
+        if (item < total) forward_item<CULL>(ctx, off, n_beams, item);
+    }
 }
 
 // Dose of a plan whose beams write volumes of their own: the sum of the beam
@@ -550,8 +570,11 @@ __device__ __forceinline__ void wet_dilate_one(const BeamCtx &c, long long brick
                 raw |= (gld(c.need, si + (long long)c.bg.bx * (sj + (long long)c.bg.by * sk)) & 1u) != 0;
         }
     }
-    // only bit 1 of this brick's own byte changes; other waves read bit 0 only
-    if (lane == 0 && __ballot(raw)) gst(c.need, brick, (unsigned char)(own | 2u));
+    // the vote runs in every lane (a ballot inside `lane == 0 && ...` would see
+    // lane 0 only); only bit 1 of this brick's own byte changes, other waves
+    // read bit 0 only
+    const bool any = __ballot(raw) != 0ull;
+    if (lane == 0 && any) gst(c.need, brick, (unsigned char)(own | 2u));
 }
 
 __global__ __launch_bounds__(256) void k_wet_need_batch(const BeamCtx *__restrict__ ctx,
@@ -1020,17 +1043,29 @@ int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_
 
 int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
                          const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
-                         bool cull, hipStream_t st, void *const *ev) {
+                         bool cull, int persist_blocks, hipStream_t st, void *const *ev) {
     mark(ev, 2 * DH_TIME_FORWARD, st);
-    for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
+    if (persist_blocks > 0 && total_bricks > 0) {
+        const long long nb = (total_bricks + 3) / 4 < persist_blocks ? (total_bricks + 3) / 4 : persist_blocks;
+        const long long stride = 4 * nb;
+        if (cull)
+            hipLaunchKernelGGL((k_pb_forward_batch<true, true>), dim3((unsigned)nb), dim3(256), 0, st, ctx, brick_off,
+                               n_beams, 0LL, total_bricks, stride);
+        else
+            hipLaunchKernelGGL((k_pb_forward_batch<false, true>), dim3((unsigned)nb), dim3(256), 0, st, ctx,
+                               brick_off, n_beams, 0LL, total_bricks, stride);
+        const int rc = check_launch("k_pb_forward_batch");
+        if (rc) return rc;
+    }
+    for (long long base = 0; persist_blocks <= 0 && base < total_bricks; base += 4 * kMaxBlocks) {
         const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
         const dim3 grid((unsigned)((n + 3) / 4));
         if (cull)
-            hipLaunchKernelGGL(k_pb_forward_batch<true>, grid, dim3(256), 0, st, ctx, brick_off, n_beams, base,
-                               total_bricks);
+            hipLaunchKernelGGL((k_pb_forward_batch<true, false>), grid, dim3(256), 0, st, ctx, brick_off, n_beams,
+                               base, total_bricks, total_bricks);
         else
-            hipLaunchKernelGGL(k_pb_forward_batch<false>, grid, dim3(256), 0, st, ctx, brick_off, n_beams, base,
-                               total_bricks);
+            hipLaunchKernelGGL((k_pb_forward_batch<false, false>), grid, dim3(256), 0, st, ctx, brick_off, n_beams,
+                               base, total_bricks, total_bricks);
         const int rc = check_launch("k_pb_forward_batch");
         if (rc) return rc;
     }

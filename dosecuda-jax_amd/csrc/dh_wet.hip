@@ -202,18 +202,32 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(BeamCtx c, unsigned nblock
 }
 
 // Batched: bricks of every beam in one launch; beam b owns bricks
-// [off[b], off[b + 1]) of the item space.
-__global__ __launch_bounds__(256) void k_wet_raymarch_batch(const BeamCtx *__restrict__ ctx,
-                                                           const long long *__restrict__ off, int n_beams,
-                                                           long long base, long long total, int use_need) {
-    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
-    if (item >= total) return;
+// [off[b], off[b + 1]) of the item space.  Waves step over items by `stride`
+// (one brick per wave when stride >= total, a persistent grid when stride =
+// 4 * gridDim.x).
+__device__ __forceinline__ void raymarch_item(const BeamCtx *__restrict__ ctx, const long long *__restrict__ off,
+                                              int n_beams, long long item, int use_need) {
     const int bm = find_owner(off, n_beams, item);
     const BeamCtx &c = load_ctx(ctx, bm);
     const long long brick = item - off[bm];
     // reach-limited pass: bricks no smoothed voxel reads stay unwritten
     if (use_need && !(gld(c.need, brick) & 2u)) return;
     raymarch_brick(c, brick, threadIdx.x & 63);
+}
+
+// The persistent form (PERSIST) is its own instantiation so that the one-shot
+// form keeps the registers of the plain kernel.
+template <bool PERSIST>
+__global__ __launch_bounds__(256) void k_wet_raymarch_batch(const BeamCtx *__restrict__ ctx,
+                                                           const long long *__restrict__ off, int n_beams,
+                                                           long long base, long long total, int use_need,
+                                                           long long stride) {
+    long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
+    if (PERSIST) {
+        for (; item < total; item += stride) raymarch_item(ctx, off, n_beams, item, use_need);
+    } else if (item < total) {
+        raymarch_item(ctx, off, n_beams, item, use_need);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -453,15 +467,22 @@ extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid
 // Batched launches: ctx is a device table of n_beams entries.
 namespace dh {
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
-                     long long total_bricks, long long total_blocks, bool use_need, hipStream_t st,
-                     void *const *ev) {
+                     long long total_bricks, long long total_blocks, bool use_need, int persist_blocks,
+                     hipStream_t st, void *const *ev) {
     // item spaces above 2^24 bricks go out in several launches
     constexpr long long kMaxItems = 1LL << 24;
     mark(ev, 2 * DH_TIME_RAYMARCH, st);
-    for (long long base = 0; base < total_bricks; base += kMaxItems) {
+    if (persist_blocks > 0 && total_bricks > 0) {
+        const long long nb = (total_bricks + 3) / 4 < persist_blocks ? (total_bricks + 3) / 4 : persist_blocks;
+        hipLaunchKernelGGL(k_wet_raymarch_batch<true>, dim3((unsigned)nb), dim3(256), 0, st, ctx, brick_off,
+                           n_beams, 0LL, total_bricks, use_need ? 1 : 0, 4 * nb);
+        const int rc = check_launch("k_wet_raymarch_batch");
+        if (rc) return rc;
+    }
+    for (long long base = 0; persist_blocks <= 0 && base < total_bricks; base += kMaxItems) {
         const long long n = total_bricks - base < kMaxItems ? total_bricks - base : kMaxItems;
-        hipLaunchKernelGGL(k_wet_raymarch_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
-                           n_beams, base, total_bricks, use_need ? 1 : 0);
+        hipLaunchKernelGGL(k_wet_raymarch_batch<false>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx,
+                           brick_off, n_beams, base, total_bricks, use_need ? 1 : 0, total_bricks);
         const int rc = check_launch("k_wet_raymarch_batch");
         if (rc) return rc;
     }

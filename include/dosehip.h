@@ -219,17 +219,23 @@ typedef struct dh_batch_beam {
     float *grad_xy;            /* [2 n_spots] dL/dx, dL/dy */
 } dh_batch_beam;
 
-/* Adjoint schedule, chosen per batch (e.g. by an autotuner):
+/* Schedules, chosen per batch (e.g. by an autotuner):
  *   adj_cs          spots per adjoint workgroup: 32, 64, or 0 (chosen per beam from its layer sizes)
  *   adj_wg_target   workgroups per beam to aim for (bricks are cut into slabs to reach it)
  *   adj_min_bricks  fewest bricks per slab
  *   adj_max_slabs   most slabs per beam (<= 65535)
- * Zero fields take the defaults (0, 32768, 64, 1024). */
+ *   fwd_blocks      forward bricks: 0 = one wave per brick, n > 0 = a persistent grid of n 256-thread
+ *                   blocks striding over the bricks of all beams
+ *   ray_blocks      the same for the ray-march
+ * Zero fields take the defaults (0, 32768, 64, 1024, 0, 0).  Results do not depend on fwd_blocks and
+ * ray_blocks; the adjoint fields change only the order of partial sums. */
 typedef struct dh_batch_tune {
     int32_t adj_cs;
     int32_t adj_wg_target;
     int32_t adj_min_bricks;
     int32_t adj_max_slabs;
+    int32_t fwd_blocks;
+    int32_t ray_blocks;
 } dh_batch_tune;
 
 /* Filled by dh_batch_setup; plain data owned by the caller, read-only to it.
@@ -249,6 +255,7 @@ typedef struct dh_batch {
     int64_t n_reduce_blocks;   /* 64-spot reduction blocks */
     int64_t tab[16];           /* byte offsets of the device tables in ws; tab[13] = end of the tables
                                   uploaded at setup (the rest of ws is scratch the phases rewrite) */
+    int32_t fwd_blocks, ray_blocks;   /* brick-kernel schedules from dh_batch_tune */
     void *const *events;       /* kernel timing events (dh_batch_timing), NULL = off */
 } dh_batch;
 

@@ -171,7 +171,8 @@ def test_batch_graph_replay(dev):
         _say("replay %d ok" % rep)
 
 
-@pytest.mark.parametrize("tune", [(32, 0, 0, 0), (64, 0, 0, 0), (0, 512, 16, 8), (32, 200000, 8, 4000)])
+@pytest.mark.parametrize("tune", [(32, 0, 0, 0), (64, 0, 0, 0), (0, 512, 16, 8), (32, 200000, 8, 4000),
+                                  (0, 0, 0, 0, 7, 3)])
 def test_batch_adjoint_schedules(dev, tune):
     from DoseCUDA.batch import BatchedDose, make_tune
     _say("adjoint schedule %s" % (tune,))
@@ -237,8 +238,73 @@ def test_batch_reach_limited_wet(dev, cutoff):
             assert torch.equal(a, b), "replay %d dose" % rep
         for (a0, a1), (b0, b1) in zip(got_a, want_a):
             assert torch.equal(a0, b0) and torch.equal(a1, b1), "replay %d gradients" % rep
+        # every voxel holds the marker or exactly the full WET (no raw WET read unwritten)
+        for e_f, e_r in zip(engines, engines_r):
+            for bf, br in zip(e_f.beams, e_r.beams):
+                keep = ~(br.wet >= 1e29)      # NaN (poisoned scratch read) stays in the comparison
+                assert torch.equal(br.wet[keep], bf.wet[keep]), "replay %d: reach WET != full WET" % rep
         skipped = sum(int((b.wet >= 1e29).sum()) for e in engines_r for b in e.beams)
         total = sum(b.wet.numel() for e in engines_r for b in e.beams)
         _say("replay %d ok, WET skipped in %d of %d voxels" % (rep, skipped, total))
         if cutoff == 0.0:
             assert skipped == 0
+
+
+def test_batch_persistent_brick_schedules(dev):
+    """Persistent forward / ray-march grids compute every brick once: WET, dose
+    and gradients equal the one-wave-per-brick schedule bit for bit."""
+    from DoseCUDA.batch import BatchedDose, make_tune
+    _say("persistent brick schedules")
+    grads = _grads(_engines(dev)[:3], seed=8)
+    out = []
+    for tune in (make_tune(), make_tune(fwd_blocks=5, ray_blocks=3), make_tune(fwd_blocks=4096, ray_blocks=4096)):
+        engines = _engines(dev)[:3]
+        bd = BatchedDose(engines, tune=tune, wet_mode="reach")
+        d = [x.clone() for x in bd.forward()]
+        a = [(m.clone(), x.clone()) for m, x in bd.adjoint(grads)]
+        w = [b.wet.clone() for e in engines for b in e.beams]
+        out.append((d, a, w))
+    torch.cuda.synchronize()
+    for d, a, w in out[1:]:
+        for x, y in zip(d, out[0][0]):
+            assert torch.equal(x, y)
+        for (x0, x1), (y0, y1) in zip(a, out[0][1]):
+            assert torch.equal(x0, y0) and torch.equal(x1, y1)
+        for x, y in zip(w, out[0][2]):
+            assert torch.equal(x, y)
+
+
+def test_autotuner_picks_and_caches(dev, tmp_path, monkeypatch):
+    """The autotuner returns a valid schedule, stores it, and the next batch of
+    the same shape takes it from the cache without tuning; results with the
+    tuned schedule match the default one (bit for bit where only brick
+    schedules change, 1e-5 for the adjoint)."""
+    from DoseCUDA import autotune as A
+    from DoseCUDA.batch import BatchedDose, tune_dict
+    _say("autotuner")
+    monkeypatch.setenv("DOSEHIP_TUNE_CACHE", str(tmp_path / "tune.json"))
+    engines = _engines(dev)[:3]
+    bd = BatchedDose(engines, tune="auto", wet_mode="reach")
+    t = tune_dict(bd.tune)
+    _say("tuned %s" % t)
+    assert t["adj_cs"] in (32, 64) and t["fwd_blocks"] in A.FWD_BLOCKS and t["ray_blocks"] in A.RAY_BLOCKS
+    assert A.lookup(A.signature(engines, 6.0, "reach")) == t
+
+    def boom(*a, **k):
+        raise AssertionError("cache miss")
+    monkeypatch.setattr(A, "tune", boom)
+    bd2 = BatchedDose(_engines(dev)[:3], tune="auto", wet_mode="reach")
+    assert tune_dict(bd2.tune) == t
+    grads = _grads(engines, seed=4)
+    ref = BatchedDose(_engines(dev)[:3], wet_mode="reach")
+    d0 = [x.clone() for x in ref.forward()]
+    a0 = [(m.clone(), x.clone()) for m, x in ref.adjoint(grads)]
+    d1 = bd.forward()
+    a1 = bd.adjoint(grads)
+    torch.cuda.synchronize()
+    for x, y in zip(d1, d0):
+        assert torch.equal(x, y)
+    for (x0, x1), (y0, y1) in zip(a1, a0):
+        if y0.numel():
+            assert float((x0 - y0).abs().max()) <= 1e-5 * float(y0.abs().max())
+            assert float((x1 - y1).abs().max()) <= 1e-5 * float(y1.abs().max())

@@ -141,15 +141,15 @@ def test_batch_sizes_and_errors_without_gpu():
     from DoseCUDA import _native as N
     lib = N.load()
     plans, beams = _fake_batch(N)
-    tune = N.dh_batch_tune(0, 0, 0, 0)
+    tune = N.dh_batch_tune(0, 0, 0, 0, 0, 0)
     n = lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(tune))
     assert n > 0 and n % 256 == 0
     # more slabs -> more adjoint partials -> larger workspace
-    big = N.dh_batch_tune(0, 1 << 20, 1, 70000)
+    big = N.dh_batch_tune(0, 1 << 20, 1, 70000, 0, 0)
     assert lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(big)) == N.DH_ERR_ARG  # over 65535
-    big = N.dh_batch_tune(0, 1 << 20, 1, 60000)
+    big = N.dh_batch_tune(0, 1 << 20, 1, 60000, 0, 0)
     assert lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(big)) > n
-    bad = N.dh_batch_tune(48, 0, 0, 0)
+    bad = N.dh_batch_tune(48, 0, 0, 0, 0, 0)
     assert lib.dh_batch_workspace_size(plans, 2, beams, 3, ctypes.byref(bad)) == N.DH_ERR_ARG
     assert "schedule" in N.last_error()
     # beams must be listed in order without gaps
@@ -164,6 +164,10 @@ def test_batch_sizes_and_errors_without_gpu():
     b = N.dh_batch()
     rc = lib.dh_batch_setup(ctypes.byref(b), plans, 2, beams, 3, None, 6.0, ctypes.c_void_p(4096), 256, None)
     assert rc == N.DH_ERR_ARG and "workspace too small" in N.last_error()
+    neg = N.dh_batch_tune(0, 0, 0, 0, -1, 0)
+    rc = lib.dh_batch_setup(ctypes.byref(b), plans, 2, beams, 3, ctypes.byref(neg), 6.0, ctypes.c_void_p(4096),
+                            1 << 40, None)
+    assert rc == N.DH_ERR_ARG and "brick schedule" in N.last_error()
     # phases refuse a batch that was never set up
     assert lib.dh_batch_forward(ctypes.byref(b), 0, None) == N.DH_ERR_ARG
     assert "not set up" in N.last_error()
