@@ -12,9 +12,48 @@
 // and no per-corner bounds tests.  On gfx950 the ray-march is bound by the
 // texture-address rate of its gathers (one address per lane per load), so two
 // 16-byte loads per sample instead of four 8-byte ones halve that work.
+//
+// Block table (after the quads): one float per 4x4x4 block of trilinear cells
+// (cells with floor(c) in [4b, 4b + 3] per axis): the common value when every
+// corner those cells touch (voxels 4b .. 4b + 4, zero past the high end) holds
+// the same rho, NaN otherwise.  Every sample whose cell lies in a uniform block
+// is exactly that value, so the march adds a run of such samples at once
+// (empty-space skipping; air around the patient is one value because the HU
+// table clamps at -1000 HU).
 #include "dh_common.h"
 
 namespace dh {
+
+// Blocks of 4x4x4 cells per axis and the table's position after the quads.
+__host__ __device__ inline long long quad_count(int nx, int ny, int nz) { return (long long)nx * ny * (nz + 1); }
+__host__ __device__ inline long long block_count(int nx, int ny, int nz) {
+    return (long long)((nx + 3) / 4) * ((ny + 3) / 4) * ((nz + 3) / 4);
+}
+
+// One thread per block of cells: compare every corner the block's cells touch
+// (quads i in [4bx, min(4bx + 3, nx - 1)], same for j, k in [4bz, min(4bz + 4,
+// nz)], all four lanes of each) with the first one.
+__global__ __launch_bounds__(256) void k_rlsp_blocks(const float4 *__restrict__ q, float *__restrict__ table, Grid g,
+                                                    long long nblk) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk) return;
+    const int nbx = (g.nx + 3) / 4, nby = (g.ny + 3) / 4;
+    const int bx = (int)(t % nbx);
+    const long long r = t / nbx;
+    const int by = (int)(r % nby), bz = (int)(r / nby);
+    const int i0 = 4 * bx, i1 = min(4 * bx + 3, g.nx - 1);
+    const int j0 = 4 * by, j1 = min(4 * by + 3, g.ny - 1);
+    const int k0 = 4 * bz, k1 = min(4 * bz + 4, g.nz);
+    const float v = q[i0 + (long long)g.nx * (j0 + (long long)g.ny * k0)].x;
+    bool same = v == v;
+    for (int k = k0; k <= k1 && same; ++k)
+        for (int j = j0; j <= j1 && same; ++j)
+            for (int i = i0; i <= i1 && same; ++i) {
+                const float4 e = q[i + (long long)g.nx * (j + (long long)g.ny * k)];
+                same = e.x == v && e.y == v && e.z == v && e.w == v;
+            }
+    table[t] = same ? v : __int_as_float(0x7fc00000);
+}
 
 // ---------------------------------------------------------------------------
 // HU -> RLSP (IMPTDoseGrid.RLSPFromHU, plan_impt.py:158-168)
@@ -154,21 +193,46 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)qv, (short)0, (int)((unsigned)nxy * (unsigned)(g.nz + 1) * 16u), 0x00020000);
     const int so_z = nxy * 16;
+    // block table after the quads: rho of uniform 4x4x4 cell blocks, NaN elsewhere
+    const float *ub = reinterpret_cast<const float *>(qv + quad_count(g.nx, g.ny, g.nz));
+    const int nbx = (g.nx + 3) >> 2, nby = (g.ny + 3) >> 2;
+    const float rdx = 1.0f / fabsf(dux), rdy = 1.0f / fabsf(duy), rdz = 1.0f / fabsf(duz);
     float acc0 = -0.05f, acc1 = 0.0f;
     int n = 0;
-    // two samples per iteration: independent loads in flight, two accumulators
-    for (; n + 1 < n_safe; n += 2) {
-        const float f0 = (float)n, f1 = (float)(n + 1);
-        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), g.nx, nxy, so_z);
-        const float v1 = trilinear_buf(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z), g.nx, nxy, so_z);
-        acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
-        acc1 = fmaf(fmaxf(v1, 0.0f), 0.1f, acc1);
-    }
-    if (n < n_safe) {
+    while (n < n_safe) {
         const float f0 = (float)n;
-        const float v0 = trilinear_buf(rs, fmaf(dux, f0, c0x), fmaf(duy, f0, c0y), fmaf(duz, f0, c0z), g.nx, nxy, so_z);
+        const float cx = fmaf(dux, f0, c0x), cy = fmaf(duy, f0, c0y), cz = fmaf(duz, f0, c0z);
+        // inside a uniform block every sample is the block's value: count the
+        // samples n .. n + T that stay in it (one step of margin against the
+        // rounding of the positions) and add them at once
+        const int ix = (int)cx, iy = (int)cy, iz = (int)cz;  // c >= 0 here: truncation is floor
+        const float u = gld(ub, (ix >> 2) + (long long)nbx * ((iy >> 2) + (long long)nby * (iz >> 2)));
+        if (u == u) {
+            const float tx = dux > 0.0f ? ((float)((ix | 3) + 1) - cx) * rdx
+                                        : (dux < 0.0f ? (cx - (float)(ix & ~3)) * rdx : 3.0e38f);
+            const float ty = duy > 0.0f ? ((float)((iy | 3) + 1) - cy) * rdy
+                                        : (duy < 0.0f ? (cy - (float)(iy & ~3)) * rdy : 3.0e38f);
+            const float tz = duz > 0.0f ? ((float)((iz | 3) + 1) - cz) * rdz
+                                        : (duz < 0.0f ? (cz - (float)(iz & ~3)) * rdz : 3.0e38f);
+            const int T = min((int)fminf(fminf(fminf(tx, ty), tz), 1.0e9f) - 1, n_safe - 1 - n);
+            if (T >= 1) {
+                acc1 = fmaf(fmaxf(u, 0.0f) * (float)(T + 1), 0.1f, acc1);
+                n += T + 1;
+                continue;
+            }
+        }
+        // two samples: independent loads in flight, two accumulators
+        const float v0 = trilinear_buf(rs, cx, cy, cz, g.nx, nxy, so_z);
         acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
-        ++n;
+        if (n + 1 < n_safe) {
+            const float f1 = (float)(n + 1);
+            const float v1 =
+                trilinear_buf(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z), g.nx, nxy, so_z);
+            acc1 = fmaf(fmaxf(v1, 0.0f), 0.1f, acc1);
+            n += 2;
+        } else {
+            n += 1;
+        }
     }
     float acc = acc0 + acc1;
     // tail: the reference's per-sample position arithmetic and bounds test
@@ -404,6 +468,14 @@ bool grid_ok(const dh_grid &g) {
 
 using namespace dh;
 
+// Block table of a quad volume (written right after its quads).
+static int launch_blocks(float *rlsp_padded, const dh_grid &g, hipStream_t st) {
+    const long long nquad = quad_count(g.nx, g.ny, g.nz), nblk = block_count(g.nx, g.ny, g.nz);
+    hipLaunchKernelGGL(k_rlsp_blocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<const float4 *>(rlsp_padded), rlsp_padded + 4 * nquad, make_grid(g), nblk);
+    return check_launch("k_rlsp_blocks");
+}
+
 extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, const double *hu_knots,
                                const double *rlsp_knots, int32_t n_knots, void *stream) {
     if (!grid_ok(g)) return DH_ERR_ARG;
@@ -426,7 +498,8 @@ extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, c
     const unsigned nb = (unsigned)((nquad + 255) / 256);
     hipLaunchKernelGGL(k_rlsp_from_hu, dim3(nb), dim3(256), 0, (hipStream_t)stream, hu,
                        reinterpret_cast<float4 *>(rlsp_padded), make_grid(g), kn, nquad);
-    return check_launch("k_rlsp_from_hu");
+    const int rc = check_launch("k_rlsp_from_hu");
+    return rc ? rc : launch_blocks(rlsp_padded, g, (hipStream_t)stream);
 }
 
 extern "C" int dh_rlsp_pad(const float *rlsp, float *rlsp_padded, dh_grid g, void *stream) {
@@ -439,7 +512,8 @@ extern "C" int dh_rlsp_pad(const float *rlsp, float *rlsp_padded, dh_grid g, voi
     const unsigned nb = (unsigned)((nquad + 255) / 256);
     hipLaunchKernelGGL(k_rlsp_pad, dim3(nb), dim3(256), 0, (hipStream_t)stream, rlsp,
                        reinterpret_cast<float4 *>(rlsp_padded), make_grid(g), nquad);
-    return check_launch("k_rlsp_pad");
+    const int rc = check_launch("k_rlsp_pad");
+    return rc ? rc : launch_blocks(rlsp_padded, g, (hipStream_t)stream);
 }
 
 extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid g, const dh_beam_geom *geom,

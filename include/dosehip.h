@@ -17,7 +17,12 @@
  *  - The ray-march reads RLSP in a "quad" layout: element (i, j, k) with
  *    i < nx, j < ny, k <= nz, index i + nx*(j + ny*k), is the float4
  *    (rho(i,j,k), rho(i+1,j,k), rho(i,j+1,k), rho(i+1,j+1,k)), rho = 0 past
- *    the high end of each axis; 4*nx*ny*(nz+1) floats, 16-byte aligned.
+ *    the high end of each axis; 4*nx*ny*(nz+1) floats, 16-byte aligned,
+ *    followed by a table of ceil(nx/4)*ceil(ny/4)*ceil(nz/4) floats, one per
+ *    4x4x4 block of trilinear cells: the rho every corner of the block's cells
+ *    holds when they are all equal, NaN otherwise (the ray-march adds runs of
+ *    samples in uniform blocks at once).  dh_rlsp_from_hu / dh_rlsp_pad write
+ *    both parts; dh_padded_len gives the total.
  *  - Return value 0 = success; negative = error, message via dh_last_error().
  *  - Entry points are re-entrant; per-thread error text.
  */
@@ -106,7 +111,8 @@ int dh_version(void);
  * truncated to len).  Returns the length of the full message. */
 int dh_last_error(char *buf, size_t len);
 
-/* Number of floats of a quad RLSP volume: 4*nx*ny*(nz+1). */
+/* Number of floats of a quad RLSP volume with its block table:
+ * 4*nx*ny*(nz+1) + ceil(nx/4)*ceil(ny/4)*ceil(nz/4). */
 int64_t dh_padded_len(dh_grid g);
 
 /* HU -> RLSP, piecewise linear through (hu_knots, rlsp_knots), clamped at both

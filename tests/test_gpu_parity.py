@@ -90,11 +90,67 @@ def test_rlsp_from_hu_matches_np_interp(dev):
     grid = D.make_grid(hu.shape, 2.0)
     out = D.rlsp_from_hu(torch.from_numpy(hu).to(dev), grid, hk, rk, dev).cpu().numpy()
     nz, ny, nx = hu.shape
-    q = out.reshape(nz + 1, ny, nx, 4)
+    q = out[: 4 * nx * ny * (nz + 1)].reshape(nz + 1, ny, nx, 4)
     pad = np.zeros((nz + 1, ny + 1, nx + 1), np.float32)
     pad[:nz, :ny, :nx] = ref
     expect = np.stack([pad[:, :ny, :nx], pad[:, :ny, 1:], pad[:, 1:, :nx], pad[:, 1:, 1:]], axis=-1)
     np.testing.assert_array_max_ulp(q, expect, maxulp=1)
+
+
+def _block_table_np(rho):
+    """Reference block table: per 4x4x4 block of cells, the common value of the
+    corners its cells touch (zero past the high end), NaN if they differ."""
+    nz, ny, nx = rho.shape
+    pad = np.zeros((nz + 1, ny + 1, nx + 1), np.float32)
+    pad[:nz, :ny, :nx] = rho
+    bz, by, bx = (nz + 3) // 4, (ny + 3) // 4, (nx + 3) // 4
+    out = np.full((bz, by, bx), np.nan, np.float32)
+    for k in range(bz):
+        for j in range(by):
+            for i in range(bx):
+                c = pad[4 * k: min(4 * k + 5, nz + 1), 4 * j: min(4 * j + 5, ny + 1), 4 * i: min(4 * i + 5, nx + 1)]
+                if np.all(c == c.flat[0]):
+                    out[k, j, i] = c.flat[0]
+    return out
+
+
+@pytest.mark.parametrize("gantry,couch", [(0.0, 0.0), (120.0, 30.0), (270.0, 0.0)])
+def test_block_table_and_uniform_skipping(dev, gantry, couch):
+    """Air box with a noisy tissue cube and a water slab: the block table equals
+    a numpy restatement, and the ray-march (which adds runs of samples in
+    uniform blocks at once) matches the oracle's sample-by-sample WET."""
+    from DoseCUDA import IMPTPlan
+    from DoseCUDA.workloads import _beam, _grid, grid_spots
+    _say("block table / skipping gantry=%g couch=%g" % (gantry, couch))
+    shape = (30, 36, 33)
+    g = _grid(shape, 2.5)
+    rng = np.random.default_rng(11)
+    hu = np.full(shape, -1000.0, np.float32)
+    hu[8:22, 10:26, 9:24] = rng.normal(40, 20, size=(14, 16, 15))
+    hu[:, 28:31, :] = 0.0
+    g.HU = hu
+    plan = IMPTPlan()
+    plan.addBeam(_beam(gantry, grid_spots([30], [0.0], [0.0]), couch=couch))
+    beam = plan.beam_list[0]
+    raw, wet, rlsp, model = _gpu_wet(g, plan, beam, dev)
+    # table of the padded volume
+    from DoseCUDA import device as D
+    grid = D.make_grid(rlsp.shape, float(np.float32(g.spacing[0])))
+    padded = D.rlsp_pad(torch.from_numpy(rlsp).to(dev), grid, dev).cpu().numpy()
+    nz, ny, nx = shape
+    table = padded[4 * nx * ny * (nz + 1):].reshape((nz + 3) // 4, (ny + 3) // 4, (nx + 3) // 4)
+    want = _block_table_np(rlsp)
+    assert np.array_equal(np.isnan(table), np.isnan(want))
+    assert np.array_equal(table[~np.isnan(want)], want[~np.isnan(want)])
+    uniform = float(np.mean(~np.isnan(want)))
+    geom = _oracle_geom(g, beam, model)
+    s = float(np.float32(g.spacing[0]))
+    raw_o = O.raymarch(rlsp, s, geom)
+    wet_o = O.smooth(raw_o, s, geom)
+    _say("uniform blocks %.2f, max |d raw| %.3e cm, max |d wet| %.3e cm"
+         % (uniform, np.abs(raw - raw_o).max(), np.abs(wet - wet_o).max()))
+    assert uniform > 0.3
+    assert np.abs(raw - raw_o).max() <= 2e-4 and np.abs(wet - wet_o).max() <= 2e-4
 
 
 def test_water_slab_end_to_end(dev, machine_dir):
