@@ -144,6 +144,220 @@ def load_traffic(kernel):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
+def _graph_ms(step, dev, warmup, steps):
+    import torch
+    from DoseCUDA.batch import GraphedStep
+    gs = GraphedStep(step, dev, warmup=max(1, warmup))
+    for _ in range(warmup):
+        gs.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gs.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, gs
+
+
+def _target_masks(g, dev, radius=25.0):
+    import numpy as np
+    import torch
+    from DoseCUDA.workloads import voxel_centres
+    z, y, x = voxel_centres(g)
+    r2 = z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2
+    t = torch.from_numpy(np.ascontiguousarray(r2 <= radius * radius)).to(dev)
+    return t
+
+
+def run_uneven(args):
+    """64 plans of uneven size (cube side U{64..256} @ 2 mm, water + 20 mm HU = 1000 slab at 60 mm, 1-4 beams
+    at random gantry, 500-40k spots log-uniform, seed 4): one batched graph step vs per-plan paths."""
+    import numpy as np
+    import torch
+    from DoseCUDA import device as D
+    from DoseCUDA.batch import BatchedDose, tune_dict
+    from DoseCUDA.engine import IMPTDoseEngine
+    from DoseCUDA.workloads import uneven_plans
+    n_plans = args.uneven_plans
+    log("building %d uneven plans" % n_plans)
+    plans = uneven_plans(n_plans, seed=4)
+    engines = [IMPTDoseEngine(g, p, gpu_id=0, cutoff_sigma=args.cutoff_sigma) for g, p in plans]
+    dev = engines[0].device
+    masks = [_target_masks(g, dev) for g, _ in plans]
+    counts = [(int(m.sum().item()), int(np.prod(e.shape))) for m, e in zip(masks, engines)]
+
+    def grad_into(d, out, m, nt, nb):
+        g = torch.where(m, (d - 2.0) * (2.0 / max(nt, 1)), torch.zeros_like(d))
+        out.copy_(g + d * (0.2 / nb))
+
+    def make_step(bd, idx):
+        def step():
+            doses = bd.forward()
+            for d, gd, k in zip(doses, bd.grad_doses, idx):
+                grad_into(d, gd, masks[k], counts[k][0], counts[k][1])
+            return doses, bd.adjoint(None, reuse_prep=True, input_order=False)
+        return step
+
+    t0 = time.perf_counter()
+    bd = BatchedDose(engines, wet_mode="reach", tune=None if args.no_tune else "auto")
+    t_setup = time.perf_counter() - t0
+    log("batch items %s, schedule %s, set-up %.1f s" % (bd.item_counts(), tune_dict(bd.tune), t_setup))
+    ms_batch, gs = _graph_ms(make_step(bd, list(range(n_plans))), dev, args.warmup, args.steps)
+    del gs, bd
+    torch.cuda.empty_cache()
+    log("batched step %.2f ms for %d plans" % (ms_batch, n_plans))
+    # each plan alone, same batched path (its own graph, default schedule)
+    single = []
+    for k, e in enumerate(engines):
+        b1 = BatchedDose([e], wet_mode="reach")
+        ms, gs = _graph_ms(make_step(b1, [k]), dev, 1, 3)
+        single.append(ms)
+        del gs, b1
+    # plain path per plan: eager per-beam calls, full WET, dense forward and adjoint
+    plain = []
+    for k, e in enumerate(engines):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for b in e.beams:
+            b.wet_raw, b.wet = D.wet_from_padded(e.rlsp_padded, e.grid, b.geo, dev, raw_out=b.wet_raw, out=b.wet)
+        dose = torch.empty(e.shape, dtype=torch.float32, device=dev)
+        if not e.beams:
+            dose.zero_()
+        for j, b in enumerate(e.beams):
+            D.pb_forward(b.wet, dose, e.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0,
+                         accumulate=j > 0)
+        gd = torch.empty_like(dose)
+        grad_into(dose, gd, masks[k], counts[k][0], counts[k][1])
+        for b in e.beams:
+            D.pb_adjoint(b.wet, gd, e.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0)
+        torch.cuda.synchronize()
+        plain.append((time.perf_counter() - t0) * 1e3)
+        for b in e.beams:
+            b._ws = None   # free the single-beam workspace again
+        torch.cuda.empty_cache()
+    single, plain = np.array(single), np.array(plain)
+    sides = [int(g.HU.shape[0]) for g, _ in plans]
+    spots = [sum(b.spot_list.shape[0] for b in p.beam_list) for _, p in plans]
+    rec = {
+        "metric": "batched fwd+adjoint of uneven plans: ms per plan (batch) and per-plan latency",
+        "workload": "%d plans, cube side U{64..256} @ 2 mm (sides %d..%d), water + 20 mm HU=1000 slab at 60 mm, "
+                    "1-4 beams at random gantry, %d..%d spots per plan (total %d), seed 4; step = reach-limited "
+                    "WET, forward, dL/dD (25 mm target, 2 Gy), adjoint" % (n_plans, min(sides), max(sides),
+                                                                          min(spots), max(spots), sum(spots)),
+        "batch_ms_per_step": ms_batch,
+        "batch_ms_per_plan": ms_batch / n_plans,
+        "alone_batched_ms": {"p50": float(np.percentile(single, 50)), "p99": float(np.percentile(single, 99)),
+                             "sum": float(single.sum())},
+        "plain_ms": {"p50": float(np.percentile(plain, 50)), "p99": float(np.percentile(plain, 99)),
+                     "sum": float(plain.sum())},
+        "speedup_batch_vs_plain_per_plan": float(plain.sum() / ms_batch),
+        "speedup_batch_vs_alone": float(single.sum() / ms_batch),
+        "setup_and_autotune_s": t_setup,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def run_body(args):
+    """Large body-size optimisation loop: 300x512x512 @ 1 mm, 4 beams, ~63k spots; WET once, then
+    `--iters` projected-gradient weight steps w <- max(0, w - eta dL/dw), each one batched graph replay
+    (forward, dL/dD, adjoint, update); final dose vs the float64 oracle on a voxel sample."""
+    import numpy as np
+    import torch
+    from DoseCUDA.batch import BatchedDose, GraphedStep, tune_dict
+    from DoseCUDA.engine import IMPTDoseEngine
+    from DoseCUDA.workloads import body_plan
+    log("building the body phantom")
+    g, plan, target = body_plan()
+    t_wall0 = time.perf_counter()
+    eng = IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=args.cutoff_sigma)
+    dev = eng.device
+    tmask = torch.from_numpy(target).to(dev)
+    body = torch.from_numpy(g.HU > -500.0).to(dev)
+    n_t, n_b = float(tmask.sum().item()), float(body.sum().item())
+    bd = BatchedDose([eng], wet_mode="full", tune=None if args.no_tune else "auto")
+    bd.compute_wet()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_wall0
+    log("set-up (HU->RLSP, WET, batch, autotune) %.1f s, schedule %s" % (t_setup, tune_dict(bd.tune)))
+    mus = [b.mu0.clone() for b in eng.beams]
+    lr = torch.zeros(1, dtype=torch.float32, device=dev)
+    loss_t = torch.zeros(1, dtype=torch.float32, device=dev)
+
+    def step():
+        doses = bd.forward(mus, None)
+        d = doses[0]
+        diff = torch.where(tmask, d - 2.0, torch.zeros_like(d))
+        gd = bd.grad_doses[0]
+        gd.copy_(diff * (2.0 / n_t) + torch.where(body, d * (0.2 / n_b), torch.zeros_like(d)))
+        grads = bd.adjoint(None, reuse_prep=True)
+        for m, (gm, _) in zip(mus, grads):
+            m.sub_(lr * gm).clamp_(min=0.0)
+        return doses
+
+    # step size from the first gradient: eta = 0.05 max(w) / max |dL/dw|
+    d0 = bd.forward(mus, None)[0]
+    loss0 = float(((d0[tmask] - 2.0) ** 2).mean() + 0.1 * (d0[body] ** 2).mean())
+    gd = bd.grad_doses[0]
+    gd.copy_(torch.where(tmask, d0 - 2.0, torch.zeros_like(d0)) * (2.0 / n_t)
+             + torch.where(body, d0 * (0.2 / n_b), torch.zeros_like(d0)))
+    g0 = bd.adjoint(None)
+    gmax = max(float(gm.abs().max()) for gm, _ in g0)
+    lr.fill_(0.05 * max(float(m.max()) for m in mus) / gmax)
+    gs = GraphedStep(lambda: step(), dev, warmup=1)   # warm-up and capture run 2 iterations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(args.iters):
+        gs.replay()
+        if (it + 1) % 10 == 0:
+            torch.cuda.synchronize()
+            log("iteration %d" % (it + 1))
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    d = bd.forward(mus, None)[0]
+    loss1 = float(((d[tmask] - 2.0) ** 2).mean() + 0.1 * (d[body] ** 2).mean())
+    eng.compute_wet(force=True)   # single-beam path: raw and smoothed WET of every voxel (same kernels)
+    # accuracy: float64 oracle at sampled voxels (half in the 10%-of-max region) on the engine's WET
+    from oracle import impt_oracle as O
+    dn = d.detach().cpu().numpy()
+    rng = np.random.default_rng(0)
+    hi = np.flatnonzero(dn.reshape(-1) > 0.1 * dn.max())
+    idx = np.unique(np.concatenate([rng.choice(hi, size=min(2000, hi.size), replace=False),
+                                    rng.choice(dn.size, size=2000, replace=False)])).astype(np.int64)
+    s = eng.spacing
+    ref = np.zeros(idx.size)
+    wet_err = 0.0
+    rlsp = O.rlsp_from_hu(g.HU, os.path.join(PKG, "DoseCUDA", "lookuptables", "protons", plan.machine_name))
+    for b, m in zip(eng.beams, mus):
+        model = b.model
+        geom = O.beam_geometry(np.asarray(b.beam.iso, np.float32), eng.origin, b.beam.gantry_angle,
+                               b.beam.couch_angle, model.VSADX, model.VSADY)
+        sl = np.array(b.beam.spot_list, dtype=np.float32)
+        sl[:, 2] = m.detach().cpu().numpy()
+        _, spots, layers, r80 = O.spots_and_layers(sl, model.divergence_params)
+        wet_gpu = b.wet.detach().cpu().numpy().astype(np.float64)
+        ref += O.pb_dose(wet_gpu, s, geom, spots, layers, r80, O.ModelArrays(model), idx=idx)
+        raw_o = O.raymarch(rlsp, s, geom, idx=idx)
+        raw_g = b.wet_raw.detach().cpu().numpy().reshape(-1)[idx] if b.wet_raw is not None else None
+        if raw_g is not None:
+            wet_err = max(wet_err, float(np.abs(raw_g - raw_o).max()))
+    got = dn.reshape(-1)[idx]
+    rec = {
+        "metric": "large body optimisation loop: wall time and final dose error vs float64 oracle",
+        "workload": "300x512x512 @ 1 mm body phantom (seed 5), 4 beams (gantry 0/90/180/270), %d spots, "
+                    "%d projected-gradient weight iterations, each one HIP-graph replay of forward + dL/dD + "
+                    "adjoint + update; WET computed once" % (sum(b.spots.n_spots for b in eng.beams), args.iters),
+        "setup_s": t_setup,
+        "loop_s": t_loop,
+        "ms_per_iteration": t_loop / args.iters * 1e3,
+        "loss_first": loss0,
+        "loss_last": loss1,
+        "final_dose_vs_oracle_max_rel_to_dmax": float(np.abs(got - ref).max() / max(dn.max(), 1e-30)),
+        "raw_wet_vs_oracle_max_cm": wet_err,
+        "oracle_sample_voxels": int(idx.size),
+        "schedule": tune_dict(bd.tune),
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +368,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain (dense, per-beam) path; 0 = skip")
     ap.add_argument("--cpu-sample", type=int, default=60000, help="voxels in the CPU-baseline sample; 0 = skip")
     ap.add_argument("--no-tune", action="store_true", help="library default schedules instead of the autotuner's")
+    ap.add_argument("--workload", choices=("head", "uneven", "body"), default="head",
+                    help="head: the headline metric (default); uneven: 64 uneven plans batched; body: 512x512x300 "
+                         "optimisation loop (each prints its own JSON line)")
+    ap.add_argument("--iters", type=int, default=50, help="optimisation iterations of the body workload")
+    ap.add_argument("--uneven-plans", type=int, default=64, help="plans of the uneven workload")
     ap.add_argument("--eager-only", action="store_true",
                     help="time eager batched steps only (no graph), e.g. under a profiler")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -183,6 +402,10 @@ def main():
     from DoseCUDA.engine import IMPTDoseEngine
 
     assert torch.cuda.is_available(), "bench.py needs a HIP GPU"
+    if args.workload == "uneven":
+        return run_uneven(args)
+    if args.workload == "body":
+        return run_body(args)
     K = args.plans
     log("building workload: %d head-phantom plans" % K)
     g, plans = plan_copies(K)

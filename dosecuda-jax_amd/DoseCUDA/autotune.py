@@ -161,9 +161,10 @@ def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=3, grad_doses=None, 
         for g, gd in zip(b2.grad_doses, grad_doses):
             g.copy_(gd)
 
+        # a fresh batch: the first (warm-up) call prepares the spot records, the timed ones reuse them
         def run():
-            N.check(b2.lib.dh_batch_adjoint(ctypes.byref(b2.batch), N.DH_BATCH_REUSE_PREP, b2._stream()),
-                    "dh_batch_adjoint")
+            flags = N.DH_BATCH_REUSE_PREP if b2.batch.prepared else 0
+            N.check(b2.lib.dh_batch_adjoint(ctypes.byref(b2.batch), flags, b2._stream()), "dh_batch_adjoint")
         return _phase_ms(run, dev, reps)
 
     for field, cands in (("adj_cs", ADJ_CS), ("adj_wg_target", ADJ_WG_TARGET), ("adj_max_slabs", ADJ_MAX_SLABS),

@@ -16,6 +16,10 @@ as createCubePhantom, plan.py:226-232) and spots in ``spot_list`` rows
                      isocentre, 25 layers x 400 spots (10000 spots), plus the
                      target / shell masks of its loss
 * uneven_plans       plans of random cube size, beam count and spot count
+* body_plan          300x512x512 @ 1 mm body (elliptic cylinder N(20, 30) HU,
+                     two lungs U(-850, -700), spine U(300, 1100)), gantries
+                     0/90/180/270, 50 layers (RS0 ids 25..74) x spots on a
+                     3 mm grid in a 30 mm disk (~63k spots), 30 mm target
 """
 
 import numpy as np
@@ -166,3 +170,35 @@ def uneven_plans(n_plans=64, seed=4, sides=(64, 256), spacing=2.0, spots=(500, 4
             plan.addBeam(_beam(ga, sl))
         out.append((g, plan))
     return out
+
+
+def body_plan(shape=(300, 512, 512), spacing=1.0, seed=5, gantries=(0.0, 90.0, 180.0, 270.0),
+              layers=range(25, 75), spot_radius=30.0, pitch=3.0):
+    """Large body phantom (seed 5): elliptic cylinder along z (semi-axes 180 mm
+    in x, 120 mm in y) of soft tissue N(20, 30) HU, two lungs (ellipsoids at
+    x = +-80 mm, semi-axes 55 x 65 x 120 mm) U(-850, -700), a spine (cylinder of
+    radius 15 mm at y = +85 mm) U(300, 1100), air outside.  Returns (dose_grid,
+    plan, target_mask) with a 30 mm spherical target at the isocentre."""
+    g = _grid(shape, spacing)
+    rng = np.random.default_rng(seed)
+    z, y, x = voxel_centres(g)
+    Z = z[:, None, None]
+    Y = y[None, :, None]
+    X = x[None, None, :]
+    hu = np.full(shape, -1000.0, dtype=np.float32)
+    body = np.broadcast_to((X / 180.0) ** 2 + (Y / 120.0) ** 2 <= 1.0, shape)
+    hu[body] = rng.normal(20.0, 30.0, size=int(body.sum())).astype(np.float32)
+    for cx in (-80.0, 80.0):
+        lung = np.broadcast_to(((X - cx) / 55.0) ** 2 + ((Y + 5.0) / 65.0) ** 2 + (Z / 120.0) ** 2 <= 1.0, shape)
+        hu[lung] = rng.uniform(-850.0, -700.0, size=int(lung.sum())).astype(np.float32)
+    spine = np.broadcast_to(X ** 2 + (Y - 85.0) ** 2 <= 15.0 ** 2, shape)
+    hu[spine] = rng.uniform(300.0, 1100.0, size=int(spine.sum())).astype(np.float32)
+    g.HU = hu
+    plan = IMPTPlan()
+    c = np.arange(-spot_radius, spot_radius + 1e-6, pitch)
+    for ga in gantries:
+        sl = grid_spots(list(layers), c, c, disk_radius=spot_radius)
+        sl[:, 2] = rng.uniform(0.0, 1.0, size=sl.shape[0]).astype(np.float32)
+        plan.addBeam(_beam(ga, sl))
+    target = np.broadcast_to(X ** 2 + Y ** 2 + Z ** 2 <= 30.0 ** 2, shape)
+    return g, plan, np.ascontiguousarray(target)

@@ -327,12 +327,26 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     return DH_OK;
 }
 
-static int prep(const dh_batch *b, hipStream_t st) {
-    return launch_prep_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_SPOT_OFF),
-                             table<long long>(b, T_LAYER_OFF), b->n_beams, b->n_spots, b->n_layers, st);
+// Spot and layer records: prepared now, or (flags & DH_BATCH_REUSE_PREP) those
+// an earlier phase call on this batch prepared.  Reusing records no call has
+// prepared would hand the kernels uninitialised workspace (spot ranges used as
+// indices), so it is an error.
+static int prepare_records(dh_batch *b, int32_t flags, hipStream_t st, const char *who) {
+    if (flags & DH_BATCH_REUSE_PREP) {
+        if (!b->prepared) {
+            set_error("%s: DH_BATCH_REUSE_PREP before any call prepared the spot records of this batch", who);
+            return DH_ERR_ARG;
+        }
+        return DH_OK;
+    }
+    if (b->n_beams == 0) return DH_OK;
+    const int rc = launch_prep_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_SPOT_OFF),
+                                     table<long long>(b, T_LAYER_OFF), b->n_beams, b->n_spots, b->n_layers, st);
+    if (!rc) b->prepared = 1;
+    return rc;
 }
 
-extern "C" int dh_batch_wet(const dh_batch *b, int32_t flags, void *stream) {
+extern "C" int dh_batch_wet(dh_batch *b, int32_t flags, void *stream) {
     if (!batch_ok(b)) return DH_ERR_ARG;
     if (!(b->tab[T_FLAGS] & F_RLSP)) {
         set_error("dh_batch_wet: a plan of the batch has no quad RLSP volume");
@@ -342,7 +356,7 @@ extern "C" int dh_batch_wet(const dh_batch *b, int32_t flags, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     const bool reach = (flags & DH_BATCH_WET_REACH) != 0;
     if (reach) {
-        int rc = (flags & DH_BATCH_REUSE_PREP) ? DH_OK : prep(b, st);
+        int rc = prepare_records(b, flags, st, "dh_batch_wet");
         if (!rc) rc = launch_wet_need_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF), b->n_beams,
                                             b->n_bricks, st, b->events);
         if (rc) return rc;
@@ -352,17 +366,17 @@ extern "C" int dh_batch_wet(const dh_batch *b, int32_t flags, void *stream) {
                             b->ray_blocks, st, b->events);
 }
 
-extern "C" int dh_batch_forward(const dh_batch *b, int32_t flags, void *stream) {
+extern "C" int dh_batch_forward(dh_batch *b, int32_t flags, void *stream) {
     if (!batch_ok(b)) return DH_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
-    int rc = (b->n_beams && !(flags & DH_BATCH_REUSE_PREP)) ? prep(b, st) : DH_OK;
+    const int rc = prepare_records(b, flags, st, "dh_batch_forward");
     if (rc) return rc;
     return launch_forward_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF), b->n_beams,
                                 b->n_bricks, table<PlanCtx>(b, T_PLANS), table<long long>(b, T_SUM_OFF), b->n_plans,
                                 b->n_sum_blocks, b->cutoff_sigma > 0.0f, b->fwd_blocks, st, b->events);
 }
 
-extern "C" int dh_batch_adjoint(const dh_batch *b, int32_t flags, void *stream) {
+extern "C" int dh_batch_adjoint(dh_batch *b, int32_t flags, void *stream) {
     if (!batch_ok(b)) return DH_ERR_ARG;
     if (b->n_spots == 0) return DH_OK;
     if (!(b->tab[T_FLAGS] & F_GRAD_DOSE) || !(b->tab[T_FLAGS] & F_GRAD_OUT)) {
@@ -370,7 +384,7 @@ extern "C" int dh_batch_adjoint(const dh_batch *b, int32_t flags, void *stream) 
         return DH_ERR_ARG;
     }
     hipStream_t st = (hipStream_t)stream;
-    int rc = (flags & DH_BATCH_REUSE_PREP) ? DH_OK : prep(b, st);
+    const int rc = prepare_records(b, flags, st, "dh_batch_adjoint");
     if (rc) return rc;
     return launch_adjoint_batch(table<BeamCtx>(b, T_CTX), b->n_beams, table<long long>(b, T_BRICK_OFF), b->n_bricks,
                                 table<long long>(b, T_PART_OFF), b->n_partials, table<long long>(b, T_ADJ32_OFF),

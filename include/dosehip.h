@@ -262,6 +262,8 @@ typedef struct dh_batch {
     int64_t tab[16];           /* byte offsets of the device tables in ws; tab[13] = end of the tables
                                   uploaded at setup (the rest of ws is scratch the phases rewrite) */
     int32_t fwd_blocks, ray_blocks;   /* brick-kernel schedules from dh_batch_tune */
+    int32_t prepared;          /* spot / layer records prepared by a phase call (DH_BATCH_REUSE_PREP needs it) */
+    int32_t reserved;
     void *const *events;       /* kernel timing events (dh_batch_timing), NULL = off */
 } dh_batch;
 
@@ -280,7 +282,10 @@ int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32_t n_plans,
 /* Flags of the phase calls.
  * DH_BATCH_REUSE_PREP  reuse the spot and layer records the previous phase call
  *                      prepared (spot arrays unchanged since) instead of
- *                      preparing them again.
+ *                      preparing them again.  An error (DH_ERR_ARG) until some
+ *                      call on this batch has prepared them: dh_batch_forward,
+ *                      dh_batch_adjoint, or dh_batch_wet with
+ *                      DH_BATCH_WET_REACH, each without this flag.
  * DH_BATCH_WET_REACH   (dh_batch_wet, cutoff_sigma > 0) compute WET only in
  *                      bricks that some layer of the beam can reach with the
  *                      CURRENT spot positions, and where smoothing reads raw
@@ -294,7 +299,7 @@ int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32_t n_plans,
 #define DH_BATCH_WET_REACH  2
 
 /* WET of every beam: ray-march then smoothing (dh_wet_raymarch + dh_wet_smooth per beam). */
-int dh_batch_wet(const dh_batch *batch, int32_t flags, void *stream);
+int dh_batch_wet(dh_batch *batch, int32_t flags, void *stream);
 
 /* Kernel timing for eager calls (not for graph capture).  events is a HOST
  * array of 2*DH_TIME_N hipEvent_t (created by the caller; NULL entries are
@@ -310,10 +315,11 @@ int dh_batch_wet(const dh_batch *batch, int32_t flags, void *stream);
 int dh_batch_timing(dh_batch *batch, void *const *events);
 
 /* Dose of every plan (dh_pb_forward per beam, beam doses added in beam order). */
-int dh_batch_forward(const dh_batch *batch, int32_t flags, void *stream);
+int dh_batch_forward(dh_batch *batch, int32_t flags, void *stream);
 
-/* Spot gradients of every beam from its plan's grad_dose (dh_pb_adjoint per beam). */
-int dh_batch_adjoint(const dh_batch *batch, int32_t flags, void *stream);
+/* Spot gradients of every beam from its plan's grad_dose (dh_pb_adjoint per beam).
+ * The three phase calls update batch->prepared (host side), hence the non-const batch. */
+int dh_batch_adjoint(dh_batch *batch, int32_t flags, void *stream);
 
 #ifdef __cplusplus
 }

@@ -308,3 +308,21 @@ def test_autotuner_picks_and_caches(dev, tmp_path, monkeypatch):
         if y0.numel():
             assert float((x0 - y0).abs().max()) <= 1e-5 * float(y0.abs().max())
             assert float((x1 - y1).abs().max()) <= 1e-5 * float(y1.abs().max())
+
+
+def test_reuse_prep_requires_prepared_records(dev):
+    """DH_BATCH_REUSE_PREP on a batch whose spot records no call has prepared is
+    an argument error (raised before any kernel runs); after a forward it works."""
+    from DoseCUDA.batch import BatchedDose
+    _say("reuse-prep guard")
+    engines = _engines(dev)[:2]
+    bd = BatchedDose(engines, wet_mode="full")
+    bd.compute_wet()                       # full WET prepares no spot records
+    assert bd.batch.prepared == 0
+    with pytest.raises(ValueError, match="REUSE_PREP"):
+        bd.adjoint(_grads(engines), reuse_prep=True)
+    bd.forward()
+    assert bd.batch.prepared == 1
+    a = bd.adjoint(_grads(engines), reuse_prep=True)
+    torch.cuda.synchronize()
+    assert all(torch.isfinite(g).all() for g, _ in a)

@@ -173,6 +173,7 @@ def test_batch_sizes_and_errors_without_gpu():
     assert lib.dh_batch_forward(ctypes.byref(b), 0, None) == N.DH_ERR_ARG
     assert "not set up" in N.last_error()
     assert lib.dh_batch_timing(ctypes.byref(b), None) == N.DH_ERR_ARG
+    assert b.prepared == 0
 
 
 def test_geometry_matches_oracle_restatement():
