@@ -5,21 +5,26 @@
 //  k_wet_raymarch  per-voxel 1 mm march toward the source, manual trilinear
 //  k_wet_smooth    6-direction lateral mean of the raw WET
 //
-// Quad layout: element (i, j, k), i < nx, j < ny, k <= nz, is the float4
-// (rho(i,j,k), rho(i+1,j,k), rho(i,j+1,k), rho(i+1,j+1,k)) with rho = 0 past
-// the high end of every axis (i = nx, j = ny, k = nz).  A trilinear sample
-// with continuous index c in [0, n) then needs two 16-byte loads (k0, k0 + 1)
-// and no per-corner bounds tests.  On gfx950 the ray-march is bound by the
-// texture-address rate of its gathers (one address per lane per load), so two
-// 16-byte loads per sample instead of four 8-byte ones halve that work.
+// Quad layout: element (i, j, k), i < nx, j < ny, k <= nz, holds the bilinear
+// form of the cell face at slice k, with a = rho(i,j,k), b = rho(i+1,j,k),
+// c = rho(i,j+1,k), d = rho(i+1,j+1,k) and rho = 0 past the high end of every
+// axis (i = nx, j = ny, k = nz):
+//     (a, b - a, c - a, (d - c) - (b - a))
+// so that rho(i + x, j + y, k) = a + x (b - a) + y ((c - a) + x (...)) is three
+// fmas.  A trilinear sample with continuous index c in [0, n) needs two
+// 16-byte loads (slices k0 and k0 + 1) and no per-corner bounds tests.  The
+// ray-march is bound by the address and data rate of its gathers (one address
+// per lane per load) together with its VALU work, so two 16-byte loads per
+// sample instead of four 8-byte ones halve the former and the bilinear form
+// trims the latter (8 instead of 14 operations per sample).
 //
 // Block table (after the quads): one float per 4x4x4 block of trilinear cells
 // (cells with floor(c) in [4b, 4b + 3] per axis): the common value when every
 // corner those cells touch (voxels 4b .. 4b + 4, zero past the high end) holds
 // the same rho, NaN otherwise.  Every sample whose cell lies in a uniform block
 // is exactly that value, so the march adds a run of such samples at once
-// (empty-space skipping; air around the patient is one value because the HU
-// table clamps at -1000 HU).
+// (empty-space skipping; air around the patient is one value because the HU table clamps
+// at -1000 HU).
 #include "dh_common.h"
 
 namespace dh {
@@ -31,8 +36,9 @@ __host__ __device__ inline long long block_count(int nx, int ny, int nz) {
 }
 
 // One thread per block of cells: compare every corner the block's cells touch
-// (quads i in [4bx, min(4bx + 3, nx - 1)], same for j, k in [4bz, min(4bz + 4,
-// nz)], all four lanes of each) with the first one.
+// (voxels 4b .. min(4b + 4, n) per axis; rho(i, j, k) is the first component
+// of quad (i, j, k), zero at i = nx or j = ny, and quad (i, j, nz) is zero)
+// with the first one.
 __global__ __launch_bounds__(256) void k_rlsp_blocks(const float4 *__restrict__ q, float *__restrict__ table, Grid g,
                                                     long long nblk) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -41,16 +47,16 @@ __global__ __launch_bounds__(256) void k_rlsp_blocks(const float4 *__restrict__ 
     const int bx = (int)(t % nbx);
     const long long r = t / nbx;
     const int by = (int)(r % nby), bz = (int)(r / nby);
-    const int i0 = 4 * bx, i1 = min(4 * bx + 3, g.nx - 1);
-    const int j0 = 4 * by, j1 = min(4 * by + 3, g.ny - 1);
+    const int i0 = 4 * bx, i1 = min(4 * bx + 4, g.nx);
+    const int j0 = 4 * by, j1 = min(4 * by + 4, g.ny);
     const int k0 = 4 * bz, k1 = min(4 * bz + 4, g.nz);
     const float v = q[i0 + (long long)g.nx * (j0 + (long long)g.ny * k0)].x;
     bool same = v == v;
     for (int k = k0; k <= k1 && same; ++k)
         for (int j = j0; j <= j1 && same; ++j)
             for (int i = i0; i <= i1 && same; ++i) {
-                const float4 e = q[i + (long long)g.nx * (j + (long long)g.ny * k)];
-                same = e.x == v && e.y == v && e.z == v && e.w == v;
+                const float e = (i < g.nx && j < g.ny) ? q[i + (long long)g.nx * (j + (long long)g.ny * k)].x : 0.0f;
+                same = e == v;
             }
     table[t] = same ? v : __int_as_float(0x7fc00000);
 }
@@ -87,12 +93,11 @@ template <typename F>
 __device__ __forceinline__ float4 make_quad(const Grid &g, int i, int j, int k, F rho) {
     // rho(i, j, k) for in-range indices, 0 past the high end
     const bool xi = i + 1 < g.nx, yj = j + 1 < g.ny, zk = k < g.nz;
-    float4 q;
-    q.x = zk ? rho(i, j, k) : 0.0f;
-    q.y = (zk && xi) ? rho(i + 1, j, k) : 0.0f;
-    q.z = (zk && yj) ? rho(i, j + 1, k) : 0.0f;
-    q.w = (zk && xi && yj) ? rho(i + 1, j + 1, k) : 0.0f;
-    return q;
+    const float a = zk ? rho(i, j, k) : 0.0f;
+    const float b = (zk && xi) ? rho(i + 1, j, k) : 0.0f;
+    const float c = (zk && yj) ? rho(i, j + 1, k) : 0.0f;
+    const float d = (zk && xi && yj) ? rho(i + 1, j + 1, k) : 0.0f;
+    return make_float4(a, b - a, c - a, (d - c) - (b - a));
 }
 
 __global__ __launch_bounds__(256) void k_rlsp_from_hu(const float *__restrict__ hu, float4 *__restrict__ out,
@@ -129,14 +134,11 @@ __global__ __launch_bounds__(256) void k_rlsp_pad(const float *__restrict__ in, 
 // margin far above float32 rounding and are read without tests, the tail is
 // tested sample by sample exactly as the per-step mask does.
 __device__ __forceinline__ float blend_quads(float4 q0, float4 q1, float ax, float ay, float az) {
-    const float x00 = fmaf(ax, q0.y - q0.x, q0.x);
-    const float x10 = fmaf(ax, q0.w - q0.z, q0.z);
-    const float x01 = fmaf(ax, q1.y - q1.x, q1.x);
-    const float x11 = fmaf(ax, q1.w - q1.z, q1.z);
-    const float y0 = fmaf(ay, x10 - x00, x00);
-    const float y1 = fmaf(ay, x11 - x01, x01);
+    const float y0 = fmaf(ay, fmaf(ax, q0.w, q0.z), fmaf(ax, q0.y, q0.x));
+    const float y1 = fmaf(ay, fmaf(ax, q1.w, q1.z), fmaf(ax, q1.y, q1.x));
     return fmaf(az, y1 - y0, y0);
 }
+
 
 // Tail samples (plain loads; c inside the grid).
 __device__ __forceinline__ float trilinear_quad(const float4 *__restrict__ qv, int nx, int nxy, float cx, float cy,
@@ -172,6 +174,38 @@ __device__ __forceinline__ float trilinear_buf(__amdgpu_buffer_rsrc_t rs, float 
     return blend_quads(q0, q1, ax, ay, az);
 }
 
+// Samples from sample n on that stay in the 4x4x4 cell block of position c,
+// minus one step of margin against the rounding of the positions (negative or
+// zero: no run).  Per axis the ray meets the block's far face (the one it moves
+// toward) at (face - c) / du steps; lim[a] = face offset 4 or 0, rds = 1 / du
+// (0 when du = 0), add = 0 (3e38 when du = 0).
+struct BlockExit {
+    int offx, offy, offz;
+    float rdx, rdy, rdz, addx, addy, addz;
+};
+
+__device__ __forceinline__ BlockExit block_exit_setup(float dux, float duy, float duz) {
+    BlockExit e;
+    e.offx = dux > 0.0f ? 4 : 0;
+    e.offy = duy > 0.0f ? 4 : 0;
+    e.offz = duz > 0.0f ? 4 : 0;
+    e.rdx = dux != 0.0f ? 1.0f / dux : 0.0f;
+    e.rdy = duy != 0.0f ? 1.0f / duy : 0.0f;
+    e.rdz = duz != 0.0f ? 1.0f / duz : 0.0f;
+    e.addx = dux != 0.0f ? 0.0f : 3.0e38f;
+    e.addy = duy != 0.0f ? 0.0f : 3.0e38f;
+    e.addz = duz != 0.0f ? 0.0f : 3.0e38f;
+    return e;
+}
+
+__device__ __forceinline__ int block_run(const BlockExit &e, float cx, float cy, float cz) {
+    // c >= 0 here: truncation is floor
+    const float tx = fmaf((float)((((int)cx >> 2) << 2) + e.offx) - cx, e.rdx, e.addx);
+    const float ty = fmaf((float)((((int)cy >> 2) << 2) + e.offy) - cy, e.rdy, e.addy);
+    const float tz = fmaf((float)((((int)cz >> 2) << 2) + e.offz) - cz, e.rdz, e.addz);
+    return (int)fminf(fminf(fminf(tx, ty), tz), 1.0e9f) - 1;
+}
+
 // Raw WET of voxel (i, j, k) of a beam; the caller guarantees (i, j, k) inside.
 __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, const Grid &g, const Geom &b, int i,
                                                 int j, int k) {
@@ -196,32 +230,25 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
     // block table after the quads: rho of uniform 4x4x4 cell blocks, NaN elsewhere
     const float *ub = reinterpret_cast<const float *>(qv + quad_count(g.nx, g.ny, g.nz));
     const int nbx = (g.nx + 3) >> 2, nby = (g.ny + 3) >> 2;
-    const float rdx = 1.0f / fabsf(dux), rdy = 1.0f / fabsf(duy), rdz = 1.0f / fabsf(duz);
+    const BlockExit ex = block_exit_setup(dux, duy, duz);
     float acc0 = -0.05f, acc1 = 0.0f;
     int n = 0;
+    // Per step: the table entry of sample n's block; in a uniform block the run
+    // of samples n .. n + T that stay in it adds T + 1 times its value at once,
+    // otherwise samples n and n + 1 are read (no quad loads for skipped runs).
     while (n < n_safe) {
         const float f0 = (float)n;
         const float cx = fmaf(dux, f0, c0x), cy = fmaf(duy, f0, c0y), cz = fmaf(duz, f0, c0z);
-        // inside a uniform block every sample is the block's value: count the
-        // samples n .. n + T that stay in it (one step of margin against the
-        // rounding of the positions) and add them at once
-        const int ix = (int)cx, iy = (int)cy, iz = (int)cz;  // c >= 0 here: truncation is floor
-        const float u = gld(ub, (ix >> 2) + (long long)nbx * ((iy >> 2) + (long long)nby * (iz >> 2)));
+        const int bx = (int)cx >> 2, by = (int)cy >> 2, bz = (int)cz >> 2;  // c >= 0: truncation is floor
+        const float u = gld(ub, bx + nbx * (by + nby * bz));
         if (u == u) {
-            const float tx = dux > 0.0f ? ((float)((ix | 3) + 1) - cx) * rdx
-                                        : (dux < 0.0f ? (cx - (float)(ix & ~3)) * rdx : 3.0e38f);
-            const float ty = duy > 0.0f ? ((float)((iy | 3) + 1) - cy) * rdy
-                                        : (duy < 0.0f ? (cy - (float)(iy & ~3)) * rdy : 3.0e38f);
-            const float tz = duz > 0.0f ? ((float)((iz | 3) + 1) - cz) * rdz
-                                        : (duz < 0.0f ? (cz - (float)(iz & ~3)) * rdz : 3.0e38f);
-            const int T = min((int)fminf(fminf(fminf(tx, ty), tz), 1.0e9f) - 1, n_safe - 1 - n);
+            const int T = min(block_run(ex, cx, cy, cz), n_safe - 1 - n);
             if (T >= 1) {
                 acc1 = fmaf(fmaxf(u, 0.0f) * (float)(T + 1), 0.1f, acc1);
                 n += T + 1;
                 continue;
             }
         }
-        // two samples: independent loads in flight, two accumulators
         const float v0 = trilinear_buf(rs, cx, cy, cz, g.nx, nxy, so_z);
         acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
         if (n + 1 < n_safe) {
@@ -248,27 +275,60 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
     return acc;
 }
 
-// One wave per 4x4x4 brick, lane = voxel.
-__device__ __forceinline__ void raymarch_brick(const BeamCtx &c, long long brick, int lane) {
-    const int bi = (int)(brick % c.bg.bx);
-    const long long bt = brick / c.bg.bx;
-    const int bj = (int)(bt % c.bg.by), bk = (int)(bt / c.bg.by);
-    const int i = bi * BRICK + (lane & 3), j = bj * BRICK + ((lane >> 2) & 3), k = bk * BRICK + (lane >> 4);
+// Work mapping.  A workgroup takes four consecutive bricks; when they lie in
+// one row of the beam's brick grid (bx0 .. bx0 + 3), wave w marches layer
+// z = 4 bk + w of all four: lane (x, y) = (lane & 15, lane >> 4), a 16 x 4
+// voxel patch.  The rays of a patch run side by side, so each of its gathers
+// touches four 256-byte runs of quads (one per y) instead of the sixteen
+// 64-byte runs of a 4x4x4 brick: the march is bound by the cache tag lookups
+// of those gathers (one 64-byte segment per clock per CU), and this cuts them
+// by about a third.  Other quadruples (a row end, a beam end) fall back to one
+// wave per brick, lane = voxel of the 4x4x4 brick.  Every voxel's value is the
+// same either way.
+__device__ __forceinline__ void raymarch_voxel_at(const BeamCtx &c, int i, int j, int k) {
     if (i >= c.g.nx || j >= c.g.ny || k >= c.g.nz) return;
     gst(c.wet_raw, i + (long long)c.g.nx * (j + (long long)c.g.ny * k), raymarch_voxel(c.rlsp_q, c.g, c.b, i, j, k));
 }
 
+__device__ __forceinline__ void raymarch_brick(const BeamCtx &c, long long brick, int lane) {
+    const int bi = (int)(brick % c.bg.bx);
+    const long long bt = brick / c.bg.bx;
+    const int bj = (int)(bt % c.bg.by), bk = (int)(bt / c.bg.by);
+    raymarch_voxel_at(c, bi * BRICK + (lane & 3), bj * BRICK + ((lane >> 2) & 3), bk * BRICK + (lane >> 4));
+}
+
+// Bricks b0 .. b0 + 3 of one row: wave w, lane -> voxel of the 16 x 4 patch at
+// layer w.  Lanes of bricks the reach pass does not need stay idle.
+__device__ __forceinline__ void raymarch_row(const BeamCtx &c, long long b0, int w, int lane, int use_need) {
+    const int bi = (int)(b0 % c.bg.bx);
+    const long long bt = b0 / c.bg.bx;
+    const int bj = (int)(bt % c.bg.by), bk = (int)(bt / c.bg.by);
+    const int x = lane & 15;
+    if (use_need && !(gld(c.need, b0 + (x >> 2)) & 2u)) return;
+    raymarch_voxel_at(c, bi * BRICK + x, bj * BRICK + (lane >> 4), bk * BRICK + w);
+}
+
+__device__ __forceinline__ bool in_one_row(const BrickGrid &bg, long long b0, long long end) {
+    return b0 + 3 < end && (int)(b0 % bg.bx) + 3 < bg.bx;
+}
+
+// Single beam: workgroup q takes bricks 4q .. 4q + 3.
 __global__ __launch_bounds__(256) void k_wet_raymarch(BeamCtx c, unsigned nblocks) {
     const unsigned blk = xcd_remap(blockIdx.x, nblocks);
-    const long long brick = (long long)blk * 4 + (threadIdx.x >> 6);
-    if (brick >= c.bg.count()) return;
-    raymarch_brick(c, brick, threadIdx.x & 63);
+    const long long b0 = (long long)blk * 4, nbr = c.bg.count();
+    const int w = wave_in_block(), lane = threadIdx.x & 63;
+    if (in_one_row(c.bg, b0, nbr)) {
+        raymarch_row(c, b0, w, lane, 0);
+    } else if (b0 + w < nbr) {
+        raymarch_brick(c, b0 + w, lane);
+    }
 }
 
 // Batched: bricks of every beam in one launch; beam b owns bricks
-// [off[b], off[b + 1]) of the item space.  Waves step over items by `stride`
-// (one brick per wave when stride >= total, a persistent grid when stride =
-// 4 * gridDim.x).
+// [off[b], off[b + 1]) of the item space.  A workgroup takes items
+// q .. q + 3 (q a multiple of 4); the row mapping needs all four in one beam
+// and one row.  Workgroups step over items by `stride` (4 * gridDim.x for the
+// persistent grid).
 __device__ __forceinline__ void raymarch_item(const BeamCtx *__restrict__ ctx, const long long *__restrict__ off,
                                               int n_beams, long long item, int use_need) {
     const int bm = find_owner(off, n_beams, item);
@@ -279,6 +339,20 @@ __device__ __forceinline__ void raymarch_item(const BeamCtx *__restrict__ ctx, c
     raymarch_brick(c, brick, threadIdx.x & 63);
 }
 
+__device__ __forceinline__ void raymarch_quad(const BeamCtx *__restrict__ ctx, const long long *__restrict__ off,
+                                              int n_beams, long long q, long long total, int use_need) {
+    const int w = wave_in_block();
+    const int bm = find_owner(off, n_beams, q);
+    // off holds the first item of each beam; the last beam ends at total
+    const long long end = bm + 1 < n_beams ? off[bm + 1] : total;
+    const BeamCtx &c = load_ctx(ctx, bm);
+    if (in_one_row(c.bg, q - off[bm], end - off[bm])) {
+        raymarch_row(c, q - off[bm], w, threadIdx.x & 63, use_need);
+    } else if (q + w < total) {
+        raymarch_item(ctx, off, n_beams, q + w, use_need);
+    }
+}
+
 // The persistent form (PERSIST) is its own instantiation so that the one-shot
 // form keeps the registers of the plain kernel.
 template <bool PERSIST>
@@ -286,11 +360,11 @@ __global__ __launch_bounds__(256) void k_wet_raymarch_batch(const BeamCtx *__res
                                                            const long long *__restrict__ off, int n_beams,
                                                            long long base, long long total, int use_need,
                                                            long long stride) {
-    long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
+    long long q = base + (long long)blockIdx.x * 4;
     if (PERSIST) {
-        for (; item < total; item += stride) raymarch_item(ctx, off, n_beams, item, use_need);
-    } else if (item < total) {
-        raymarch_item(ctx, off, n_beams, item, use_need);
+        for (; q < total; q += stride) raymarch_quad(ctx, off, n_beams, q, total, use_need);
+    } else if (q < total) {
+        raymarch_quad(ctx, off, n_beams, q, total, use_need);
     }
 }
 
@@ -330,28 +404,33 @@ __device__ __forceinline__ StencilEntry stencil_entry(const Stencil *st, int e) 
     return StencilEntry{gld(st->dx, e), gld(st->dy, e), gld(st->dz, e), gld(st->tie, e), (float)gld(st->dr, e)};
 }
 
+// Indices fit in 32 bits: the grid check keeps nx * ny * (nz + 1) below 2^28.
+// Every entry issues its load (a skipped one reads the centre voxel again and
+// is not added), so the 54 loads of a voxel need no branches and can be in
+// flight together; the sum runs over the kept entries in table order.
 template <class ST>
 __device__ __forceinline__ float smooth_voxel(const float *__restrict__ raw, const Grid &g, const ST &st, long long t) {
-    const int i = (int)(t % g.nx);
-    const long long r = t / g.nx;
-    const int j = (int)(r % g.ny), k = (int)(r / g.ny);
-    const float wc = gld(raw, t);
+    const int ti = (int)t;
+    const int i = ti % g.nx;
+    const int r = ti / g.nx;
+    const int j = r % g.ny, k = r / g.ny;
+    const float wc = gld(raw, ti);
     const float max_dr = fminf(wc * 10.0f, 10.0f);
     float sum = wc;
     int cnt = 1;
-    // entries are ordered direction-major, dr = 1..9 within a direction
-    for (int a = 0; a < 6; ++a) {
-        for (int d = 0; d < 9; ++d) {
-            const StencilEntry se = stencil_entry(st, a * 9 + d);
-            if (!(se.dr < max_dr)) break;
-            const int ni = stencil_axis(i, se.dx, se.tie & 1u);
-            const int nj = stencil_axis(j, se.dy, se.tie & 2u);
-            const int nk = stencil_axis(k, se.dz, se.tie & 4u);
-            if ((unsigned)ni < (unsigned)g.nx && (unsigned)nj < (unsigned)g.ny && (unsigned)nk < (unsigned)g.nz) {
-                sum += gld(raw, ni + (long long)g.nx * (nj + (long long)g.ny * nk));
-                ++cnt;
-            }
-        }
+    // entries are ordered direction-major, dr = 1..9 within a direction: an
+    // entry is kept while dr < max_dr, so a direction stops at its first miss
+#pragma unroll
+    for (int e = 0; e < N_STENCIL; ++e) {
+        const StencilEntry se = stencil_entry(st, e);
+        const int ni = stencil_axis(i, se.dx, se.tie & 1u);
+        const int nj = stencil_axis(j, se.dy, se.tie & 2u);
+        const int nk = stencil_axis(k, se.dz, se.tie & 4u);
+        const bool ok = se.dr < max_dr && (unsigned)ni < (unsigned)g.nx && (unsigned)nj < (unsigned)g.ny &&
+                        (unsigned)nk < (unsigned)g.nz;
+        const float v = gld(raw, ok ? ni + g.nx * (nj + g.ny * nk) : ti);
+        sum = ok ? sum + v : sum;
+        cnt += ok ? 1 : 0;
     }
     return sum / (float)cnt;
 }
@@ -494,7 +573,7 @@ extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, c
         }
     }
     for (int t = n_knots; t < MAX_KNOTS; ++t) kn.hu[t] = kn.rlsp[t] = 0.0;
-    const long long nquad = dh_padded_len(g) / 4;
+    const long long nquad = quad_count(g.nx, g.ny, g.nz);
     const unsigned nb = (unsigned)((nquad + 255) / 256);
     hipLaunchKernelGGL(k_rlsp_from_hu, dim3(nb), dim3(256), 0, (hipStream_t)stream, hu,
                        reinterpret_cast<float4 *>(rlsp_padded), make_grid(g), kn, nquad);
@@ -508,7 +587,7 @@ extern "C" int dh_rlsp_pad(const float *rlsp, float *rlsp_padded, dh_grid g, voi
         set_error("dh_rlsp_pad: null pointer");
         return DH_ERR_ARG;
     }
-    const long long nquad = dh_padded_len(g) / 4;
+    const long long nquad = quad_count(g.nx, g.ny, g.nz);
     const unsigned nb = (unsigned)((nquad + 255) / 256);
     hipLaunchKernelGGL(k_rlsp_pad, dim3(nb), dim3(256), 0, (hipStream_t)stream, rlsp,
                        reinterpret_cast<float4 *>(rlsp_padded), make_grid(g), nquad);

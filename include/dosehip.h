@@ -16,13 +16,15 @@
  *    (numpy axis 2 of a (z, y, x) array), as in CudaClasses.cuh:189-191.
  *  - The ray-march reads RLSP in a "quad" layout: element (i, j, k) with
  *    i < nx, j < ny, k <= nz, index i + nx*(j + ny*k), is the float4
- *    (rho(i,j,k), rho(i+1,j,k), rho(i,j+1,k), rho(i+1,j+1,k)), rho = 0 past
- *    the high end of each axis; 4*nx*ny*(nz+1) floats, 16-byte aligned,
- *    followed by a table of ceil(nx/4)*ceil(ny/4)*ceil(nz/4) floats, one per
- *    4x4x4 block of trilinear cells: the rho every corner of the block's cells
- *    holds when they are all equal, NaN otherwise (the ray-march adds runs of
- *    samples in uniform blocks at once).  dh_rlsp_from_hu / dh_rlsp_pad write
- *    both parts; dh_padded_len gives the total.
+ *    (a, b - a, c - a, (d - c) - (b - a)) of a = rho(i,j,k), b = rho(i+1,j,k),
+ *    c = rho(i,j+1,k), d = rho(i+1,j+1,k), rho = 0 past the high end of each
+ *    axis (the bilinear form of the slice-k cell face); 4*nx*ny*(nz+1)
+ *    floats, 16-byte aligned, followed by a table of
+ *    ceil(nx/4)*ceil(ny/4)*ceil(nz/4) floats, one per 4x4x4 block of
+ *    trilinear cells: the rho every corner of the block's cells holds when
+ *    they are all equal, NaN otherwise (the ray-march adds runs of samples in
+ *    uniform blocks at once).  dh_rlsp_from_hu / dh_rlsp_pad write both
+ *    parts; dh_padded_len gives the total.
  *  - Return value 0 = success; negative = error, message via dh_last_error().
  *  - Entry points are re-entrant; per-thread error text.
  */

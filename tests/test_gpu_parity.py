@@ -91,10 +91,21 @@ def test_rlsp_from_hu_matches_np_interp(dev):
     out = D.rlsp_from_hu(torch.from_numpy(hu).to(dev), grid, hk, rk, dev).cpu().numpy()
     nz, ny, nx = hu.shape
     q = out[: 4 * nx * ny * (nz + 1)].reshape(nz + 1, ny, nx, 4)
+    expect = _quads_np(ref)
+    # first component: np.interp's value within 1 ulp
+    np.testing.assert_array_max_ulp(q[..., 0], expect[..., 0], maxulp=1)
+    # bilinear terms: differences of values that are each within 1 ulp
+    np.testing.assert_allclose(q[..., 1:], expect[..., 1:], rtol=0, atol=4 * np.finfo(np.float32).eps *
+                               float(np.abs(ref).max()))
+
+
+def _quads_np(rho):
+    """Padded quads of rho without flags: (a, b - a, c - a, (d - c) - (b - a)) in float32."""
+    nz, ny, nx = rho.shape
     pad = np.zeros((nz + 1, ny + 1, nx + 1), np.float32)
-    pad[:nz, :ny, :nx] = ref
-    expect = np.stack([pad[:, :ny, :nx], pad[:, :ny, 1:], pad[:, 1:, :nx], pad[:, 1:, 1:]], axis=-1)
-    np.testing.assert_array_max_ulp(q, expect, maxulp=1)
+    pad[:nz, :ny, :nx] = rho
+    a, b, c, d = pad[:, :ny, :nx], pad[:, :ny, 1:], pad[:, 1:, :nx], pad[:, 1:, 1:]
+    return np.stack([a, b - a, c - a, (d - c) - (b - a)], axis=-1)
 
 
 def _block_table_np(rho):
@@ -142,6 +153,9 @@ def test_block_table_and_uniform_skipping(dev, gantry, couch):
     want = _block_table_np(rlsp)
     assert np.array_equal(np.isnan(table), np.isnan(want))
     assert np.array_equal(table[~np.isnan(want)], want[~np.isnan(want)])
+    # quads: the bilinear form, bit for bit
+    q = padded[: 4 * nx * ny * (nz + 1)].reshape(nz + 1, ny, nx, 4)
+    assert np.array_equal(q.view(np.uint32), _quads_np(rlsp).view(np.uint32))
     uniform = float(np.mean(~np.isnan(want)))
     geom = _oracle_geom(g, beam, model)
     s = float(np.float32(g.spacing[0]))
