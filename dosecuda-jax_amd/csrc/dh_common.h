@@ -74,6 +74,21 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblocks) {
     return base + slot;
 }
 
+// Chunked form: XCD x takes chunks of C consecutive workgroups, chunk k of XCD
+// x being workgroups ((k * 8 + x) * C .. + C).  All XCDs move through the work
+// together (the working set of the whole chip stays one region of the item
+// space, which the shared Infinity Cache holds) while each XCD's L2 sees runs
+// of C neighbouring workgroups.  Bijective on [0, nblocks): the last partial
+// round of 8 chunks keeps the identity mapping.
+template <unsigned C>
+__device__ __forceinline__ unsigned xcd_chunk(unsigned bid, unsigned nblocks) {
+    constexpr unsigned NXCD = 8;
+    const unsigned full = nblocks / (NXCD * C) * (NXCD * C);
+    if (bid >= full) return bid;
+    const unsigned xcd = bid % NXCD, slot = bid / NXCD;
+    return ((slot / C) * NXCD + xcd) * C + slot % C;
+}
+
 // Voxel centre in image coordinates relative to the isocentre (pointIJKtoXYZ,
 // CudaClasses.cuh:199-203): p = ijk * s - iso.
 __device__ __forceinline__ float3 voxel_xyz(const Grid &g, const Geom &b, int i, int j, int k) {
