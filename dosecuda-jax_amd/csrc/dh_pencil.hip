@@ -176,6 +176,7 @@ struct VoxTerms {
     float Af, Bf;    // primary / halo amplitude (idd * weight / (2 pi sigma^2 + 1e-8))
     float a1, a2;    // exponent scales in the exp2 domain: -0.5 log2(e) / sigma^2 (0 if sigma <= 0)
     float sh;        // halo sigma (culling bound)
+    float st;        // primary sigma where positive, else 0 (culling bound of the primary term)
 };
 
 __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, float wet, float dsrc) {
@@ -205,6 +206,7 @@ __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, f
     t.a1 = pos1 ? (-0.5f * kLog2e) * fast_rcp(st2) : 0.0f;
     t.a2 = pos2 ? (-0.5f * kLog2e) * fast_rcp(sh2) : 0.0f;
     t.sh = fast_sqrt(sh2);
+    t.st = pos1 ? st : 0.0f;
     return t;
 }
 
@@ -304,17 +306,27 @@ __global__ __launch_bounds__(64) void k_layer_prep_batch(const BeamCtx *__restri
 // a time (reaches the brick's WET range, and with culling: some spot of the
 // layer within reach of the brick); the wave then walks the surviving layers.
 // Per layer, each 64-spot chunk is judged lane-parallel against the brick
-// (distance from the brick centre to the spot axis vs cutoff * max halo sigma
-// + brick radius) and the surviving spots are evaluated four at a time, their
-// records coming in through scalar loads.
+// (distance from the brick centre to the spot axis vs cutoff * max sigma +
+// brick radius, once with the halo sigma and once with the narrower primary
+// sigma) and the surviving spots are evaluated four at a time, their records
+// coming in through scalar loads: spots within reach of the primary Gaussian
+// with both terms, the rest of the halo's reach with the halo term alone (the
+// primary term there is below exp(-cutoff^2/2) of its peak, as for any culled
+// spot).
 //
 // COUNT: no dose; *n_dense += in-range voxels x layer spots (the reference's
-// work), *n_exec += in-range voxels x spots evaluated here.
+// work), *n_exec += in-range voxels x spots evaluated here, a spot evaluated
+// with the halo term alone counting one half.
 __device__ __forceinline__ void spot_eval(const SpotExt &q, float hx, float hy, float hz, float a1, float a2,
                                           float &P, float &H) {
     const float d2 = axis_d2(hx, hy, hz, q);
     P = fmaf(q.mu, __builtin_amdgcn_exp2f(a1 * d2), P);
     H = fmaf(q.mu, __builtin_amdgcn_exp2f(a2 * d2), H);
+}
+
+// Halo term only: the spot is beyond reach of the (narrower) primary Gaussian.
+__device__ __forceinline__ void spot_eval_h(const SpotExt &q, float hx, float hy, float hz, float a2, float &H) {
+    H = fmaf(q.mu, __builtin_amdgcn_exp2f(a2 * axis_d2(hx, hy, hz, q)), H);
 }
 
 struct VoxelOf {
@@ -352,6 +364,7 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
     const SpotExt *__restrict__ sp = c.sp;
 
     float acc = 0.0f;
+    unsigned long long n_exec2 = 0;  // COUNT: twice the evaluated pairs
     for (int l0 = 0; l0 < c.n_layers; l0 += WAVE) {
         bool lrel = false;
         if (l0 + lane < c.n_layers) {
@@ -368,25 +381,34 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
             const unsigned long long amask = __ballot(act);
             if (!amask) continue;
             const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
-            float rc2 = 0.0f;
+            float rcH2 = 0.0f, rcP2 = 0.0f;
             if (CULL) {
-                const float rc = fmaf(c.cut, wave_max(act ? T.sh : 0.0f), c.rbrick);
-                rc2 = rc * rc;
+                const float rh = fmaf(c.cut, wave_max(act ? T.sh : 0.0f), c.rbrick);
+                const float rp = fmaf(c.cut, wave_max(act ? T.st : 0.0f), c.rbrick);
+                rcH2 = rh * rh;
+                rcP2 = rp * rp;
             }
             float P = 0.0f, H = 0.0f;
             const SpotExt *lsp = gptr(sp, L.start);
             for (int c0 = 0; c0 < L.n; c0 += WAVE) {
                 const int nin = min(WAVE, L.n - c0);
-                unsigned long long mask;
+                unsigned long long mask, hmask;
                 if (CULL) {
-                    bool rel = false;
-                    if (lane < nin) rel = axis_d2(B.hc.x, B.hc.y, B.hc.z, lsp[c0 + lane]) < rc2;
-                    mask = __ballot(rel);
+                    bool relH = false, relP = false;
+                    if (lane < nin) {
+                        const float d2c = axis_d2(B.hc.x, B.hc.y, B.hc.z, lsp[c0 + lane]);
+                        relH = d2c < rcH2;
+                        relP = d2c < rcP2;  // implies relH: the halo sigma is the wider
+                    }
+                    mask = __ballot(relP);
+                    hmask = __ballot(relH) & ~mask;
                 } else {
                     mask = nin == WAVE ? ~0ull : ((1ull << nin) - 1ull);
+                    hmask = 0ull;
                 }
                 if (COUNT) {
-                    n_exec += (unsigned long long)__popcll(amask) * (unsigned)__popcll(mask);
+                    n_exec2 += (unsigned long long)__popcll(amask) * (2u * (unsigned)__popcll(mask) +
+                                                                      (unsigned)__popcll(hmask));
                     continue;
                 }
                 const SpotExt *csp = lsp + c0;
@@ -410,11 +432,32 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                     mask &= mask - 1ull;
                     spot_eval(csp[s0], h.x, h.y, h.z, T.a1, T.a2, P, H);
                 }
+                while (__popcll(hmask) >= 4) {
+                    const int s0 = __builtin_ctzll(hmask);
+                    hmask &= hmask - 1ull;
+                    const int s1 = __builtin_ctzll(hmask);
+                    hmask &= hmask - 1ull;
+                    const int s2 = __builtin_ctzll(hmask);
+                    hmask &= hmask - 1ull;
+                    const int s3 = __builtin_ctzll(hmask);
+                    hmask &= hmask - 1ull;
+                    const SpotExt q0 = csp[s0], q1 = csp[s1], q2 = csp[s2], q3 = csp[s3];
+                    spot_eval_h(q0, h.x, h.y, h.z, T.a2, H);
+                    spot_eval_h(q1, h.x, h.y, h.z, T.a2, H);
+                    spot_eval_h(q2, h.x, h.y, h.z, T.a2, H);
+                    spot_eval_h(q3, h.x, h.y, h.z, T.a2, H);
+                }
+                while (hmask) {
+                    const int s0 = __builtin_ctzll(hmask);
+                    hmask &= hmask - 1ull;
+                    spot_eval_h(csp[s0], h.x, h.y, h.z, T.a2, H);
+                }
             }
             if (act) acc = fmaf(T.Af, P, fmaf(T.Bf, H, acc));
         }
     }
     if (COUNT) {
+        n_exec += n_exec2 / 2;
         for (int l = 0; l < c.n_layers; ++l) {
             const LayerC L = gld(lc, l);
             if (L.ok) n_dense += (unsigned long long)__popcll(__ballot(wet <= L.lim)) * (unsigned)L.n;
