@@ -52,6 +52,17 @@ struct BrickGrid {
     __host__ __device__ long long count() const { return (long long)bx * by * bz; }
 };
 
+// Brick index -> brick coordinates.  Bricks of one beam fit in 32 bits (at
+// most nx*ny*nz < 2^28), and 32-bit unsigned division costs the VALU a few
+// instructions where 64-bit division is a long software sequence.
+__device__ __forceinline__ void brick_coords(const BrickGrid &bg, long long brick, int &bi, int &bj, int &bk) {
+    const unsigned b = (unsigned)brick, bx = (unsigned)bg.bx, by = (unsigned)bg.by;
+    const unsigned t = b / bx;
+    bi = (int)(b - t * bx);
+    bk = (int)(t / by);
+    bj = (int)(t - (unsigned)bk * by);
+}
+
 inline BrickGrid brick_grid(const Grid &g) {
     BrickGrid b;
     b.bx = (g.nx + BRICK - 1) / BRICK;

@@ -219,10 +219,7 @@ __device__ __forceinline__ float axis_d2(float hx, float hy, float hz, const Spo
 }
 
 __device__ __forceinline__ void brick_of(const BrickGrid &bg, long long brick, int &bi, int &bj, int &bk) {
-    bi = (int)(brick % bg.bx);
-    const long long t = brick / bg.bx;
-    bj = (int)(t % bg.by);
-    bk = (int)(t / bg.by);
+    brick_coords(bg, brick, bi, bj, bk);
 }
 
 // ---------------------------------------------------------------------------
@@ -816,7 +813,9 @@ __global__ __launch_bounds__(64) void k_pb_adjoint_batch(const BeamCtx *__restri
     const int bm = find_owner(off, n_beams, item);
     const BeamCtx &c = load_ctx(ctx, bm);
     const long long local = item - off[bm];
-    const int cid = (int)(local / c.nslab), slab = (int)(local % c.nslab);
+    // items of one beam fit in 32 bits (chunks x slabs)
+    const unsigned lu = (unsigned)local, ns = (unsigned)c.nslab;
+    const int cid = (int)(lu / ns), slab = (int)(lu - (unsigned)(lu / ns) * ns);
     unsigned long long nd = 0, ne = 0;
     adjoint_item<CULL, CS, false>(c, cid, slab, threadIdx.x, s_vox, nd, ne);
 }

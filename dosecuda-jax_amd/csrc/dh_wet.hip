@@ -291,25 +291,23 @@ __device__ __forceinline__ void raymarch_voxel_at(const BeamCtx &c, int i, int j
 }
 
 __device__ __forceinline__ void raymarch_brick(const BeamCtx &c, long long brick, int lane) {
-    const int bi = (int)(brick % c.bg.bx);
-    const long long bt = brick / c.bg.bx;
-    const int bj = (int)(bt % c.bg.by), bk = (int)(bt / c.bg.by);
+    int bi, bj, bk;
+    brick_coords(c.bg, brick, bi, bj, bk);
     raymarch_voxel_at(c, bi * BRICK + (lane & 3), bj * BRICK + ((lane >> 2) & 3), bk * BRICK + (lane >> 4));
 }
 
 // Bricks b0 .. b0 + 3 of one row: wave w, lane -> voxel of the 16 x 4 patch at
 // layer w.  Lanes of bricks the reach pass does not need stay idle.
 __device__ __forceinline__ void raymarch_row(const BeamCtx &c, long long b0, int w, int lane, int use_need) {
-    const int bi = (int)(b0 % c.bg.bx);
-    const long long bt = b0 / c.bg.bx;
-    const int bj = (int)(bt % c.bg.by), bk = (int)(bt / c.bg.by);
+    int bi, bj, bk;
+    brick_coords(c.bg, b0, bi, bj, bk);
     const int x = lane & 15;
     if (use_need && !(gld(c.need, b0 + (x >> 2)) & 2u)) return;
     raymarch_voxel_at(c, bi * BRICK + x, bj * BRICK + (lane >> 4), bk * BRICK + w);
 }
 
 __device__ __forceinline__ bool in_one_row(const BrickGrid &bg, long long b0, long long end) {
-    return b0 + 3 < end && (int)(b0 % bg.bx) + 3 < bg.bx;
+    return b0 + 3 < end && (int)((unsigned)b0 % (unsigned)bg.bx) + 3 < bg.bx;
 }
 
 // Single beam: workgroup q takes bricks 4q .. 4q + 3.
@@ -456,9 +454,10 @@ __global__ __launch_bounds__(256) void k_wet_smooth_batch(const BeamCtx *__restr
     const long long nvox = (long long)c.g.nx * c.g.ny * c.g.nz;
     if (t >= nvox) return;
     if (use_need) {
-        const int i = (int)(t % c.g.nx);
-        const long long r = t / c.g.nx;
-        const int j = (int)(r % c.g.ny), k = (int)(r / c.g.ny);
+        // voxel indices fit in 32 bits (the grid check keeps nx*ny*(nz+1) below 2^28)
+        const unsigned tu = (unsigned)t, nx = (unsigned)c.g.nx, ny = (unsigned)c.g.ny;
+        const unsigned r = tu / nx;
+        const int i = (int)(tu - r * nx), k = (int)(r / ny), j = (int)(r - (unsigned)k * ny);
         const long long brick = (i >> 2) + (long long)c.bg.bx * ((j >> 2) + (long long)c.bg.by * (k >> 2));
         if (!(gld(c.need, brick) & 1u)) {
             gst(c.wet, t, kWetOutside);
