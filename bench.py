@@ -73,6 +73,21 @@ def loss_grad(dose, tmask, bmask, n_t, n_b, out=None):
     return g
 
 
+class LossGrad:
+    """The same dL/dD as loss_grad as one fused element-wise kernel:
+    dL/dD = a * D + b with a = 2/n_t [target] + 0.2/n_b [body], b = -4/n_t [target]."""
+
+    def __init__(self, tmask, bmask, n_t, n_b):
+        self.a = tmask.float() * (2.0 / n_t) + bmask.float() * (0.2 / n_b)
+        self.b = tmask.float() * (-4.0 / n_t)
+
+    def __call__(self, dose, out=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(dose)
+        return torch.addcmul(self.b, self.a, dose, out=out)
+
+
 def plan_copies(K):
     """K head-phantom plans: copy 0 with the seed-2 weights, the others with weights of their own."""
     import numpy as np
@@ -414,6 +429,7 @@ def main():
     tmask, bmask = masks(g, dev)
     n_t = float(tmask.sum().item())
     n_b = float(bmask.sum().item())
+    lgrad = LossGrad(tmask, bmask, n_t, n_b)
     nvox = int(np.prod(engines[0].shape))
     nb = len(engines[0].beams)
     n_spots = sum(b.spots.n_spots for b in engines[0].beams)
@@ -429,7 +445,7 @@ def main():
     def step():
         doses = bd.forward(mus, xys)
         for d, gd in zip(doses, bd.grad_doses):
-            loss_grad(d, tmask, bmask, n_t, n_b, out=gd)
+            lgrad(d, out=gd)
         return doses, bd.adjoint(None, reuse_prep=True, input_order=False)
 
     log("batch items %s" % bd.item_counts())
@@ -497,7 +513,7 @@ def main():
             b.spots.set_weights_positions(b.mu0, b.xy0)
             D.pb_forward(b.wet, dose, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0,
                          accumulate=k > 0)
-        gd = loss_grad(dose, tmask, bmask, n_t, n_b)
+        gd = lgrad(dose)
         for b in e0.beams:
             D.pb_adjoint(b.wet, gd, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0)
         return dose, gd
@@ -514,11 +530,21 @@ def main():
     else:
         e0.compute_wet(force=True)
         dose_d = e0.dose()
-        gd_dense = loss_grad(dose_d, tmask, bmask, n_t, n_b)
+        gd_dense = lgrad(dose_d)
     counts_ref = torch.zeros(4, dtype=torch.int64, device=dev)
     for b in e0.beams:
         D.pb_count_pairs(b.wet, gd_dense, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes,
                          cutoff_sigma=args.cutoff_sigma, counts=counts_ref)
+    # drop-in path with host arrays in and out (PCIe transfers included): computeIMPTPlan of plan copy 0,
+    # forward only (the reference's call has no gradient)
+    dropin_ms = None
+    if args.plain_steps > 0:
+        log("drop-in computeIMPTPlan (host arrays, PCIe included) %d calls" % args.plain_steps)
+        g.computeIMPTPlan(plans[0])
+        td = time.perf_counter()
+        for _ in range(args.plain_steps):
+            g.computeIMPTPlan(plans[0])
+        dropin_ms = (time.perf_counter() - td) / args.plain_steps * 1e3
     cc = [int(v) for v in counts_cull.cpu().tolist()]
     cr = [int(v) for v in counts_ref.cpu().tolist()]
     ref_pairs_plan = cr[0] + cr[2]           # reference-equivalent pairs per plan
@@ -583,6 +609,7 @@ def main():
         },
         "ms_per_plan": ms_plan,
         "plain_ms_per_plan": plain_ms,
+        "dropin_forward_ms_per_plan_host_arrays": dropin_ms,
         "speedup_vs_plain_per_plan": (plain_ms / ms_plan) if plain_ms else None,
         "eager_ms_per_step": ms_eager,
         "kernels": kernels,

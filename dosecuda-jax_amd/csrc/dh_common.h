@@ -170,7 +170,9 @@ constexpr int N_STENCIL = 54;
 struct Stencil {
     signed char dx[N_STENCIL], dy[N_STENCIL], dz[N_STENCIL];
     unsigned char tie[N_STENCIL], dr[N_STENCIL];
+    unsigned char pad[2];  // 272 bytes: the batched smoothing reads the table in whole dwords
 };
+static_assert(sizeof(Stencil) % 4 == 0, "the batched smoothing reads Stencil in dwords");
 
 // Everything the kernels of one beam read or write.  Passed by value to the
 // single-beam kernels and read from a device table by the batched ones.
