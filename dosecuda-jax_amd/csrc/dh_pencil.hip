@@ -685,6 +685,7 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
     if (sval) q = gld(c.sp, spot);
     else q = SpotExt{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f};
     const float itt = 1.0f / (q.tx * q.tx + q.ty * q.ty + 1.0f);
+    const float ttx = q.tx * itt, tty = q.ty * itt;  // t_xy / |t|^2
     const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
     const unsigned long long chunk_n = (unsigned long long)min(CS, L.n - c0);
     LayerC R = L;  // reach of this chunk's spots only
@@ -692,7 +693,8 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
 
     const long long nb = c.bg.count();
     const long long b0 = nb * slab / c.nslab, b1 = nb * (slab + 1) / c.nslab;
-    float K = 0.0f, FX = 0.0f, FY = 0.0f;
+    // dL/dx = FX0 + FX1 / VSADX with FX0 = sum F p_x, FX1 = sum F q_r p_x (same for y)
+    float K = 0.0f, FX0 = 0.0f, FX1 = 0.0f, FY0 = 0.0f, FY1 = 0.0f;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
 
     for (long long base = b0; base < b1; base += WAVE) {
@@ -762,7 +764,7 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
                     // q_e = (e.t)/|t|^2; d2 = e_xy . p (no cancellation)
                     const float u = fmaf(e.hz, q.tx, e.hx - q.x);
                     const float w = fmaf(e.hz, q.ty, e.hy - q.y);
-                    const float qe = fmaf(u, q.tx, w * q.ty) * itt;
+                    const float qe = fmaf(u, ttx, w * tty);
                     const float px = fmaf(-qe, q.tx, u), py = fmaf(-qe, q.ty, w);
                     const float d2 = fmaf(u, px, w * py);
                     const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
@@ -771,9 +773,11 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
                     K = fmaf(e.GB, e2, K);
                     const float F = fmaf(e.GAa, e1, e.GBb * e2);
                     // q of the ray r = e - hz t is qe - hz
-                    const float qr = qe - e.hz;
-                    FX = fmaf(F * fmaf(qr, ivx, 1.0f), px, FX);
-                    FY = fmaf(F * fmaf(qr, ivy, 1.0f), py, FY);
+                    const float Fq = F * (qe - e.hz);
+                    FX0 = fmaf(F, px, FX0);
+                    FX1 = fmaf(Fq, px, FX1);
+                    FY0 = fmaf(F, py, FY0);
+                    FY1 = fmaf(Fq, py, FY1);
                 }
             }
             __syncthreads();
@@ -781,6 +785,7 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
     }
     if (COUNT) return;
     // fold the voxel subsets (fixed order: deterministic)
+    float FX = fmaf(FX1, ivx, FX0), FY = fmaf(FY1, ivy, FY0);
 #pragma unroll
     for (int off = CS; off < WAVE; off <<= 1) {
         K += __shfl_xor(K, off);
