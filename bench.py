@@ -571,6 +571,10 @@ def main():
         roofline = {"kernel": dom_name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(dom_name),
                     "avg_launch_ms": kms[dom_name], "bytes_per_launch": byts}
+        if dom_name == "k_wet_raymarch_batch":
+            # measured with rocprofv3 counters (TA_TA_BUSY, TCP_TOTAL_CACHE_ACCESSES), not from this run
+            roofline["limiter"] = ("L1 tag lookups of the 16-byte trilinear gathers: TA busy 90-100% of cycles at "
+                                   "about one 64-byte segment per clock per CU; HBM is not the bound")
     kernels = {}
     for name, ms in kms.items():
         r = {"avg_ms_per_step": ms}
