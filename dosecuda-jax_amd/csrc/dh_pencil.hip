@@ -563,6 +563,8 @@ __global__ __launch_bounds__(256) void k_brick_summary_batch(const BeamCtx *__re
     if (item >= total) return;
     const int bm = find_owner(off, n_beams, item);
     const BeamCtx &c = load_ctx(ctx, bm);
+    // a beam without spots has no adjoint work and no summary table (bsum null)
+    if (c.n_spots == 0) return;
     brick_summary_one(c, item - off[bm], threadIdx.x & 63);
 }
 
@@ -979,7 +981,10 @@ bool pencil_grid_ok(const dh_grid &g) {
 }
 
 bool spot_set_ok(const dh_spot_set *ss) {
-    if (!ss || ss->n_spots < 0 || ss->n_layers < 0 || (ss->n_spots > 0 && (!ss->spots || !ss->layers))) {
+    // spots without layers are allowed (no spot is delivered: zero dose and
+    // gradients); the layer table is only read when n_layers > 0
+    if (!ss || ss->n_spots < 0 || ss->n_layers < 0 || (ss->n_spots > 0 && !ss->spots) ||
+        (ss->n_spots > 0 && ss->n_layers > 0 && !ss->layers)) {
         set_error("invalid spot set (n_spots=%d, n_layers=%d)", ss ? ss->n_spots : -1, ss ? ss->n_layers : -1);
         return false;
     }

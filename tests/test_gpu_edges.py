@@ -1,0 +1,126 @@
+"""Edge cases of the HIP path vs the FP64 oracle (GPU).
+
+* degenerate grids (1 or 2 voxels along an axis, 1x1x1): WET within 2e-4 cm
+  of the oracle, dose within 1e-3 of D_max (or 1e-6 absolute when there is
+  hardly any dose), with and without culling;
+* a beam whose energy ids are all outside the model (no usable layer) and a
+  beam with a negative id (which blocks every layer, as importLayers does):
+  zero dose and zero gradients, like the oracle;
+* spots far outside the volume: dose equals the oracle's (about zero) and
+  the gradients are finite;
+* a batch whose plans include a beam without spots: WET, dose and gradients
+  bit-identical to the single-beam path.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import impt_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[gpu-edges] " + msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda", 0)
+
+
+def _grid_plan(shape, spacing, spots, gantry=30.0, couch=20.0, seed=3):
+    from DoseCUDA import IMPTPlan
+    from DoseCUDA.workloads import _beam, _grid
+    g = _grid(shape, spacing)
+    g.HU = np.random.default_rng(seed).uniform(-300.0, 400.0, size=shape).astype(np.float32)
+    plan = IMPTPlan()
+    plan.addBeam(_beam(gantry, np.asarray(spots, np.float32), couch=couch))
+    return g, plan
+
+
+def _spots(ids, coords=(-3.0, 0.0, 3.0), mu=0.7):
+    return [(x, y, mu, e) for e in ids for y in coords for x in coords]
+
+
+def _oracle_dose_wet(g, plan, machine_dir):
+    total, _, wets = O.compute_plan(g, plan, machine_dir, return_wet=True)
+    return total, wets
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 6, 3), (2, 2, 2), (5, 1, 7), (3, 9, 1), (6, 5, 4)])
+@pytest.mark.parametrize("cutoff", [0.0, 6.0])
+def test_degenerate_grids(dev, machine_dir, shape, cutoff):
+    from DoseCUDA.engine import IMPTDoseEngine
+    _say("grid %s cutoff %g" % (shape, cutoff))
+    g, plan = _grid_plan(shape, 3.0, _spots([0, 5, 10]))
+    eng = IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=cutoff)
+    eng.compute_wet()
+    dose = eng.dose().cpu().numpy().astype(np.float64)
+    ref, wets = _oracle_dose_wet(g, plan, machine_dir)
+    d_wet = np.abs(eng.beams[0].wet.cpu().numpy() - wets[0]).max()
+    dmax = float(ref.max())
+    err = float(np.abs(dose - ref).max())
+    _say("  wet %.2e cm, dose err %.2e (D_max %.3g)" % (d_wet, err, dmax))
+    assert d_wet <= 2e-4
+    assert err <= max(1e-3 * dmax, 1e-6)
+
+
+@pytest.mark.parametrize("ids", [[98, 120], [-1, 3, 7]])
+def test_beam_without_usable_layers(dev, machine_dir, ids):
+    from DoseCUDA.engine import IMPTDoseEngine
+    _say("energy ids %s" % ids)
+    g, plan = _grid_plan((12, 14, 10), 3.0, _spots(ids))
+    eng = IMPTDoseEngine(g, plan, gpu_id=0)
+    params = eng.parameters()
+    dose = eng.differentiable_dose(params)
+    (dose * dose).sum().backward()
+    ref, _ = _oracle_dose_wet(g, plan, machine_dir)
+    assert float(np.abs(ref).max()) == 0.0
+    assert float(dose.detach().abs().max()) == 0.0
+    assert float(params[0][0].grad.abs().max()) == 0.0 and float(params[0][1].grad.abs().max()) == 0.0
+
+
+def test_spots_out_of_reach(dev, machine_dir):
+    from DoseCUDA.engine import IMPTDoseEngine
+    _say("spots far outside the volume")
+    g, plan = _grid_plan((10, 12, 14), 3.0, _spots([5, 20], coords=(400.0, 430.0)))
+    eng = IMPTDoseEngine(g, plan, gpu_id=0)
+    params = eng.parameters()
+    dose = eng.differentiable_dose(params)
+    (dose * dose).sum().backward()
+    ref, _ = _oracle_dose_wet(g, plan, machine_dir)
+    D = dose.detach().cpu().numpy().astype(np.float64)
+    assert float(np.abs(D - ref).max()) <= max(1e-3 * float(ref.max()), 1e-12)
+    assert torch.isfinite(params[0][0].grad).all() and torch.isfinite(params[0][1].grad).all()
+
+
+def test_batch_with_empty_beam(dev):
+    from DoseCUDA import IMPTPlan
+    from DoseCUDA.batch import BatchedDose
+    from DoseCUDA.engine import IMPTDoseEngine
+    from DoseCUDA.workloads import _beam, _grid
+    _say("batch with a beam without spots")
+    g = _grid((16, 20, 18), 2.5)
+    g.HU = np.random.default_rng(8).uniform(-200.0, 300.0, size=(16, 20, 18)).astype(np.float32)
+    plan = IMPTPlan()
+    plan.addBeam(_beam(0.0, np.asarray(_spots([2, 8]), np.float32)))
+    plan.addBeam(_beam(90.0, np.zeros((0, 4), np.float32)))
+    plan.addBeam(_beam(200.0, np.asarray(_spots([4]), np.float32), couch=15.0))
+    engines = [IMPTDoseEngine(g, plan, gpu_id=0) for _ in range(2)]
+    single = engines[0]
+    params = single.parameters()
+    d_single = single.differentiable_dose(params)
+    (d_single * d_single).sum().backward()
+    bd = BatchedDose(engines, wet_mode="reach")
+    mus = [b.mu0.clone() for e in engines for b in e.beams]
+    xys = [b.xy0.clone() for e in engines for b in e.beams]
+    doses = bd.forward(mus, xys)
+    grads = bd.adjoint([2.0 * d for d in doses])
+    torch.cuda.synchronize()
+    for d in doses:
+        assert torch.equal(d, d_single.detach())
+    for k, (gm, gx) in enumerate(grads[:3]):
+        assert torch.equal(gm, params[k][0].grad) and torch.equal(gx, params[k][1].grad)

@@ -86,5 +86,5 @@ def test_target_plan_gradients_vs_central_fd():
     fd_mu, fd_xy = np.array(fd_mu), np.array(fd_xy)
     e_mu = np.abs(np.array(got_mu) - fd_mu).max() / np.abs(fd_mu).max()
     e_xy = np.abs(np.array(got_xy) - fd_xy).max() / np.abs(fd_xy).max()
-    _say("spots %d, loss %.5f, rel err dL/dmu %.2e, dL/dxy %.2e" % (len(pick), float(loss), e_mu, e_xy))
+    _say("spots %d, loss %.5f, rel err dL/dmu %.2e, dL/dxy %.2e" % (len(pick), float(loss.detach()), e_mu, e_xy))
     assert e_mu <= 1e-3 and e_xy <= 1e-3

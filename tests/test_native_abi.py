@@ -161,6 +161,16 @@ def test_batch_sizes_and_errors_without_gpu():
     beams[0].wet = None
     assert lib.dh_batch_workspace_size(plans, 2, beams, 3, None) == N.DH_ERR_ARG
     beams[0].wet = 4096
+    # spots without layers are a valid (empty) delivery; spots without a spot
+    # array, or layers without a layer table, are not
+    beams[0].spots = N.dh_spot_set(4096, None, 100, 0)
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, None) > 0
+    beams[0].spots = N.dh_spot_set(None, 4096, 100, 3)
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, None) == N.DH_ERR_ARG
+    assert "invalid spot set" in N.last_error()
+    beams[0].spots = N.dh_spot_set(4096, None, 100, 3)
+    assert lib.dh_batch_workspace_size(plans, 2, beams, 3, None) == N.DH_ERR_ARG
+    beams[0].spots = N.dh_spot_set(4096, 4096, 100, 3)
     # setup rejects a small workspace before touching the device
     b = N.dh_batch()
     rc = lib.dh_batch_setup(ctypes.byref(b), plans, 2, beams, 3, None, 6.0, ctypes.c_void_p(4096), 256, None)
