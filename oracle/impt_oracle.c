@@ -1,8 +1,8 @@
 /*
  * impt_oracle.c - CPU restatement (float64) of the reference's proton
- * pencil-beam dose path.  TEST INFRASTRUCTURE ONLY: it is the checker the HIP
- * kernels are compared against and the CPU baseline of the benchmark; nothing
- * in the product links or calls it.
+ * pencil-beam dose path.  TEST INFRASTRUCTURE ONLY: it is the reference the
+ * HIP kernels are compared against in the tests and the CPU baseline of the
+ * benchmark; nothing in the product links or calls it.
  *
  * Primary source: DoseCUDA/Jax/impt_jax.py (the JAX path), evaluated in
  * float64 and in the axis convention of the CUDA extension (numpy axis 2 = x,

@@ -1,7 +1,7 @@
 """FP64 oracle of the proton pencil-beam dose path -- TEST INFRASTRUCTURE ONLY.
 
 Imported by tests/, by __graft_entry__.smoke() and by bench.py's CPU-baseline
-leg, and only as the checker: the product (DoseCUDA package, libdosehip.so)
+leg, and only to compare against: the product (DoseCUDA package, libdosehip.so)
 never imports it.
 
 What it restates (reference files under DoseCUDA/):

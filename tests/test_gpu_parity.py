@@ -3,7 +3,7 @@
 Tolerances (float32 kernels vs float64 oracle):
   * WET (cm): max |dWET| <= 2e-4 cm (0.002 mm) over the volume, raw and
     smoothed.
-  * dose: max |dD| / D_max <= 1e-3 end to end (north-star bar 5e-3), and
+  * dose: max |dD| / D_max <= 1e-3 end to end (the dose bar is 5e-3), and
     <= 2e-5 for the pencil-beam kernel alone on identical WET input; 2%/2 mm
     gamma pass rate 1.0.
   * gradients: relative error of dL/dmu, dL/dxy vs the analytic FP64 adjoint
