@@ -203,6 +203,7 @@ struct BeamCtx {
 
 // Most brick offsets a smoothing stencil reaches (54 entries x 2 x 2 x 2).
 constexpr int MAX_TOUCH = 432;
+static_assert(MAX_TOUCH % 8 == 0, "the dilation pass reads the touch table eight entries at a time");
 
 // WET (cm) written where the reach-limited WET pass skips a brick: beyond any
 // layer's range (1.1 R80), so the dose kernels skip those voxels.
@@ -324,6 +325,25 @@ __device__ __forceinline__ T gld(const T *p, long long i) {
 template <class T>
 __device__ __forceinline__ void gst(T *p, long long i, const T &v) {
     *gptr(p, i) = v;
+}
+
+// Tables that an earlier kernel wrote and the current one only reads (layer
+// records, touch sets, stencils), read through the constant address space: a
+// wave-uniform index then becomes a scalar load even where the kernel also
+// stores through other pointers.
+#define DH_CONST __attribute__((address_space(4)))
+
+template <class T>
+__device__ __forceinline__ T cld(const T *p, long long i) {
+    static_assert(sizeof(T) % 4 == 0, "cld reads whole dwords");
+    const DH_CONST int *q = (const DH_CONST int *)(p + i);
+    union {
+        T t;
+        int w[sizeof(T) / 4];
+    } u;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) u.w[k] = q[k];
+    return u.t;
 }
 
 __device__ __forceinline__ const BeamCtx &load_ctx(const BeamCtx *__restrict__ ctx, int b) { return ctx[b]; }

@@ -580,63 +580,95 @@ __global__ __launch_bounds__(256) void k_brick_summary_batch(const BeamCtx *__re
 //           WET for, i.e. b - o for an offset o of the beam's touch set, has
 //           bit 0 (raw WET needed).
 // Every WET value the dose kernels then read equals the full computation's.
-__device__ __forceinline__ void wet_need_one(const BeamCtx &c, long long brick, int lane) {
-    bool need = !(c.cut > 0.0f);
-    if (!need) {
+// Both passes run one lane per brick: a wave takes 64 consecutive items of the
+// batch's brick space and walks the beams they belong to (normally one), so
+// one context load serves 64 bricks.  `mine`: the lane's item belongs to the
+// beam in hand; the other lanes count as decided and store nothing.
+__device__ __forceinline__ void wet_need_lanes(const BeamCtx &c, long long brick, bool mine) {
+    const bool all = !(c.cut > 0.0f);
+    bool need = mine && all, done = !mine || all;
+    if (__ballot(!done)) {
         int bi, bj, bk;
-        brick_of(c.bg, brick, bi, bj, bk);
+        brick_of(c.bg, mine ? brick : 0, bi, bj, bk);
         const float3 vc = brick_centre(c.g, c.b, bi, bj, bk);
         const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
-        for (int l0 = 0; l0 < c.n_layers; l0 += WAVE) {
-            if (l0 + lane < c.n_layers) {
-                const LayerC L = gld(c.lc, l0 + lane);
-                if (L.ok) need |= !reach_skip(L, brick_info_at(c.b, vc, -1.0f, L.lim), ivx, ivy, c.cut, c.rbrick);
-            }
+        for (int l = 0; l < c.n_layers; ++l) {
+            if (!__ballot(!done)) break;
+            const LayerC L = cld(c.lc, l);
+            if (!done && L.ok && !reach_skip(L, brick_info_at(c.b, vc, -1.0f, L.lim), ivx, ivy, c.cut, c.rbrick))
+                need = done = true;
         }
-        need = __ballot(need) != 0ull;
     }
-    if (lane == 0) gst(c.need, brick, (unsigned char)(need ? 1u : 0u));
+    if (mine) gst(c.need, brick, (unsigned char)(need ? 1u : 0u));
 }
 
-__device__ __forceinline__ void wet_dilate_one(const BeamCtx &c, long long brick, int lane) {
-    int bi, bj, bk;
-    brick_of(c.bg, brick, bi, bj, bk);
-    const unsigned char own = gld(c.need, brick);
-    bool raw = (own & 1u) != 0;
-    for (int t0 = 0; t0 < c.n_touch && !__ballot(raw); t0 += WAVE) {
-        if (t0 + lane < c.n_touch) {
-            const int key = gld(c.touch, t0 + lane);
-            const int si = bi - ((key & 255) - 128), sj = bj - (((key >> 8) & 255) - 128),
-                      sk = bk - (((key >> 16) & 255) - 128);
-            if ((unsigned)si < (unsigned)c.bg.bx && (unsigned)sj < (unsigned)c.bg.by && (unsigned)sk < (unsigned)c.bg.bz)
-                raw |= (gld(c.need, si + (long long)c.bg.bx * (sj + (long long)c.bg.by * sk)) & 1u) != 0;
-        }
+// Offsets of the touch set go eight at a time (their loads in flight
+// together), with a vote after each group; only bit 1 of this brick's own
+// byte changes, and other waves read bit 0 only.
+__device__ __forceinline__ void wet_dilate_lanes(const BeamCtx &c, long long brick, bool mine) {
+    int bi = 0, bj = 0, bk = 0;
+    unsigned own = 0;
+    if (mine) {
+        brick_of(c.bg, brick, bi, bj, bk);
+        own = gld(c.need, brick);
     }
-    // the vote runs in every lane (a ballot inside `lane == 0 && ...` would see
-    // lane 0 only); only bit 1 of this brick's own byte changes, other waves
-    // read bit 0 only
-    const bool any = __ballot(raw) != 0ull;
-    if (lane == 0 && any) gst(c.need, brick, (unsigned char)(own | 2u));
+    bool raw = !mine || (own & 1u);
+    const int bx = c.bg.bx, by = c.bg.by, bz = c.bg.bz;
+    for (int t0 = 0; t0 < c.n_touch; t0 += 8) {
+        if (!__ballot(!raw)) break;
+        const bool open = !raw;
+        // the table holds MAX_TOUCH (a multiple of 8) entries per beam: the
+        // eight keys load together, entries past n_touch are not used
+        int key[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) key[u] = cld(c.touch, t0 + u);
+        unsigned hit = 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int si = bi - ((key[u] & 255) - 128), sj = bj - (((key[u] >> 8) & 255) - 128),
+                      sk = bk - (((key[u] >> 16) & 255) - 128);
+            const bool in = open && t0 + u < c.n_touch && (unsigned)si < (unsigned)bx && (unsigned)sj < (unsigned)by &&
+                            (unsigned)sk < (unsigned)bz;
+            // brick indices fit in 32 bits (the grid check keeps nx*ny*(nz+1)
+            // below 2^28); every lane loads (brick 0 where the offset is not
+            // used), so the eight loads need no branches and fly together
+            const unsigned at = in ? (unsigned)si + (unsigned)bx * ((unsigned)sj + (unsigned)by * (unsigned)sk) : 0u;
+            const unsigned v = gld(c.need, at);
+            hit |= in ? v : 0u;
+        }
+        raw = raw || (hit & 1u);
+    }
+    if (mine && raw) gst(c.need, brick, (unsigned char)(own | 2u));
+}
+
+template <bool DILATE>
+__device__ __forceinline__ void need_wave(const BeamCtx *__restrict__ ctx, const long long *__restrict__ off,
+                                          int n_beams, long long base, long long total) {
+    const long long w0 = base + ((long long)blockIdx.x * 4 + wave_in_block()) * WAVE;
+    if (w0 >= total) return;  // whole wave leaves together
+    const long long item = w0 + (threadIdx.x & 63);
+    const long long last = w0 + WAVE < total ? w0 + WAVE : total;
+    for (long long first = w0; first < last;) {
+        const int bm = find_owner(off, n_beams, first);
+        const long long end = bm + 1 < n_beams && off[bm + 1] < last ? off[bm + 1] : last;
+        const BeamCtx &c = load_ctx(ctx, bm);
+        const bool mine = item >= first && item < end;
+        if (DILATE) wet_dilate_lanes(c, item - off[bm], mine);
+        else wet_need_lanes(c, item - off[bm], mine);
+        first = end;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_wet_need_batch(const BeamCtx *__restrict__ ctx,
                                                        const long long *__restrict__ off, int n_beams,
                                                        long long base, long long total) {
-    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
-    if (item >= total) return;
-    const int bm = find_owner(off, n_beams, item);
-    const BeamCtx &c = load_ctx(ctx, bm);
-    wet_need_one(c, item - off[bm], threadIdx.x & 63);
+    need_wave<false>(ctx, off, n_beams, base, total);
 }
 
 __global__ __launch_bounds__(256) void k_wet_dilate_batch(const BeamCtx *__restrict__ ctx,
                                                          const long long *__restrict__ off, int n_beams,
                                                          long long base, long long total) {
-    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
-    if (item >= total) return;
-    const int bm = find_owner(off, n_beams, item);
-    const BeamCtx &c = load_ctx(ctx, bm);
-    wet_dilate_one(c, item - off[bm], threadIdx.x & 63);
+    need_wave<true>(ctx, off, n_beams, base, total);
 }
 
 // ---------------------------------------------------------------------------
@@ -1075,19 +1107,21 @@ int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long 
 int launch_wet_need_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
                           hipStream_t st, void *const *ev) {
     mark(ev, 2 * DH_TIME_NEED, st);
-    for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
-        const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
-        hipLaunchKernelGGL(k_wet_need_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
-                           n_beams, base, total_bricks);
-        const int rc = check_launch("k_wet_need_batch");
-        if (rc) return rc;
-    }
-    for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
-        const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
-        hipLaunchKernelGGL(k_wet_dilate_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
-                           n_beams, base, total_bricks);
-        const int rc = check_launch("k_wet_dilate_batch");
-        if (rc) return rc;
+    // 256 bricks per workgroup (one per lane)
+    constexpr long long kPer = 4 * WAVE;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (long long base = 0; base < total_bricks; base += kPer * kMaxBlocks) {
+            const long long n = total_bricks - base < kPer * kMaxBlocks ? total_bricks - base : kPer * kMaxBlocks;
+            const dim3 grid((unsigned)((n + kPer - 1) / kPer));
+            if (pass == 0)
+                hipLaunchKernelGGL(k_wet_need_batch, grid, dim3(256), 0, st, ctx, brick_off, n_beams, base,
+                                   total_bricks);
+            else
+                hipLaunchKernelGGL(k_wet_dilate_batch, grid, dim3(256), 0, st, ctx, brick_off, n_beams, base,
+                                   total_bricks);
+            const int rc = check_launch(pass == 0 ? "k_wet_need_batch" : "k_wet_dilate_batch");
+            if (rc) return rc;
+        }
     }
     mark(ev, 2 * DH_TIME_NEED + 1, st);
     return DH_OK;

@@ -25,6 +25,7 @@
 // is exactly that value, so the march adds a run of such samples at once
 // (empty-space skipping; air around the patient is one value because the HU table clamps
 // at -1000 HU).
+#include <algorithm>
 #include "dh_common.h"
 
 namespace dh {
@@ -404,7 +405,6 @@ __device__ __forceinline__ StencilEntry stencil_entry(const Stencil &st, int e) 
 // constant address space in whole dwords, so the (wave-uniform) entries come
 // in through scalar loads and scalar bit-field extracts rather than one
 // vector byte load per field.
-#define DH_CONST __attribute__((address_space(4)))
 
 __device__ __forceinline__ int stencil_byte(const Stencil *st, int off, bool is_signed) {
     const int w = ((const DH_CONST int *)st)[off >> 2];
@@ -545,6 +545,12 @@ int make_touch(const Stencil &st, int *out, int cap) {
                     if (!seen && n < cap) out[n++] = key;
                 }
     }
+    // nearest offsets first: the dilation pass stops at the first hit
+    auto len2 = [](int key) {
+        const int ox = (key & 255) - 128, oy = ((key >> 8) & 255) - 128, oz = ((key >> 16) & 255) - 128;
+        return ox * ox + oy * oy + oz * oz;
+    };
+    std::stable_sort(out, out + n, [&](int a, int b) { return len2(a) < len2(b); });
     return n;
 }
 
