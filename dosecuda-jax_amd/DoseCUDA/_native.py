@@ -115,6 +115,7 @@ SIGNATURES = {
     "dh_pb_count_pairs": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom),
                                          ctypes.POINTER(dh_beam_model), ctypes.POINTER(dh_spot_set),
                                          ctypes.c_float, _P, _P, ctypes.c_int64, _P]),
+    "dh_spot_params": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int32, _P]),
     "dh_batch_workspace_size": (ctypes.c_int64, [ctypes.POINTER(dh_batch_plan), ctypes.c_int32,
                                                  ctypes.POINTER(dh_batch_beam), ctypes.c_int32,
                                                  ctypes.POINTER(dh_batch_tune)]),

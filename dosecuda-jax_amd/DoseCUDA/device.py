@@ -97,11 +97,23 @@ class DeviceSpots:
 
     def set_weights_positions(self, mu=None, xy=None):
         """Overwrite weights / positions from tensors in INPUT spot order
-        (device, float32): mu [S], xy [S, 2]."""
+        (device): mu [S], xy [S, 2]; one library launch (dh_spot_params)."""
+        if mu is None and xy is None:
+            return
         if mu is not None:
-            self.spots[:, 2] = mu.detach().to(torch.float32)[self.order_t]
+            mu = mu.detach().to(device=self.device, dtype=torch.float32).contiguous()
+            if mu.numel() != self.n_spots:
+                raise ValueError("spot weights: %d values for %d spots" % (mu.numel(), self.n_spots))
         if xy is not None:
-            self.spots[:, :2] = xy.detach().to(torch.float32)[self.order_t]
+            xy = xy.detach().to(device=self.device, dtype=torch.float32).contiguous()
+            if xy.numel() != 2 * self.n_spots:
+                raise ValueError("spot positions: %d values for %d spots" % (xy.numel(), self.n_spots))
+        if self.n_spots == 0:
+            return
+        lib = N.load()
+        N.check(lib.dh_spot_params(N.ptr(self.spots), N.ptr(self.order_t), None if mu is None else N.ptr(mu),
+                                   None if xy is None else N.ptr(xy), self.n_spots, N.stream_handle(self.device)),
+                "dh_spot_params")
 
     def to_input_order(self, sorted_vals):
         out = torch.empty_like(sorted_vals)

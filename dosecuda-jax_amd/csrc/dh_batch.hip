@@ -35,7 +35,7 @@ enum Table {
     T_SMOOTH_OFF,   // smoothing blocks
     T_SPOT_OFF,     // spot records
     T_LAYER_OFF,    // layer records
-    T_PART_OFF,     // adjoint partials
+    T_PART_OFF,     // adjoint partial records per beam (bookkeeping; no kernel reads it)
     T_ADJ32_OFF,    // adjoint workgroups, 32-spot chunks
     T_ADJ64_OFF,    // adjoint workgroups, 64-spot chunks
     T_REDUCE_OFF,   // reduction blocks
@@ -222,7 +222,7 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
         // plans whose dose is not written by their beam directly (several
         // beams, none, or a beam with a volume of its own) get the sum pass
         L.sum.push_back(L.n_sum);
-        if (!direct) L.n_sum += (nvox + 255) / 256;
+        if (!direct) L.n_sum += (nvox + PLAN_SUM_VOXELS - 1) / PLAN_SUM_VOXELS;
     }
     L.bytes = off;
     L.flags = flags;
@@ -387,10 +387,9 @@ extern "C" int dh_batch_adjoint(dh_batch *b, int32_t flags, void *stream) {
     const int rc = prepare_records(b, flags, st, "dh_batch_adjoint");
     if (rc) return rc;
     return launch_adjoint_batch(table<BeamCtx>(b, T_CTX), b->n_beams, table<long long>(b, T_BRICK_OFF), b->n_bricks,
-                                table<long long>(b, T_PART_OFF), b->n_partials, table<long long>(b, T_ADJ32_OFF),
-                                b->n_adj32, table<long long>(b, T_ADJ64_OFF), b->n_adj64,
-                                table<long long>(b, T_REDUCE_OFF), b->n_reduce_blocks, b->cutoff_sigma > 0.0f, st,
-                                b->events);
+                                table<long long>(b, T_ADJ32_OFF), b->n_adj32, table<long long>(b, T_ADJ64_OFF),
+                                b->n_adj64, table<long long>(b, T_REDUCE_OFF), b->n_reduce_blocks,
+                                b->cutoff_sigma > 0.0f, st, b->events);
 }
 
 extern "C" int dh_batch_timing(dh_batch *b, void *const *events) {

@@ -201,6 +201,9 @@ struct BeamCtx {
     int n_touch, pad1;
 };
 
+// Voxels per block of the plan-sum pass (256 threads x 4).
+constexpr long long PLAN_SUM_VOXELS = 1024;
+
 // Most brick offsets a smoothing stencil reaches (54 entries x 2 x 2 x 2).
 constexpr int MAX_TOUCH = 432;
 static_assert(MAX_TOUCH % 8 == 0, "the dilation pass reads the touch table eight entries at a time");
@@ -281,10 +284,9 @@ int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_b
                          const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
                          bool cull, int persist_blocks, hipStream_t st, void *const *ev = nullptr);
 int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
-                         const long long *part_off, long long total_part, const long long *item_off32,
-                         long long total_items32, const long long *item_off64, long long total_items64,
-                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st,
-                         void *const *ev = nullptr);
+                         const long long *item_off32, long long total_items32, const long long *item_off64,
+                         long long total_items64, const long long *reduce_off, long long total_reduce, bool cull,
+                         hipStream_t st, void *const *ev = nullptr);
 
 // ---------------------------------------------------------------------------
 // wave64 helpers

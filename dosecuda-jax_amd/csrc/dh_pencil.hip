@@ -277,6 +277,22 @@ __device__ __forceinline__ void layer_prep_one(const BeamCtx &cx, int l, int lan
 
 __global__ __launch_bounds__(64) void k_layer_prep(BeamCtx c) { layer_prep_one(c, blockIdx.x, threadIdx.x); }
 
+// Spot weights / positions from input-order arrays (one thread per sorted spot).
+__global__ __launch_bounds__(256) void k_spot_params(dh_spot *__restrict__ spots, const long long *__restrict__ order,
+                                                    const float *__restrict__ mu, const float *__restrict__ xy, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long r = gld(order, i);
+    if ((unsigned long long)r >= (unsigned long long)n) return;
+    dh_spot s = gld(spots, i);
+    if (mu) s.mu = gld(mu, r);
+    if (xy) {
+        s.x = gld(xy, 2 * r);
+        s.y = gld(xy, 2 * r + 1);
+    }
+    gst(spots, i, s);
+}
+
 // Batched preparation: one thread per spot of every beam, then one wave per
 // layer of every beam.
 __global__ __launch_bounds__(256) void k_spot_prep_batch(const BeamCtx *__restrict__ ctx,
@@ -524,17 +540,28 @@ __global__ __launch_bounds__(256) void k_pb_forward_batch(const BeamCtx *__restr
 // Dose of a plan whose beams write volumes of their own: the sum of the beam
 // volumes, added in beam order (the order of single-beam calls with
 // accumulate = 1); zero for a plan without beams.  Plan p owns blocks
-// [off[p], off[p + 1]) of 256 voxels.
+// [off[p], off[p + 1]) of PLAN_SUM_VOXELS voxels; a thread takes four voxels
+// 256 apart, so each beam's four loads are in flight together.
 __global__ __launch_bounds__(256) void k_plan_sum(const PlanCtx *__restrict__ plans, const BeamCtx *__restrict__ ctx,
                                                  const long long *__restrict__ off, int n_plans) {
     const long long blk = blockIdx.x;
     const int p = find_owner(off, n_plans, blk);
     const PlanCtx P = plans[p];
-    const long long t = (blk - off[p]) * 256 + threadIdx.x;
-    if (t >= P.nvox) return;
-    float d = 0.0f;  // 0 + D_0 == D_0: the same sums as beam-by-beam accumulation
-    for (int b = P.beam0; b < P.beam0 + P.n_beams; ++b) d += gld(ctx[b].dose, t);
-    gst(P.dose, t, d);
+    const long long t0 = (blk - off[p]) * PLAN_SUM_VOXELS + threadIdx.x;
+    float d[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // 0 + D_0 == D_0: the same sums as beam-by-beam accumulation
+    for (int b = P.beam0; b < P.beam0 + P.n_beams; ++b) {
+        const float *bd = ctx[b].dose;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long long t = t0 + 256 * q;
+            if (t < P.nvox) d[q] += gld(bd, t);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const long long t = t0 + 256 * q;
+        if (t < P.nvox) gst(P.dose, t, d[q]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -556,16 +583,54 @@ __global__ __launch_bounds__(256) void k_brick_summary(BeamCtx c) {
     brick_summary_one(c, brick, threadIdx.x & 63);
 }
 
+// Batched: a wave takes SUMMARY_BRICKS consecutive items and issues the loads
+// of all of them before the reductions (one brick per wave leaves the loads
+// of a wave to run one after the other).  A beam without spots has no adjoint
+// work and no summary table (bsum null).
+constexpr int SUMMARY_BRICKS = 16;
+
 __global__ __launch_bounds__(256) void k_brick_summary_batch(const BeamCtx *__restrict__ ctx,
                                                             const long long *__restrict__ off, int n_beams,
                                                             long long base, long long total) {
-    const long long item = base + (long long)blockIdx.x * 4 + wave_in_block();
-    if (item >= total) return;
-    const int bm = find_owner(off, n_beams, item);
-    const BeamCtx &c = load_ctx(ctx, bm);
-    // a beam without spots has no adjoint work and no summary table (bsum null)
-    if (c.n_spots == 0) return;
-    brick_summary_one(c, item - off[bm], threadIdx.x & 63);
+    const long long w0 = base + ((long long)blockIdx.x * 4 + wave_in_block()) * SUMMARY_BRICKS;
+    if (w0 >= total) return;  // whole wave leaves together
+    const int lane = threadIdx.x & 63;
+    int bm = find_owner(off, n_beams, w0);
+    long long next = bm + 1 < n_beams ? off[bm + 1] : total;  // first item of the next beam
+    int owner[SUMMARY_BRICKS];
+    float gv[SUMMARY_BRICKS], wv[SUMMARY_BRICKS];  // dL/dD (0 off the grid) and WET per item
+#pragma unroll
+    for (int q = 0; q < SUMMARY_BRICKS; ++q) {
+        const long long item = w0 + q;
+        while (item >= next && bm + 1 < n_beams) {
+            ++bm;
+            next = bm + 1 < n_beams ? off[bm + 1] : total;
+        }
+        owner[q] = bm;
+        gv[q] = wv[q] = 0.0f;
+        if (item < total) {
+            const BeamCtx &c = load_ctx(ctx, bm);
+            if (c.n_spots > 0) {
+                int bi, bj, bk, i, j, k;
+                const VoxelOf vo = brick_voxel(c.g, c.bg, item - off[bm], lane, bi, bj, bk, i, j, k);
+                if (vo.valid) {
+                    gv[q] = gld(c.gdose, vo.idx);
+                    wv[q] = gld(c.wet, vo.idx);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SUMMARY_BRICKS; ++q) {
+        const long long item = w0 + q;
+        if (item >= total) break;
+        const BeamCtx &c = load_ctx(ctx, owner[q]);
+        if (c.n_spots == 0) continue;
+        const bool act = gv[q] != 0.0f;
+        const float v = wv[q] * 10.0f;
+        const float lo = wave_min(act ? v : 3.0e38f), hi = wave_max(act ? v : -3.0e38f);
+        if (lane == 0) gst(c.bsum, item - off[owner[q]], make_float2(lo, hi));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -859,29 +924,37 @@ __global__ __launch_bounds__(64) void k_pb_adjoint_batch(const BeamCtx *__restri
     adjoint_item<CULL, CS, false>(c, cid, slab, threadIdx.x, s_vox, nd, ne);
 }
 
-// Zero the adjoint partials (spots that no chunk covers keep zeros).
-__global__ __launch_bounds__(256) void k_zero4(float4 *__restrict__ p, long long n) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-__global__ __launch_bounds__(256) void k_zero4_batch(const BeamCtx *__restrict__ ctx,
-                                                    const long long *__restrict__ off, int n_beams, long long total) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int bm = find_owner(off, n_beams, t);
-    gst(ctx[bm].partial, t - off[bm], make_float4(0.f, 0.f, 0.f, 0.f));
-}
-
 // 256 threads = 64 spots x 4 slab groups; each group sums every 4th slab,
-// then the 4 group sums are added in a fixed order.
+// then the 4 group sums are added in a fixed order.  Every adjoint workgroup of
+// a usable layer writes its chunk's partials for its slab, so the partials of
+// a spot in a usable layer are all written and nothing needs clearing first;
+// a spot outside every usable layer (layer records are disjoint runs) gets
+// zero gradients without reading its partials.
 __device__ __forceinline__ void reduce_spots(const BeamCtx &c, long long block, int tid) {
     __shared__ float4 s_part[4][64];
     const int sl = tid & 63, grp = tid >> 6;
     const long long s = block * 64 + sl;
+    bool covered = false;
+    for (int l = 0; l < c.n_layers; ++l) {
+        const LayerC L = cld(c.lc, l);  // n = 0 for a layer the kernels skip
+        covered = covered || (unsigned long long)(s - L.start) < (unsigned long long)L.n;
+    }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (s < c.n_spots) {
-        for (int d = grp; d < c.nslab; d += 4) {
+    if (s < c.n_spots && covered) {
+        // eight slabs' loads in flight at a time; the sums keep slab order
+        int d = grp;
+        for (; d + 28 < c.nslab; d += 32) {
+            float4 p[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) p[u] = gld(c.partial, (long long)(d + 4 * u) * c.n_spots + s);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc.x += p[u].x;
+                acc.y += p[u].y;
+                acc.z += p[u].z;
+            }
+        }
+        for (; d < c.nslab; d += 4) {
             const float4 p = gld(c.partial, (long long)d * c.n_spots + s);
             acc.x += p.x;
             acc.y += p.y;
@@ -1182,21 +1255,16 @@ static int launch_adjoint_items(const BeamCtx *ctx, int n_beams, const long long
 }
 
 int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
-                         const long long *part_off, long long total_part, const long long *item_off32,
-                         long long total_items32, const long long *item_off64, long long total_items64,
-                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st,
-                         void *const *ev) {
+                         const long long *item_off32, long long total_items32, const long long *item_off64,
+                         long long total_items64, const long long *reduce_off, long long total_reduce, bool cull,
+                         hipStream_t st, void *const *ev) {
     int rc;
-    for (long long base = 0; base < total_bricks; base += 4 * kMaxBlocks) {
-        const long long n = total_bricks - base < 4 * kMaxBlocks ? total_bricks - base : 4 * kMaxBlocks;
-        hipLaunchKernelGGL(k_brick_summary_batch, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx, brick_off,
-                           n_beams, base, total_bricks);
+    constexpr long long kPer = 4 * SUMMARY_BRICKS;  // bricks per workgroup
+    for (long long base = 0; base < total_bricks; base += kPer * kMaxBlocks) {
+        const long long n = total_bricks - base < kPer * kMaxBlocks ? total_bricks - base : kPer * kMaxBlocks;
+        hipLaunchKernelGGL(k_brick_summary_batch, dim3((unsigned)((n + kPer - 1) / kPer)), dim3(256), 0, st, ctx,
+                           brick_off, n_beams, base, total_bricks);
         if ((rc = check_launch("k_brick_summary_batch"))) return rc;
-    }
-    if (total_part > 0) {
-        hipLaunchKernelGGL(k_zero4_batch, dim3((unsigned)((total_part + 255) / 256)), dim3(256), 0, st, ctx,
-                           part_off, n_beams, total_part);
-        if ((rc = check_launch("k_zero4_batch"))) return rc;
     }
     mark(ev, 2 * DH_TIME_ADJOINT, st);
     if ((rc = launch_adjoint_items<64>(ctx, n_beams, item_off64, total_items64, cull, st))) return rc;
@@ -1237,6 +1305,18 @@ static BeamCtx single_ctx(const dh_grid &g, const dh_beam_geom *geom, const Mode
     const BeamWs w = carve_beam(ws, grid_bricks(g), ss->n_spots, ss->n_layers, default_tune());
     fill_pencil_ctx(c, g, *geom, m, *ss, w, cutoff_sigma);
     return c;
+}
+
+extern "C" int dh_spot_params(dh_spot *spots, const int64_t *order, const float *mu, const float *xy, int32_t n,
+                              void *stream) {
+    if (n < 0 || (n > 0 && (!spots || !order))) {
+        set_error("dh_spot_params: invalid arguments (n=%d)", n);
+        return DH_ERR_ARG;
+    }
+    if (n == 0 || (!mu && !xy)) return DH_OK;
+    hipLaunchKernelGGL(k_spot_params, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, spots,
+                       reinterpret_cast<const long long *>(order), mu, xy, n);
+    return check_launch("k_spot_params");
 }
 
 extern "C" int64_t dh_pb_workspace_size(int32_t n_spots, int32_t n_layers, dh_grid g) {
@@ -1284,9 +1364,6 @@ extern "C" int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g
     c.grad_xy = grad_xy;
     int rc = prepare(st, c);
     if (rc) return rc;
-    const long long np = (long long)c.nslab * S;
-    hipLaunchKernelGGL(k_zero4, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, c.partial, np);
-    if ((rc = check_launch("k_zero4"))) return rc;
     if ((rc = launch_adjoint<false>(st, c, nullptr))) return rc;
     hipLaunchKernelGGL(k_pb_reduce, dim3((S + 63) / 64), dim3(256), 0, st, c);
     return check_launch("k_pb_reduce");

@@ -185,6 +185,17 @@ int dh_pb_count_pairs(const float *wet, const float *grad_dose, dh_grid g,
                       unsigned long long *counts,
                       void *workspace, int64_t workspace_bytes, void *stream);
 
+/* Spot weights and positions from arrays in the caller's (input) row order:
+ * for i < n, with r = order[i] (device int64, a permutation of 0..n-1:
+ * sorted position -> input row),
+ *   spots[i].mu = mu[r]; spots[i].x = xy[2r]; spots[i].y = xy[2r + 1].
+ * mu or xy may be NULL (that field is kept); energy ids are never changed.
+ * An entry with r outside [0, n) is left as it is.  One launch per call; this
+ * is the per-call AoS fill of proton_spot (dosemodule.cu:204-225) for spots
+ * already sorted on the device, so moving spots needs no host round trip. */
+int dh_spot_params(dh_spot *spots, const int64_t *order, const float *mu, const float *xy, int32_t n,
+                   void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched plans.  Many plans (each its own dose grid, 1..n beams, any spot
  * counts) computed together: every phase is one launch over the work items of
@@ -257,7 +268,7 @@ typedef struct dh_batch {
     int64_t n_bricks;          /* ray-march / forward / brick-summary items (bricks of all beams) */
     int64_t n_smooth_blocks;   /* 256-voxel smoothing blocks of all beams */
     int64_t n_spots, n_layers; /* prepared spot and layer records */
-    int64_t n_sum_blocks;      /* 256-voxel blocks of plans with several beams */
+    int64_t n_sum_blocks;      /* 1024-voxel blocks of plans with several beams */
     int64_t n_partials;        /* adjoint partial records */
     int64_t n_adj32, n_adj64;  /* adjoint workgroups of 32- and 64-spot chunks */
     int64_t n_reduce_blocks;   /* 64-spot reduction blocks */
