@@ -79,7 +79,7 @@ class dh_batch(ctypes.Structure):
                 ("n_adj32", ctypes.c_int64), ("n_adj64", ctypes.c_int64),
                 ("n_reduce_blocks", ctypes.c_int64), ("tab", ctypes.c_int64 * 16),
                 ("fwd_blocks", ctypes.c_int32), ("ray_blocks", ctypes.c_int32),
-                ("prepared", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("prepared", ctypes.c_int32), ("n_summary_rows", ctypes.c_int32),
                 ("events", ctypes.c_void_p)]
 
 

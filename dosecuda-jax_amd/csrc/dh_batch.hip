@@ -44,6 +44,7 @@ enum Table {
     T_TOUCH,        // int [n_beams][MAX_TOUCH]: brick offsets smoothing reads
     T_HEADER_END,   // first byte after the tables (uploaded at setup)
     T_FLAGS,        // not an offset: capability bits below
+    T_ROW_OFF,      // long long [n_beams]: brick-summary rows (rows of beams with spots)
     N_TABLES
 };
 static_assert(N_TABLES <= 16, "dh_batch.tab holds 16 entries");
@@ -71,12 +72,12 @@ static AdjTune tune_from(const dh_batch_tune *t) {
 struct Layout {
     std::vector<BeamCtx> ctx;
     std::vector<PlanCtx> plans;
-    std::vector<long long> brick, smooth, spot, layer, part, adj32, adj64, reduce, sum;
+    std::vector<long long> brick, smooth, spot, layer, part, adj32, adj64, reduce, sum, rows;
     std::vector<Stencil> stencil;
     std::vector<int> touch;
     long long tab[N_TABLES];
     long long n_bricks = 0, n_smooth = 0, n_spots = 0, n_layers = 0, n_sum = 0, n_part = 0, n_adj32 = 0,
-              n_adj64 = 0, n_reduce = 0;
+              n_adj64 = 0, n_reduce = 0, n_rows = 0;
     long long bytes = 0;
     int flags = 0;
 };
@@ -146,6 +147,8 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
     off += align_up((long long)n_beams * sizeof(Stencil));
     L.tab[T_TOUCH] = off;
     off += align_up((long long)n_beams * MAX_TOUCH * 4);
+    L.tab[T_ROW_OFF] = off;
+    off += align_up((long long)(n_beams > 0 ? n_beams : 1) * 8);
     L.tab[T_HEADER_END] = off;
     L.touch.assign((size_t)n_beams * MAX_TOUCH, 0);
 
@@ -218,11 +221,19 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
             if (w.cs == 32) L.n_adj32 += items; else L.n_adj64 += items;
             L.reduce.push_back(L.n_reduce);
             L.n_reduce += (S + 63) / 64;
+            // brick summary: one workgroup per row of bricks along x; a beam
+            // without spots has no adjoint work and gets none
+            L.rows.push_back(L.n_rows);
+            if (S > 0) L.n_rows += (long long)c.bg.by * c.bg.bz;
         }
         // plans whose dose is not written by their beam directly (several
         // beams, none, or a beam with a volume of its own) get the sum pass
         L.sum.push_back(L.n_sum);
         if (!direct) L.n_sum += (nvox + PLAN_SUM_VOXELS - 1) / PLAN_SUM_VOXELS;
+    }
+    if (L.n_rows >= (1LL << 31)) {
+        set_error("dh_batch: %lld brick rows, at most 2^31 - 1", L.n_rows);
+        return false;
     }
     L.bytes = off;
     L.flags = flags;
@@ -297,6 +308,7 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     put(T_SUM_OFF, D.sum.data(), D.sum.size() * 8);
     put(T_STENCIL, D.stencil.data(), D.stencil.size() * sizeof(Stencil));
     put(T_TOUCH, D.touch.data(), D.touch.size() * 4);
+    put(T_ROW_OFF, D.rows.data(), D.rows.size() * 8);
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(workspace, img.data(), img.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -319,8 +331,9 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     batch->n_adj32 = D.n_adj32;
     batch->n_adj64 = D.n_adj64;
     batch->n_reduce_blocks = D.n_reduce;
-    for (int i = 0; i < T_FLAGS; ++i) batch->tab[i] = D.tab[i];
+    for (int i = 0; i < N_TABLES; ++i) batch->tab[i] = D.tab[i];
     batch->tab[T_FLAGS] = D.flags;
+    batch->n_summary_rows = (int32_t)D.n_rows;
     batch->fwd_blocks = tune && tune->fwd_blocks > 0 ? tune->fwd_blocks : 0;
     batch->ray_blocks = tune && tune->ray_blocks > 0 ? tune->ray_blocks : 0;
     batch->magic = kMagic;
@@ -386,10 +399,10 @@ extern "C" int dh_batch_adjoint(dh_batch *b, int32_t flags, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     const int rc = prepare_records(b, flags, st, "dh_batch_adjoint");
     if (rc) return rc;
-    return launch_adjoint_batch(table<BeamCtx>(b, T_CTX), b->n_beams, table<long long>(b, T_BRICK_OFF), b->n_bricks,
-                                table<long long>(b, T_ADJ32_OFF), b->n_adj32, table<long long>(b, T_ADJ64_OFF),
-                                b->n_adj64, table<long long>(b, T_REDUCE_OFF), b->n_reduce_blocks,
-                                b->cutoff_sigma > 0.0f, st, b->events);
+    return launch_adjoint_batch(table<BeamCtx>(b, T_CTX), b->n_beams, table<long long>(b, T_ROW_OFF),
+                                b->n_summary_rows, table<long long>(b, T_ADJ32_OFF), b->n_adj32,
+                                table<long long>(b, T_ADJ64_OFF), b->n_adj64, table<long long>(b, T_REDUCE_OFF),
+                                b->n_reduce_blocks, b->cutoff_sigma > 0.0f, st, b->events);
 }
 
 extern "C" int dh_batch_timing(dh_batch *b, void *const *events) {

@@ -283,7 +283,7 @@ int launch_prep_batch(const BeamCtx *ctx, const long long *spot_off, const long 
 int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_beams, long long total_bricks,
                          const PlanCtx *plans, const long long *sum_off, int n_plans, long long total_sum_blocks,
                          bool cull, int persist_blocks, hipStream_t st, void *const *ev = nullptr);
-int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
+int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *row_off, long long total_rows,
                          const long long *item_off32, long long total_items32, const long long *item_off64,
                          long long total_items64, const long long *reduce_off, long long total_reduce, bool cull,
                          hipStream_t st, void *const *ev = nullptr);

@@ -583,53 +583,50 @@ __global__ __launch_bounds__(256) void k_brick_summary(BeamCtx c) {
     brick_summary_one(c, brick, threadIdx.x & 63);
 }
 
-// Batched: a wave takes SUMMARY_BRICKS consecutive items and issues the loads
-// of all of them before the reductions (one brick per wave leaves the loads
-// of a wave to run one after the other).  A beam without spots has no adjoint
-// work and no summary table (bsum null).
-constexpr int SUMMARY_BRICKS = 16;
-
+// Batched: one workgroup per row of bricks along x (4 voxel rows in y x 4
+// slices in z, the whole x extent) of a beam with spots.  Thread t reads voxel
+// x0 + t of the 16 voxel rows, so every load is one contiguous run of the
+// row; then the four lanes of each brick fold their ranges.  min / max are
+// exact, so the result equals the one-wave-per-brick summary's.
 __global__ __launch_bounds__(256) void k_brick_summary_batch(const BeamCtx *__restrict__ ctx,
                                                             const long long *__restrict__ off, int n_beams,
-                                                            long long base, long long total) {
-    const long long w0 = base + ((long long)blockIdx.x * 4 + wave_in_block()) * SUMMARY_BRICKS;
-    if (w0 >= total) return;  // whole wave leaves together
-    const int lane = threadIdx.x & 63;
-    int bm = find_owner(off, n_beams, w0);
-    long long next = bm + 1 < n_beams ? off[bm + 1] : total;  // first item of the next beam
-    int owner[SUMMARY_BRICKS];
-    float gv[SUMMARY_BRICKS], wv[SUMMARY_BRICKS];  // dL/dD (0 off the grid) and WET per item
+                                                            long long base) {
+    const long long row = base + blockIdx.x;
+    const int bm = find_owner(off, n_beams, row);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    const unsigned r = (unsigned)(row - off[bm]), by = (unsigned)c.bg.by;
+    const int bj = (int)(r % by), bk = (int)(r / by);
+    const int nx = c.g.nx, ny = c.g.ny, nz = c.g.nz;
+    for (int x0 = 0; x0 < BRICK * c.bg.bx; x0 += 256) {
+        const int x = x0 + (int)threadIdx.x;
+        float gv[16], wv[16];
 #pragma unroll
-    for (int q = 0; q < SUMMARY_BRICKS; ++q) {
-        const long long item = w0 + q;
-        while (item >= next && bm + 1 < n_beams) {
-            ++bm;
-            next = bm + 1 < n_beams ? off[bm + 1] : total;
+        for (int q = 0; q < 16; ++q) {
+            const int j = BRICK * bj + (q & 3), k = BRICK * bk + (q >> 2);
+            const bool ok = x < nx && j < ny && k < nz;
+            // voxel indices fit in 32 bits; an unused lane loads voxel 0
+            const unsigned idx = ok ? (unsigned)x + (unsigned)nx * ((unsigned)j + (unsigned)ny * (unsigned)k) : 0u;
+            const float g = gld(c.gdose, idx), w = gld(c.wet, idx);
+            gv[q] = ok ? g : 0.0f;
+            wv[q] = w;
         }
-        owner[q] = bm;
-        gv[q] = wv[q] = 0.0f;
-        if (item < total) {
-            const BeamCtx &c = load_ctx(ctx, bm);
-            if (c.n_spots > 0) {
-                int bi, bj, bk, i, j, k;
-                const VoxelOf vo = brick_voxel(c.g, c.bg, item - off[bm], lane, bi, bj, bk, i, j, k);
-                if (vo.valid) {
-                    gv[q] = gld(c.gdose, vo.idx);
-                    wv[q] = gld(c.wet, vo.idx);
-                }
+        float lo = 3.0e38f, hi = -3.0e38f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if (gv[q] != 0.0f) {
+                const float v = wv[q] * 10.0f;
+                lo = fminf(lo, v);
+                hi = fmaxf(hi, v);
             }
         }
-    }
-#pragma unroll
-    for (int q = 0; q < SUMMARY_BRICKS; ++q) {
-        const long long item = w0 + q;
-        if (item >= total) break;
-        const BeamCtx &c = load_ctx(ctx, owner[q]);
-        if (c.n_spots == 0) continue;
-        const bool act = gv[q] != 0.0f;
-        const float v = wv[q] * 10.0f;
-        const float lo = wave_min(act ? v : 3.0e38f), hi = wave_max(act ? v : -3.0e38f);
-        if (lane == 0) gst(c.bsum, item - off[owner[q]], make_float2(lo, hi));
+        // x0 is a multiple of 256: lanes 4m..4m+3 hold the four columns of one brick
+        lo = fminf(lo, __shfl_xor(lo, 1));
+        hi = fmaxf(hi, __shfl_xor(hi, 1));
+        lo = fminf(lo, __shfl_xor(lo, 2));
+        hi = fmaxf(hi, __shfl_xor(hi, 2));
+        const int bi = x >> 2;
+        if ((threadIdx.x & 3) == 0 && bi < c.bg.bx)
+            gst(c.bsum, bi + (long long)c.bg.bx * (bj + (long long)c.bg.by * bk), make_float2(lo, hi));
     }
 }
 
@@ -1254,16 +1251,14 @@ static int launch_adjoint_items(const BeamCtx *ctx, int n_beams, const long long
     return DH_OK;
 }
 
-int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *brick_off, long long total_bricks,
+int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *row_off, long long total_rows,
                          const long long *item_off32, long long total_items32, const long long *item_off64,
                          long long total_items64, const long long *reduce_off, long long total_reduce, bool cull,
                          hipStream_t st, void *const *ev) {
     int rc;
-    constexpr long long kPer = 4 * SUMMARY_BRICKS;  // bricks per workgroup
-    for (long long base = 0; base < total_bricks; base += kPer * kMaxBlocks) {
-        const long long n = total_bricks - base < kPer * kMaxBlocks ? total_bricks - base : kPer * kMaxBlocks;
-        hipLaunchKernelGGL(k_brick_summary_batch, dim3((unsigned)((n + kPer - 1) / kPer)), dim3(256), 0, st, ctx,
-                           brick_off, n_beams, base, total_bricks);
+    for (long long base = 0; base < total_rows; base += kMaxBlocks) {
+        const long long n = total_rows - base < kMaxBlocks ? total_rows - base : kMaxBlocks;
+        hipLaunchKernelGGL(k_brick_summary_batch, dim3((unsigned)n), dim3(256), 0, st, ctx, row_off, n_beams, base);
         if ((rc = check_launch("k_brick_summary_batch"))) return rc;
     }
     mark(ev, 2 * DH_TIME_ADJOINT, st);

@@ -265,7 +265,7 @@ typedef struct dh_batch {
     int32_t n_plans, n_beams;
     float cutoff_sigma;
     int32_t magic;
-    int64_t n_bricks;          /* ray-march / forward / brick-summary items (bricks of all beams) */
+    int64_t n_bricks;          /* ray-march / forward items (bricks of all beams) */
     int64_t n_smooth_blocks;   /* 256-voxel smoothing blocks of all beams */
     int64_t n_spots, n_layers; /* prepared spot and layer records */
     int64_t n_sum_blocks;      /* 1024-voxel blocks of plans with several beams */
@@ -276,7 +276,7 @@ typedef struct dh_batch {
                                   uploaded at setup (the rest of ws is scratch the phases rewrite) */
     int32_t fwd_blocks, ray_blocks;   /* brick-kernel schedules from dh_batch_tune */
     int32_t prepared;          /* spot / layer records prepared by a phase call (DH_BATCH_REUSE_PREP needs it) */
-    int32_t reserved;
+    int32_t n_summary_rows;    /* brick-summary items: rows of bricks along x of the beams with spots */
     void *const *events;       /* kernel timing events (dh_batch_timing), NULL = off */
 } dh_batch;
 
