@@ -14,6 +14,7 @@
 // reads the beam's context (pointers, geometry, schedule) from a device table,
 // so the launches need no host-side state beyond item counts and can be
 // captured in a HIP graph.
+#include <algorithm>
 #include <stdlib.h>
 #include <string.h>
 
@@ -43,14 +44,14 @@ enum Table {
     T_STENCIL,      // Stencil [n_beams]
     T_TOUCH,        // int [n_beams][MAX_TOUCH]: brick offsets smoothing reads
     T_HEADER_END,   // first byte after the tables (uploaded at setup)
-    T_FLAGS,        // not an offset: capability bits below
+    T_FLAGS,        // not an offset: capability bits below; bits 32+: tiled-smoothing box (floats)
     T_ROW_OFF,      // long long [n_beams]: brick-summary rows (rows of beams with spots)
     N_TABLES
 };
 static_assert(N_TABLES <= 16, "dh_batch.tab holds 16 entries");
 static_assert(T_HEADER_END == 13, "dh_batch.tab[13] is documented as the end of the uploaded tables");
 
-enum Flags { F_RLSP = 1, F_GRAD_DOSE = 2, F_GRAD_OUT = 4 };
+enum Flags { F_RLSP = 1, F_GRAD_DOSE = 2, F_GRAD_OUT = 4, F_SMOOTH_TILE = 8 };
 
 static long long bricks_of(const dh_grid &g) {
     return (long long)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
@@ -72,12 +73,13 @@ static AdjTune tune_from(const dh_batch_tune *t) {
 struct Layout {
     std::vector<BeamCtx> ctx;
     std::vector<PlanCtx> plans;
-    std::vector<long long> brick, smooth, spot, layer, part, adj32, adj64, reduce, sum, rows;
+    std::vector<long long> brick, smooth, smooth_tiles, spot, layer, part, adj32, adj64, reduce, sum, rows;
     std::vector<Stencil> stencil;
     std::vector<int> touch;
     long long tab[N_TABLES];
     long long n_bricks = 0, n_smooth = 0, n_spots = 0, n_layers = 0, n_sum = 0, n_part = 0, n_adj32 = 0,
-              n_adj64 = 0, n_reduce = 0, n_rows = 0;
+              n_adj64 = 0, n_reduce = 0, n_rows = 0, n_tiles = 0;
+    int box_max = 0;  // largest tiled-smoothing box of the batch (floats)
     long long bytes = 0;
     int flags = 0;
 };
@@ -190,6 +192,7 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
             make_stencil(B.geom, (double)P.grid.spacing, L.stencil[b]);
             c.stencil = reinterpret_cast<const Stencil *>(at(L.tab[T_STENCIL] + (long long)b * sizeof(Stencil)));
             c.n_touch = make_touch(L.stencil[b], L.touch.data() + (size_t)b * MAX_TOUCH, MAX_TOUCH);
+            L.box_max = std::max(L.box_max, smooth_box(L.stencil[b], c.slo, c.sbox));
             c.touch = reinterpret_cast<const int *>(at(L.tab[T_TOUCH] + (long long)b * MAX_TOUCH * 4));
             c.dose = B.dose;
             if (c.dose && c.dose != P.dose) direct = false;
@@ -209,6 +212,9 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
             L.n_bricks += nb;
             L.smooth.push_back(L.n_smooth);
             L.n_smooth += (nvox + 255) / 256;
+            L.smooth_tiles.push_back(L.n_tiles);
+            L.n_tiles += (long long)((P.grid.nx + STX - 1) / STX) * ((P.grid.ny + STY - 1) / STY) *
+                         ((P.grid.nz + STZ - 1) / STZ);
             L.spot.push_back(L.n_spots);
             L.n_spots += S;
             L.layer.push_back(L.n_layers);
@@ -230,6 +236,12 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
         // beams, none, or a beam with a volume of its own) get the sum pass
         L.sum.push_back(L.n_sum);
         if (!direct) L.n_sum += (nvox + PLAN_SUM_VOXELS - 1) / PLAN_SUM_VOXELS;
+    }
+    // smoothing items: tiles when every box fits the LDS budget, else blocks
+    if (L.box_max <= SMOOTH_BOX_MAX) {
+        L.smooth = L.smooth_tiles;
+        L.n_smooth = L.n_tiles;
+        flags |= F_SMOOTH_TILE;
     }
     if (L.n_rows >= (1LL << 31)) {
         set_error("dh_batch: %lld brick rows, at most 2^31 - 1", L.n_rows);
@@ -332,7 +344,7 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     batch->n_adj64 = D.n_adj64;
     batch->n_reduce_blocks = D.n_reduce;
     for (int i = 0; i < N_TABLES; ++i) batch->tab[i] = D.tab[i];
-    batch->tab[T_FLAGS] = D.flags;
+    batch->tab[T_FLAGS] = D.flags | (D.flags & F_SMOOTH_TILE ? (long long)D.box_max << 32 : 0LL);
     batch->n_summary_rows = (int32_t)D.n_rows;
     batch->fwd_blocks = tune && tune->fwd_blocks > 0 ? tune->fwd_blocks : 0;
     batch->ray_blocks = tune && tune->ray_blocks > 0 ? tune->ray_blocks : 0;
@@ -374,9 +386,10 @@ extern "C" int dh_batch_wet(dh_batch *b, int32_t flags, void *stream) {
                                             b->n_bricks, st, b->events);
         if (rc) return rc;
     }
+    const int box = (b->tab[T_FLAGS] & F_SMOOTH_TILE) ? (int)(b->tab[T_FLAGS] >> 32) : 0;
     return launch_wet_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF),
                             table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks, reach,
-                            b->ray_blocks, st, b->events);
+                            b->ray_blocks, box, st, b->events);
 }
 
 extern "C" int dh_batch_forward(dh_batch *b, int32_t flags, void *stream) {

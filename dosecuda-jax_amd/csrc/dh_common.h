@@ -198,8 +198,20 @@ struct BeamCtx {
     BrickGrid bg;
     int n_spots, n_layers, nslab, cs;
     float cut, rbrick;
-    int n_touch, pad1;
+    int n_touch;
+    int sbox[3];            // tiled smoothing: LDS box (tile plus stencil halo) per axis
+    int slo[3];             // tiled smoothing: halo below the tile per axis
 };
+
+// Tiled smoothing (batched): a workgroup smooths a 32 x 4 x 8 voxel tile
+// (8 x 1 x 2 bricks) from an LDS box holding the tile plus the stencil's
+// halo; beams whose box exceeds SMOOTH_BOX_MAX floats use the gather kernel.
+constexpr int STX = 32, STY = 4, STZ = 8;
+constexpr int SMOOTH_BOX_MAX = 16384;  // 64 KB of LDS
+
+// Smoothing halo of a stencil (voxels below / above the tile per axis) and
+// the resulting box; returns the box volume in floats.
+int smooth_box(const Stencil &st, int slo[3], int sbox[3]);
 
 // Voxels per block of the plan-sum pass (256 threads x 4).
 constexpr long long PLAN_SUM_VOXELS = 1024;
@@ -261,9 +273,11 @@ inline void mark(void *const *ev, int slot, hipStream_t st) {
 void make_stencil(const dh_beam_geom &b, double s, Stencil &st);
 int make_touch(const Stencil &st, int *out, int cap);
 bool grid_ok(const dh_grid &g);
+// block_off / total_blocks: smoothing items per beam -- 32 x 4 x 8 tiles when
+// smooth_box_floats > 0 (the largest box of the batch), else 256-voxel blocks.
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
                      long long total_bricks, long long total_blocks, bool use_need, int persist_blocks,
-                     hipStream_t st, void *const *ev = nullptr);
+                     int smooth_box_floats, hipStream_t st, void *const *ev = nullptr);
 // dh_pencil.hip
 int adjoint_chunk(int n_spots, int n_layers);
 int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t);

@@ -483,6 +483,161 @@ __global__ __launch_bounds__(256) void k_wet_smooth_batch(const BeamCtx *__restr
     gst(c.wet, t, smooth_voxel(c.wet_raw, c.g, c.stencil, t));
 }
 
+// Tiled batched smoothing: one workgroup per 32 x 4 x 8 tile of a beam.  The
+// workgroup first copies the tile and the stencil's halo (a box of the raw
+// WET, coalesced rows) into LDS, then each thread smooths four voxels with the
+// 54 neighbour reads coming from LDS.  Entries, the kept-entry test and the
+// summation order are those of smooth_voxel, so the results are the same.
+// Tiles whose bricks all lack bit 0 of the need mask only write kWetOutside.
+// A wave whose voxels keep every entry (centre WET above 0.9 cm) in a tile
+// whose box lies inside the grid takes a path without the per-entry tests.
+__global__ __launch_bounds__(256) void k_wet_smooth_tile_batch(const BeamCtx *__restrict__ ctx,
+                                                              const long long *__restrict__ toff, int n_beams,
+                                                              long long base, long long total, int use_need) {
+    extern __shared__ float box[];
+    const long long tile = base + xcd_chunk<256>(blockIdx.x, gridDim.x);
+    if (tile >= total) return;  // whole workgroup leaves together
+    const int bm = find_owner(toff, n_beams, tile);
+    const BeamCtx &c = load_ctx(ctx, bm);
+    const int nx = c.g.nx, ny = c.g.ny, nz = c.g.nz;
+    // tile coordinates (tile counts fit in 32 bits)
+    const unsigned t = (unsigned)(tile - toff[bm]);
+    const unsigned ntx = (unsigned)(nx + STX - 1) / STX, nty = (unsigned)(ny + STY - 1) / STY;
+    const unsigned tr = t / ntx;
+    const int x0 = (int)(t - tr * ntx) * STX, y0 = (int)(tr % nty) * STY, z0 = (int)(tr / nty) * STZ;
+    const int tid = threadIdx.x;
+
+    if (use_need) {
+        bool nd = false;
+        if (tid < 16) {  // the tile's 8 x 1 x 2 bricks
+            const int bi = (x0 >> 2) + (tid & 7), bj = y0 >> 2, bk = (z0 >> 2) + (tid >> 3);
+            if (bi < c.bg.bx && bk < c.bg.bz)
+                nd = (gld(c.need, bi + (long long)c.bg.bx * (bj + (long long)c.bg.by * bk)) & 1u) != 0;
+        }
+        if (!__syncthreads_or(nd)) {
+            for (int v = tid; v < STX * STY * STZ; v += 256) {
+                const int i = x0 + (v & (STX - 1)), j = y0 + ((v / STX) & (STY - 1)), k = z0 + v / (STX * STY);
+                if (i < nx && j < ny && k < nz)
+                    gst(c.wet, (unsigned)i + (unsigned)nx * ((unsigned)j + (unsigned)ny * (unsigned)k), kWetOutside);
+            }
+            return;
+        }
+    }
+
+    // box of the raw WET, by rows along x: each wave takes every fourth row and
+    // issues 16 rows' loads before storing them, so they are in flight together
+    const int BX = c.sbox[0], BY = c.sbox[1], BZ = c.sbox[2];
+    const int gx0 = x0 - c.slo[0], gy0 = y0 - c.slo[1], gz0 = z0 - c.slo[2];
+    {
+        const int wave = tid >> 6, lane = tid & 63, rows = BY * BZ;
+        for (int cx = 0; cx < BX; cx += 64) {  // one pass unless BX > 64 (spacing below 1 mm)
+            const int bx = cx + lane, gx = gx0 + bx;
+            const bool col = bx < BX && (unsigned)gx < (unsigned)nx;
+            int by = wave, bz = 0;  // row = by + BY * bz, advancing by 4 rows
+            while (by >= BY) {
+                by -= BY;
+                ++bz;
+            }
+            for (int r0 = wave; r0 < rows; r0 += 64) {
+                float vals[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int gy = gy0 + by, gz = gz0 + bz;
+                    const bool in = col && r0 + 4 * u < rows && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
+                    // voxel indices fit in 32 bits (the grid check keeps nx*ny*(nz+1) below 2^28); a
+                    // position outside the grid loads voxel 0 and stores 0 (such entries are never kept)
+                    const unsigned at = in ? (unsigned)gx + (unsigned)nx * ((unsigned)gy + (unsigned)ny * (unsigned)gz) : 0u;
+                    const float v = gld(c.wet_raw, at);
+                    vals[u] = in ? v : 0.0f;
+                    by += 4;
+                    while (by >= BY) {
+                        by -= BY;
+                        ++bz;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (bx < BX && r0 + 4 * u < rows) box[bx + BX * (r0 + 4 * u)] = vals[u];
+            }
+        }
+    }
+    __syncthreads();
+
+    // the 54 entries live in lane e of two registers (read back with
+    // readlane, so the loop below touches no memory but LDS): the entry's
+    // linear box offset with its tie bits, and its packed fields
+    const int sxy = BX * BY;
+    int fast_e = 0, gen_e = 0;
+    {
+        const int e = tid & 63;
+        if (e < N_STENCIL) {
+            const signed char *sb = reinterpret_cast<const signed char *>(c.stencil);
+            const int dx = gld(sb, e), dy = gld(sb, N_STENCIL + e), dz = gld(sb, 2 * N_STENCIL + e);
+            const int tie = (unsigned char)gld(sb, 3 * N_STENCIL + e), dr = (unsigned char)gld(sb, 4 * N_STENCIL + e);
+            fast_e = ((dx + BX * dy + sxy * dz) << 3) | tie;
+            gen_e = (dx + 64) | (dy + 64) << 7 | (dz + 64) << 14 | tie << 21 | dr << 24;
+        }
+    }
+    const bool interior = gx0 >= 0 && gy0 >= 0 && gz0 >= 0 && gx0 + BX <= nx && gy0 + BY <= ny && gz0 + BZ <= nz;
+#pragma unroll 1
+    for (int q = 0; q < STX * STY * STZ / 256; ++q) {
+        const int v = tid + 256 * q;
+        const int lx = v & (STX - 1), ly = (v / STX) & (STY - 1), lz = v / (STX * STY);
+        const int i = x0 + lx, j = y0 + ly, k = z0 + lz;
+        bool live = i < nx && j < ny && k < nz;
+        const unsigned vidx = live ? (unsigned)i + (unsigned)nx * ((unsigned)j + (unsigned)ny * (unsigned)k) : 0u;
+        if (use_need && live &&
+            !(gld(c.need, (i >> 2) + (long long)c.bg.bx * ((j >> 2) + (long long)c.bg.by * (k >> 2))) & 1u)) {
+            gst(c.wet, vidx, kWetOutside);
+            live = false;
+        }
+        const int cidx = (lx + c.slo[0]) + BX * ((ly + c.slo[1]) + BY * (lz + c.slo[2]));
+        const float wc = box[cidx];
+        const float max_dr = fminf(wc * 10.0f, 10.0f);
+        float sum = wc;
+        int cnt = 1;
+        if (interior && __all(!live || 9.0f < max_dr)) {
+            // every entry kept and inside the grid: sum in table order, 55 terms
+#pragma unroll
+            for (int e = 0; e < N_STENCIL; ++e) {
+                const int f = __builtin_amdgcn_readlane(fast_e, e);
+                int idx = cidx + (f >> 3);
+                if (f & 7) {  // wave-uniform: a half-voxel tie resolved by the neighbour's parity
+                    const int g = __builtin_amdgcn_readlane(gen_e, e);
+                    if (f & 1) idx += (i + (g & 127) - 64) & 1;
+                    if (f & 2) idx += BX * ((j + ((g >> 7) & 127) - 64) & 1);
+                    if (f & 4) idx += sxy * ((k + ((g >> 14) & 127) - 64) & 1);
+                }
+                sum += box[idx];
+            }
+            cnt = 1 + N_STENCIL;
+        } else {
+            // entries kept need dr < max_dr: no live lane of the wave keeps an
+            // entry with dr > dmax (wave-uniform), so those are skipped
+            int kk = live ? (int)ceilf(max_dr) - 1 : 0;
+            kk = kk < 0 ? 0 : (kk > 9 ? 9 : kk);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) kk = max(kk, __shfl_xor(kk, off));
+            const int dmax = __builtin_amdgcn_readfirstlane(kk);
+#pragma unroll
+            for (int e = 0; e < N_STENCIL; ++e) {
+                if (e % 9 + 1 > dmax) continue;  // entry e has dr = e % 9 + 1
+                const int g = __builtin_amdgcn_readlane(gen_e, e);
+                const int tie = (g >> 21) & 7;
+                const int ni = stencil_axis(i, (g & 127) - 64, tie & 1);
+                const int nj = stencil_axis(j, ((g >> 7) & 127) - 64, tie & 2);
+                const int nk = stencil_axis(k, ((g >> 14) & 127) - 64, tie & 4);
+                const bool ok = (float)((g >> 24) & 15) < max_dr && (unsigned)ni < (unsigned)nx &&
+                                (unsigned)nj < (unsigned)ny && (unsigned)nk < (unsigned)nz;
+                const float val = box[ok ? (ni - gx0) + BX * ((nj - gy0) + BY * (nk - gz0)) : cidx];
+                sum = ok ? sum + val : sum;
+                cnt += ok ? 1 : 0;
+            }
+        }
+        if (live) gst(c.wet, vidx, sum / (float)cnt);
+    }
+}
+
 // Host: tabulate the smoothing stencil of a beam.
 void make_stencil(const dh_beam_geom &b, double s, Stencil &st) {
     static const double kc[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
@@ -525,6 +680,24 @@ void make_stencil(const dh_beam_geom &b, double s, Stencil &st) {
 // floor((l + base [+ 1]) / 4).  Packed as (ox + 128) | (oy + 128) << 8 |
 // (oz + 128) << 16; returns the count (at most 54 * 8).
 static int floor4(int n) { return n >= 0 ? n / 4 : -((-n + 3) / 4); }
+
+int smooth_box(const Stencil &st, int slo[3], int sbox[3]) {
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int e = 0; e < N_STENCIL; ++e) {
+        const int base[3] = {st.dx[e], st.dy[e], st.dz[e]};
+        for (int ax = 0; ax < 3; ++ax) {
+            const int t = (st.tie[e] >> ax) & 1;  // a tie may add one voxel
+            lo[ax] = std::max(lo[ax], -base[ax]);
+            hi[ax] = std::max(hi[ax], base[ax] + t);
+        }
+    }
+    const int tile[3] = {STX, STY, STZ};
+    for (int ax = 0; ax < 3; ++ax) {
+        slo[ax] = lo[ax];
+        sbox[ax] = tile[ax] + lo[ax] + hi[ax];
+    }
+    return sbox[0] * sbox[1] * sbox[2];
+}
 
 int make_touch(const Stencil &st, int *out, int cap) {
     int n = 0;
@@ -645,7 +818,7 @@ extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid
 namespace dh {
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
                      long long total_bricks, long long total_blocks, bool use_need, int persist_blocks,
-                     hipStream_t st, void *const *ev) {
+                     int smooth_box_floats, hipStream_t st, void *const *ev) {
     // item spaces above 2^24 bricks go out in several launches
     constexpr long long kMaxItems = 1LL << 24;
     mark(ev, 2 * DH_TIME_RAYMARCH, st);
@@ -667,9 +840,14 @@ int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long 
     mark(ev, 2 * DH_TIME_SMOOTH, st);
     for (long long base = 0; base < total_blocks; base += kMaxItems / 4) {
         const long long n = total_blocks - base < kMaxItems / 4 ? total_blocks - base : kMaxItems / 4;
-        hipLaunchKernelGGL(k_wet_smooth_batch, dim3((unsigned)n), dim3(256), 0, st, ctx, block_off, n_beams, base,
-                           total_blocks, use_need ? 1 : 0);
-        const int rc = check_launch("k_wet_smooth_batch");
+        if (smooth_box_floats > 0)
+            hipLaunchKernelGGL(k_wet_smooth_tile_batch, dim3((unsigned)n), dim3(256),
+                               (unsigned)smooth_box_floats * sizeof(float), st, ctx, block_off, n_beams, base,
+                               total_blocks, use_need ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_wet_smooth_batch, dim3((unsigned)n), dim3(256), 0, st, ctx, block_off, n_beams,
+                               base, total_blocks, use_need ? 1 : 0);
+        const int rc = check_launch(smooth_box_floats > 0 ? "k_wet_smooth_tile_batch" : "k_wet_smooth_batch");
         if (rc) return rc;
     }
     mark(ev, 2 * DH_TIME_SMOOTH + 1, st);

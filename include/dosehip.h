@@ -266,7 +266,8 @@ typedef struct dh_batch {
     float cutoff_sigma;
     int32_t magic;
     int64_t n_bricks;          /* ray-march / forward items (bricks of all beams) */
-    int64_t n_smooth_blocks;   /* 256-voxel smoothing blocks of all beams */
+    int64_t n_smooth_blocks;   /* smoothing items of all beams: 32x4x8-voxel tiles, or 256-voxel blocks
+                                  when a beam's stencil halo is too wide for the tiled kernel */
     int64_t n_spots, n_layers; /* prepared spot and layer records */
     int64_t n_sum_blocks;      /* 1024-voxel blocks of plans with several beams */
     int64_t n_partials;        /* adjoint partial records */
