@@ -483,14 +483,31 @@ __global__ __launch_bounds__(256) void k_wet_smooth_batch(const BeamCtx *__restr
     gst(c.wet, t, smooth_voxel(c.wet_raw, c.g, c.stencil, t));
 }
 
+// Box index of stencil entry e for the voxel (i, j, k) at box index cidx:
+// the entry's linear offset (lane e of fast_e, with its tie bits) plus, for a
+// half-voxel tie, the neighbour's parity (stencil_axis).
+__device__ __forceinline__ int tile_box_index(int fast_e, int gen_e, int e, int cidx, int i, int j, int k, int BX,
+                                              int sxy) {
+    const int f = __builtin_amdgcn_readlane(fast_e, e);
+    int idx = cidx + (f >> 3);
+    if (f & 7) {  // wave-uniform
+        const int g = __builtin_amdgcn_readlane(gen_e, e);
+        if (f & 1) idx += (i + (g & 127) - 64) & 1;
+        if (f & 2) idx += BX * ((j + ((g >> 7) & 127) - 64) & 1);
+        if (f & 4) idx += sxy * ((k + ((g >> 14) & 127) - 64) & 1);
+    }
+    return idx;
+}
+
 // Tiled batched smoothing: one workgroup per 32 x 4 x 8 tile of a beam.  The
 // workgroup first copies the tile and the stencil's halo (a box of the raw
 // WET, coalesced rows) into LDS, then each thread smooths four voxels with the
 // 54 neighbour reads coming from LDS.  Entries, the kept-entry test and the
 // summation order are those of smooth_voxel, so the results are the same.
 // Tiles whose bricks all lack bit 0 of the need mask only write kWetOutside.
-// A wave whose voxels keep every entry (centre WET above 0.9 cm) in a tile
-// whose box lies inside the grid takes a path without the per-entry tests.
+// A wave whose voxels have all their neighbours inside the grid skips the
+// bounds tests, and one whose voxels also keep every entry (centre WET above
+// 0.9 cm) skips the radius test too.
 __global__ __launch_bounds__(256) void k_wet_smooth_tile_batch(const BeamCtx *__restrict__ ctx,
                                                               const long long *__restrict__ toff, int n_beams,
                                                               long long base, long long total, int use_need) {
@@ -578,7 +595,9 @@ __global__ __launch_bounds__(256) void k_wet_smooth_tile_batch(const BeamCtx *__
             gen_e = (dx + 64) | (dy + 64) << 7 | (dz + 64) << 14 | tie << 21 | dr << 24;
         }
     }
-    const bool interior = gx0 >= 0 && gy0 >= 0 && gz0 >= 0 && gx0 + BX <= nx && gy0 + BY <= ny && gz0 + BZ <= nz;
+    // halo above the tile per axis: a voxel at least slo below / shi above
+    // the grid's edges has every stencil neighbour inside the grid
+    const int shx = BX - STX - c.slo[0], shy = BY - STY - c.slo[1], shz = BZ - STZ - c.slo[2];
 #pragma unroll 1
     for (int q = 0; q < STX * STY * STZ / 256; ++q) {
         const int v = tid + 256 * q;
@@ -591,47 +610,52 @@ __global__ __launch_bounds__(256) void k_wet_smooth_tile_batch(const BeamCtx *__
             gst(c.wet, vidx, kWetOutside);
             live = false;
         }
+        const bool safe = i >= c.slo[0] && i + shx < nx && j >= c.slo[1] && j + shy < ny && k >= c.slo[2] &&
+                          k + shz < nz;
         const int cidx = (lx + c.slo[0]) + BX * ((ly + c.slo[1]) + BY * (lz + c.slo[2]));
         const float wc = box[cidx];
         const float max_dr = fminf(wc * 10.0f, 10.0f);
         float sum = wc;
         int cnt = 1;
-        if (interior && __all(!live || 9.0f < max_dr)) {
-            // every entry kept and inside the grid: sum in table order, 55 terms
+        // entry e has dr = e % 9 + 1 (direction major) and is kept where dr <
+        // max_dr and its neighbour is inside the grid
+        if (__all(!live || (safe && 9.0f < max_dr))) {
+            // every entry kept: sum in table order, 55 terms
 #pragma unroll
-            for (int e = 0; e < N_STENCIL; ++e) {
-                const int f = __builtin_amdgcn_readlane(fast_e, e);
-                int idx = cidx + (f >> 3);
-                if (f & 7) {  // wave-uniform: a half-voxel tie resolved by the neighbour's parity
-                    const int g = __builtin_amdgcn_readlane(gen_e, e);
-                    if (f & 1) idx += (i + (g & 127) - 64) & 1;
-                    if (f & 2) idx += BX * ((j + ((g >> 7) & 127) - 64) & 1);
-                    if (f & 4) idx += sxy * ((k + ((g >> 14) & 127) - 64) & 1);
-                }
-                sum += box[idx];
-            }
+            for (int e = 0; e < N_STENCIL; ++e) sum += box[tile_box_index(fast_e, gen_e, e, cidx, i, j, k, BX, sxy)];
             cnt = 1 + N_STENCIL;
         } else {
-            // entries kept need dr < max_dr: no live lane of the wave keeps an
-            // entry with dr > dmax (wave-uniform), so those are skipped
+            // no live lane of the wave keeps an entry with dr > dmax (wave-uniform)
             int kk = live ? (int)ceilf(max_dr) - 1 : 0;
             kk = kk < 0 ? 0 : (kk > 9 ? 9 : kk);
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) kk = max(kk, __shfl_xor(kk, off));
             const int dmax = __builtin_amdgcn_readfirstlane(kk);
+            if (__all(!live || safe)) {
+                // every neighbour inside the grid: only the radius test
 #pragma unroll
-            for (int e = 0; e < N_STENCIL; ++e) {
-                if (e % 9 + 1 > dmax) continue;  // entry e has dr = e % 9 + 1
-                const int g = __builtin_amdgcn_readlane(gen_e, e);
-                const int tie = (g >> 21) & 7;
-                const int ni = stencil_axis(i, (g & 127) - 64, tie & 1);
-                const int nj = stencil_axis(j, ((g >> 7) & 127) - 64, tie & 2);
-                const int nk = stencil_axis(k, ((g >> 14) & 127) - 64, tie & 4);
-                const bool ok = (float)((g >> 24) & 15) < max_dr && (unsigned)ni < (unsigned)nx &&
-                                (unsigned)nj < (unsigned)ny && (unsigned)nk < (unsigned)nz;
-                const float val = box[ok ? (ni - gx0) + BX * ((nj - gy0) + BY * (nk - gz0)) : cidx];
-                sum = ok ? sum + val : sum;
-                cnt += ok ? 1 : 0;
+                for (int e = 0; e < N_STENCIL; ++e) {
+                    if (e % 9 + 1 > dmax) continue;
+                    const float val = box[tile_box_index(fast_e, gen_e, e, cidx, i, j, k, BX, sxy)];
+                    const bool ok = (float)(e % 9 + 1) < max_dr;
+                    sum = ok ? sum + val : sum;
+                    cnt += ok ? 1 : 0;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < N_STENCIL; ++e) {
+                    if (e % 9 + 1 > dmax) continue;
+                    const int g = __builtin_amdgcn_readlane(gen_e, e);
+                    const int tie = (g >> 21) & 7;
+                    const int ni = stencil_axis(i, (g & 127) - 64, tie & 1);
+                    const int nj = stencil_axis(j, ((g >> 7) & 127) - 64, tie & 2);
+                    const int nk = stencil_axis(k, ((g >> 14) & 127) - 64, tie & 4);
+                    const bool ok = (float)(e % 9 + 1) < max_dr && (unsigned)ni < (unsigned)nx &&
+                                    (unsigned)nj < (unsigned)ny && (unsigned)nk < (unsigned)nz;
+                    const float val = box[ok ? (ni - gx0) + BX * ((nj - gy0) + BY * (nk - gz0)) : cidx];
+                    sum = ok ? sum + val : sum;
+                    cnt += ok ? 1 : 0;
+                }
             }
         }
         if (live) gst(c.wet, vidx, sum / (float)cnt);
