@@ -46,6 +46,22 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 // natural log through v_log_f32 (log2)
 __device__ __forceinline__ float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
 
+// Largest value over the wave of a non-negative float, wave-uniform.  The bit
+// patterns of non-negative floats order like integers, so an integer max is
+// exact; the reduction runs on DPP lane moves (quad swaps, the row mirrors,
+// then the row broadcasts), which leaves the maximum in lane 63.  Needs all 64
+// lanes active.
+__device__ __forceinline__ float wave_max_nonneg(float x) {
+    int v = __float_as_int(x);
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false));  // row_bcast15 -> rows 1, 3
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false));  // row_bcast31 -> rows 2, 3
+    return __int_as_float(__builtin_amdgcn_readlane(v, 63));
+}
+
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
@@ -396,8 +412,9 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
             const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
             float rcH2 = 0.0f, rcP2 = 0.0f;
             if (CULL) {
-                const float rh = fmaf(c.cut, wave_max(act ? T.sh : 0.0f), c.rbrick);
-                const float rp = fmaf(c.cut, wave_max(act ? T.st : 0.0f), c.rbrick);
+                // sigmas are non-negative (sh a square root, st 0 unless positive)
+                const float rh = fmaf(c.cut, wave_max_nonneg(act ? T.sh : 0.0f), c.rbrick);
+                const float rp = fmaf(c.cut, wave_max_nonneg(act ? T.st : 0.0f), c.rbrick);
                 rcH2 = rh * rh;
                 rcP2 = rp * rp;
             }
@@ -826,7 +843,7 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
 
             bool rel = sval;
             if (CULL) {
-                const float rc = fmaf(c.cut, wave_max(act ? T.sh : 0.0f), c.rbrick);
+                const float rc = fmaf(c.cut, wave_max_nonneg(act ? T.sh : 0.0f), c.rbrick);
                 const float3 hc = image_to_head(c.b, brick_centre(c.g, c.b, bi, bj, bk));
                 rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
             }
