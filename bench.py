@@ -212,8 +212,15 @@ def run_uneven(args):
             return doses, bd.adjoint(None, reuse_prep=True, input_order=False)
         return step
 
+    def tune_grad(doses):
+        out = [torch.empty_like(d) for d in doses]
+        for k, (d, g) in enumerate(zip(doses, out)):
+            grad_into(d, g, masks[k], counts[k][0], counts[k][1])
+        return out
+
     t0 = time.perf_counter()
-    bd = BatchedDose(engines, wet_mode="reach", tune=None if args.no_tune else "auto")
+    bd = BatchedDose(engines, wet_mode="reach", tune=None if args.no_tune else "auto", tune_grad_fn=tune_grad,
+                     tune_grad_key="uneven-loss")
     t_setup = time.perf_counter() - t0
     log("batch items %s, schedule %s, set-up %.1f s" % (bd.item_counts(), tune_dict(bd.tune), t_setup))
     ms_batch, gs = _graph_ms(make_step(bd, list(range(n_plans))), dev, args.warmup, args.steps)
@@ -435,7 +442,8 @@ def main():
     n_spots = sum(b.spots.n_spots for b in engines[0].beams)
 
     t_tune = time.perf_counter()
-    bd = BatchedDose(engines, wet_mode="reach", tune=None if args.no_tune else "auto")
+    bd = BatchedDose(engines, wet_mode="reach", tune=None if args.no_tune else "auto",
+                     tune_grad_fn=lambda doses: [lgrad(d) for d in doses], tune_grad_key="head-loss")
     t_tune = time.perf_counter() - t_tune
     from DoseCUDA.batch import tune_dict
     log("schedule %s (set-up incl. autotuning %.1f s)" % (tune_dict(bd.tune), t_tune))

@@ -57,7 +57,7 @@ def _bucket(x):
     return int(round(math.log2(max(1.0, float(x))) * 2))   # half-octave buckets
 
 
-def signature(engines, cutoff_sigma, wet_mode):
+def signature(engines, cutoff_sigma, wet_mode, grad="synthetic"):
     dev = engines[0].device
     arch = torch.cuda.get_device_properties(dev).gcnArchName.split(":")[0]
     beams = [b for e in engines for b in e.beams]
@@ -67,7 +67,7 @@ def signature(engines, cutoff_sigma, wet_mode):
     layers = sum(b.spots.n_layers for b in beams if b.spots.n_spots)
     return {"arch": arch, "beams": len(beams), "bricks": _bucket(bricks), "spots": _bucket(spots),
             "spots_per_layer": _bucket(spots / max(layers, 1)), "cull": bool(cutoff_sigma > 0),
-            "wet": wet_mode}
+            "wet": wet_mode, "grad": grad}
 
 
 def _key(sig):
@@ -119,8 +119,11 @@ def _phase_ms(fn, device, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=3, grad_doses=None, log=None):
-    """Search the schedules for this batch; returns (tune dict, timings)."""
+def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=3, grad_doses=None, log=None, grad_fn=None):
+    """Search the schedules for this batch; returns (tune dict, timings).
+    The adjoint candidates run on dL/dD = grad_doses, else grad_fn(plan doses)
+    (the caller's loss gradient), else a fixed pseudo-random field over the
+    voxels with dose."""
     from .batch import BatchedDose, make_tune
     cutoff = float(engines[0].cutoff_sigma if cutoff_sigma is None else cutoff_sigma)
     say = log or (lambda m: None)
@@ -149,6 +152,8 @@ def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=3, grad_doses=None, 
     # the adjoint needs dL/dD: a fixed pseudo-random field over the voxels with dose
     ray()
     fwd()
+    if grad_doses is None and grad_fn is not None:
+        grad_doses = [g.detach().clone() for g in grad_fn([d.clone() for d in bd.doses])]
     if grad_doses is None:
         gen = torch.Generator(device=dev)
         gen.manual_seed(1234)
@@ -180,13 +185,18 @@ def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=3, grad_doses=None, 
     return best, timings
 
 
-def get_tune(engines, cutoff_sigma=None, wet_mode="reach", path=None, log=None, retune=False):
-    """Cached schedule for this batch shape, tuning it on a cache miss."""
+def get_tune(engines, cutoff_sigma=None, wet_mode="reach", path=None, log=None, retune=False, grad_fn=None,
+             grad_key="synthetic"):
+    """Cached schedule for this batch shape, tuning it on a cache miss.
+    grad_fn: the caller's dL/dD as a function of the plan doses (adjoint
+    candidates are timed on it); grad_key names it in the cache key."""
     from .batch import make_tune
     cutoff = float(engines[0].cutoff_sigma if cutoff_sigma is None else cutoff_sigma)
-    sig = signature(engines, cutoff, wet_mode)
+    if grad_fn is not None and grad_key == "synthetic":
+        grad_key = "given"
+    sig = signature(engines, cutoff, wet_mode, grad_key)
     t = None if retune else lookup(sig, path)
     if t is None:
-        t, timings = tune(engines, cutoff, wet_mode, log=log)
+        t, timings = tune(engines, cutoff, wet_mode, log=log, grad_fn=grad_fn)
         store(sig, t, timings, path)
     return make_tune(**t), t

@@ -58,10 +58,15 @@ class BatchedDose:
     multi-beam plans write workspace scratch.
     tune: dh_batch_tune, None for the defaults, or "auto" for the autotuner's
     schedule for this batch shape (DoseCUDA.autotune; tuned on a cache miss).
+    tune_grad_fn: with tune="auto", the caller's dL/dD as a function of the
+    list of plan doses, so the adjoint schedule is tuned on the gradient field
+    the caller will use (default: a pseudo-random field where there is dose);
+    tune_grad_key names that function in the tuning cache.
     wet_mode: "full" or "reach" (see the module notes).
     """
 
-    def __init__(self, engines, cutoff_sigma=None, tune=None, keep_beam_doses=False, wet_mode="full"):
+    def __init__(self, engines, cutoff_sigma=None, tune=None, keep_beam_doses=False, wet_mode="full",
+                 tune_grad_fn=None, tune_grad_key="synthetic"):
         if wet_mode not in ("full", "reach"):
             raise ValueError("wet_mode must be 'full' or 'reach'")
         self.wet_mode = wet_mode
@@ -78,7 +83,8 @@ class BatchedDose:
             if tune != "auto":
                 raise ValueError("tune must be a dh_batch_tune, None or 'auto'")
             from .autotune import get_tune
-            tune, _ = get_tune(self.engines, self.cutoff_sigma, wet_mode)
+            tune, _ = get_tune(self.engines, self.cutoff_sigma, wet_mode, grad_fn=tune_grad_fn,
+                               grad_key=tune_grad_key)
         self.tune = tune if tune is not None else make_tune()
         dev = self.device
         self.doses, self.grad_doses, self.beam_doses = [], [], []
