@@ -868,11 +868,15 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
                 e.pad0 = e.pad1 = e.pad2 = 0.0f;
                 s_vox[pos] = e;
             }
+            // pad the list to whole rounds of G voxels with zero voxels: every
+            // term of a zero voxel is an exact +0 (GA = GB = 0, e = exp2(0)),
+            // so the sums are unchanged and all lanes run the same trip count
+            const int na = __popcll(am), nit = (na + G - 1) / G;
+            if (G > 1 && lane >= na && lane < nit * G) s_vox[lane] = LdsVox{};
             __syncthreads();
-            const int na = __popcll(am);
             if (rel) {
-                for (int t = group; t < na; t += G) {
-                    const LdsVox e = s_vox[t];
+                for (int it = 0; it < nit; ++it) {
+                    const LdsVox e = s_vox[it * G + group];
                     // in-plane offset e_xy, its part perpendicular to the axis p,
                     // q_e = (e.t)/|t|^2; d2 = e_xy . p (no cancellation)
                     const float u = fmaf(e.hz, q.tx, e.hx - q.x);
