@@ -388,7 +388,8 @@ def main():
     ap.add_argument("--plans", type=int, default=4, help="head-phantom plans per batch (K)")
     ap.add_argument("--cutoff-sigma", type=float, default=6.0)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain (dense, per-beam) path; 0 = skip")
-    ap.add_argument("--cpu-sample", type=int, default=60000, help="voxels in the CPU-baseline sample; 0 = skip")
+    ap.add_argument("--cpu-sample", type=int, default=800000,
+                    help="voxels in the CPU-baseline sample (about 10-15 s of CPU work on 16 cores); 0 = skip")
     ap.add_argument("--no-tune", action="store_true", help="library default schedules instead of the autotuner's")
     ap.add_argument("--workload", choices=("head", "uneven", "body"), default="head",
                     help="head: the headline metric (default); uneven: 64 uneven plans batched; body: 512x512x300 "

@@ -85,7 +85,7 @@ class dh_batch(ctypes.Structure):
 
 DH_TIME_RAYMARCH, DH_TIME_SMOOTH, DH_TIME_FORWARD, DH_TIME_ADJOINT, DH_TIME_NEED = range(5)
 DH_TIME_N = 5
-TIMED_KERNELS = {DH_TIME_RAYMARCH: "k_wet_raymarch_batch", DH_TIME_SMOOTH: "k_wet_smooth_batch",
+TIMED_KERNELS = {DH_TIME_RAYMARCH: "k_wet_raymarch_batch", DH_TIME_SMOOTH: "k_wet_smooth_tile_batch",
                  DH_TIME_FORWARD: "k_pb_forward_batch", DH_TIME_ADJOINT: "k_pb_adjoint_batch",
                  DH_TIME_NEED: "k_wet_need_batch+k_wet_dilate_batch"}
 

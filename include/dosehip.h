@@ -321,7 +321,7 @@ int dh_batch_wet(dh_batch *batch, int32_t flags, void *stream);
  * are recorded on the stream right before and right after the launches of
  * timed kernel k.  events == NULL turns timing off. */
 #define DH_TIME_RAYMARCH 0  /* k_wet_raymarch_batch */
-#define DH_TIME_SMOOTH   1  /* k_wet_smooth_batch */
+#define DH_TIME_SMOOTH   1  /* k_wet_smooth_tile_batch (k_wet_smooth_batch for wide halos) */
 #define DH_TIME_FORWARD  2  /* k_pb_forward_batch */
 #define DH_TIME_ADJOINT  3  /* k_pb_adjoint_batch (both chunk sizes) */
 #define DH_TIME_NEED     4  /* k_wet_need_batch + k_wet_dilate_batch */
