@@ -91,6 +91,8 @@ TIMED_KERNELS = {DH_TIME_RAYMARCH: "k_wet_raymarch_batch", DH_TIME_SMOOTH: "k_we
 
 
 DH_BATCH_TAB_HEADER_END = 13   # dh_batch.tab index: end of the tables uploaded at setup
+DH_BATCH_TAB_FLAGS = 14        # dh_batch.tab index: capability bits (not an offset)
+DH_BATCH_FLAG_SMOOTH_TILE = 8  # ... bit: smoothing runs the tiled (LDS) kernel
 DH_BATCH_REUSE_PREP = 1
 DH_BATCH_WET_REACH = 2
 

@@ -219,6 +219,12 @@ class BatchedDose:
                 k += 1
         return out
 
+    @property
+    def smooth_tiled(self):
+        """True when this batch smooths WET with the tiled (LDS) kernel; a beam
+        whose stencil halo box exceeds 64 KB puts the batch on the gather kernel."""
+        return bool(int(self.batch.tab[N.DH_BATCH_TAB_FLAGS]) & N.DH_BATCH_FLAG_SMOOTH_TILE)
+
     def poison_scratch(self):
         """Fill the workspace scratch (everything after the set-up tables) with
         NaN bit patterns: a phase that read scratch it had not written would

@@ -274,7 +274,9 @@ typedef struct dh_batch {
     int64_t n_adj32, n_adj64;  /* adjoint workgroups of 32- and 64-spot chunks */
     int64_t n_reduce_blocks;   /* 64-spot reduction blocks */
     int64_t tab[16];           /* byte offsets of the device tables in ws; tab[13] = end of the tables
-                                  uploaded at setup (the rest of ws is scratch the phases rewrite) */
+                                  uploaded at setup (the rest of ws is scratch the phases rewrite);
+                                  tab[14] = flags, not an offset: bit 3 = tiled smoothing, whose LDS
+                                  box (floats) is in bits 32 and up */
     int32_t fwd_blocks, ray_blocks;   /* brick-kernel schedules from dh_batch_tune */
     int32_t prepared;          /* spot / layer records prepared by a phase call (DH_BATCH_REUSE_PREP needs it) */
     int32_t n_summary_rows;    /* brick-summary items: rows of bricks along x of the beams with spots */

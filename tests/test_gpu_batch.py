@@ -115,6 +115,38 @@ def test_batch_matches_single_path(dev, cutoff):
     assert float(got[1].max()) > 0.0
 
 
+@pytest.mark.parametrize("wet_mode", ["full", "reach"])
+def test_batch_smoothing_kernels(dev, wet_mode):
+    """Both batched smoothing kernels equal the single-beam smoothing bit for
+    bit: the tiled (LDS) kernel on 2-2.5 mm grids, the gather kernel for a
+    batch with a 1 mm grid and an oblique beam (halo box above 64 KB)."""
+    from DoseCUDA.batch import BatchedDose
+    from DoseCUDA.engine import IMPTDoseEngine
+    _say("smoothing kernels, %s WET" % wet_mode)
+    g = _phantom((20, 40, 36), 1.0, 11)
+    wide = [IMPTDoseEngine(g, _plan([120.0, 0.0], [30.0, 0.0], [20, 24], radius=6.0, seed=3), cutoff_sigma=6.0)]
+    for engines, tiled in ((wide, False), (_engines(dev)[:3], True)):
+        want = []
+        for e in engines:
+            e.compute_wet(force=True)
+            want.append([b.wet.clone() for b in e.beams])
+        for e in engines:
+            for b in e.beams:
+                b.wet.fill_(float("nan"))
+        bd = BatchedDose(engines, wet_mode=wet_mode)
+        assert bd.smooth_tiled == tiled
+        if wet_mode == "full":
+            bd.compute_wet(force=True)
+        else:
+            bd.forward()
+        torch.cuda.synchronize()
+        for e, ws in zip(engines, want):
+            for b, w in zip(e.beams, ws):
+                keep = ~(b.wet >= 1e29) if wet_mode == "reach" else torch.ones_like(w, dtype=torch.bool)
+                assert torch.equal(b.wet[keep], w[keep]), "tiled=%s" % tiled
+                assert int(keep.sum()) > 0
+
+
 def test_batch_beam_doses(dev):
     from DoseCUDA.batch import BatchedDose
     _say("per-beam dose volumes")
