@@ -877,20 +877,22 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
             if (rel) {
                 for (int it = 0; it < nit; ++it) {
                     const LdsVox e = s_vox[it * G + group];
-                    // in-plane offset e_xy, its part perpendicular to the axis p,
-                    // q_e = (e.t)/|t|^2; d2 = e_xy . p (no cancellation)
-                    const float u = fmaf(e.hz, q.tx, e.hx - q.x);
-                    const float w = fmaf(e.hz, q.ty, e.hy - q.y);
-                    const float qe = fmaf(u, ttx, w * tty);
-                    const float px = fmaf(-qe, q.tx, u), py = fmaf(-qe, q.ty, w);
+                    // r = (a, b, hz) the voxel relative to the spot, (u, w) its
+                    // in-plane offset e_xy = (a, b) + hz t_xy; q = (t.r)/|t|^2
+                    // = (e_xy . t_xy)/|t|^2 - hz and p = (a, b) - q t_xy, the
+                    // part of e_xy perpendicular to the axis; d2 = e_xy . p
+                    // (no cancellation)
+                    const float a = e.hx - q.x, b = e.hy - q.y;
+                    const float u = fmaf(e.hz, q.tx, a), w = fmaf(e.hz, q.ty, b);
+                    const float qr = fmaf(u, ttx, fmaf(w, tty, -e.hz));
+                    const float px = fmaf(-qr, q.tx, a), py = fmaf(-qr, q.ty, b);
                     const float d2 = fmaf(u, px, w * py);
                     const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
                     const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
                     K = fmaf(e.GA, e1, K);
                     K = fmaf(e.GB, e2, K);
                     const float F = fmaf(e.GAa, e1, e.GBb * e2);
-                    // q of the ray r = e - hz t is qe - hz
-                    const float Fq = F * (qe - e.hz);
+                    const float Fq = F * qr;
                     FX0 = fmaf(F, px, FX0);
                     FX1 = fmaf(Fq, px, FX1);
                     FY0 = fmaf(F, py, FY0);
