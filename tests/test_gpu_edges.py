@@ -8,8 +8,9 @@
   zero dose and zero gradients, like the oracle;
 * spots far outside the volume: dose equals the oracle's (about zero) and
   the gradients are finite;
-* a batch whose plans include a beam without spots: WET, dose and gradients
-  bit-identical to the single-beam path.
+* a batch whose plans include a beam without spots, and a batch of the
+  degenerate grids: WET, dose and gradients bit-identical to the single-beam
+  path.
 """
 
 import numpy as np
@@ -66,6 +67,42 @@ def test_degenerate_grids(dev, machine_dir, shape, cutoff):
     _say("  wet %.2e cm, dose err %.2e (D_max %.3g)" % (d_wet, err, dmax))
     assert d_wet <= 2e-4
     assert err <= max(1e-3 * dmax, 1e-6)
+
+
+@pytest.mark.parametrize("wet_mode", ["full", "reach"])
+def test_batch_degenerate_grids(dev, wet_mode):
+    """A batch of the degenerate grids (one plan each, oblique beams): WET,
+    dose and gradients bit-identical to the single-beam path."""
+    from DoseCUDA.batch import BatchedDose
+    from DoseCUDA.engine import IMPTDoseEngine
+    _say("batch of degenerate grids, %s WET" % wet_mode)
+    shapes = [(1, 1, 1), (1, 6, 3), (2, 2, 2), (5, 1, 7), (3, 9, 1), (6, 5, 4), (9, 33, 37)]
+    engines = []
+    for n, shape in enumerate(shapes):
+        g, plan = _grid_plan(shape, 3.0, _spots([0, 5, 10]), gantry=30.0 + 40.0 * n, couch=20.0, seed=n)
+        engines.append(IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=6.0))
+    rng = np.random.default_rng(2)
+    grads = [torch.from_numpy(rng.normal(size=e.shape).astype(np.float32)).to(dev) for e in engines]
+    want = []
+    for e, gd in zip(engines, grads):
+        e.compute_wet(force=True)
+        want.append(([b.wet.clone() for b in e.beams], e.dose().clone(),
+                     [(m.clone(), x.clone()) for m, x in e.adjoint(gd)]))
+    bd = BatchedDose(engines, wet_mode=wet_mode)
+    if wet_mode == "full":
+        bd.compute_wet(force=True)
+    doses = bd.forward()
+    grads_b = bd.adjoint(grads)
+    torch.cuda.synchronize()
+    k = 0
+    for p, e in enumerate(engines):
+        wets, dose, adj = want[p]
+        assert torch.equal(doses[p], dose), "dose plan %d" % p
+        for j, b in enumerate(e.beams):
+            keep = ~(b.wet >= 1e29) if wet_mode == "reach" else torch.ones_like(b.wet, dtype=torch.bool)
+            assert torch.equal(b.wet[keep], wets[j][keep]), "WET plan %d" % p
+            assert torch.equal(grads_b[k][0], adj[j][0]) and torch.equal(grads_b[k][1], adj[j][1]), "grad plan %d" % p
+            k += 1
 
 
 @pytest.mark.parametrize("ids", [[98, 120], [-1, 3, 7]])
