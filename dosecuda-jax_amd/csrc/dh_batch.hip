@@ -4,12 +4,15 @@
 // (plan_impt.py:170-206, proton_spot_cuda IMPTClasses.cu:301-340).  Here a
 // batch of plans with uneven grids and spot counts runs each phase as one
 // launch over the items of every beam:
-//   wet      ray-march bricks of all beams, then smoothing blocks of all beams
+//   wet      (reach mode: the need mask and its dilation, one lane per brick)
+//            ray-march bricks of all beams, then smoothing tiles of all beams
+//            (256-voxel blocks when a stencil halo is too wide for LDS tiles)
 //   forward  spot records, layer records, dose bricks of all beams, then the
 //            fixed-order sum of the beam doses where beams write volumes of
 //            their own (plans with several beams)
-//   adjoint  brick summaries, partial zeroing, (chunk, slab) workgroups of all
-//            beams (one launch per chunk size), per-spot reduction
+//   adjoint  brick summaries (one workgroup per row of bricks), (chunk, slab)
+//            workgroups of all beams (one launch per chunk size), per-spot
+//            reduction
 // A work item finds its beam by binary search in a table of item offsets and
 // reads the beam's context (pointers, geometry, schedule) from a device table,
 // so the launches need no host-side state beyond item counts and can be
@@ -32,8 +35,8 @@ static long long align_up(long long x) { return (x + kAlign - 1) / kAlign * kAli
 enum Table {
     T_CTX = 0,      // BeamCtx [n_beams]
     T_PLANS,        // PlanCtx [n_plans]
-    T_BRICK_OFF,    // long long [n_beams]: ray-march / forward / summary items
-    T_SMOOTH_OFF,   // smoothing blocks
+    T_BRICK_OFF,    // long long [n_beams]: ray-march / forward / need-mask items
+    T_SMOOTH_OFF,   // smoothing tiles (F_SMOOTH_TILE) or 256-voxel blocks
     T_SPOT_OFF,     // spot records
     T_LAYER_OFF,    // layer records
     T_PART_OFF,     // adjoint partial records per beam (bookkeeping; no kernel reads it)
