@@ -21,6 +21,7 @@ import numpy as np
 import pandas as pd
 
 from .plan import Plan, Beam, DoseGrid, VolumeObject  # noqa: F401  (re-exported names)
+from . import imageio
 from . import dose_kernels
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -194,19 +195,14 @@ class IMPTDoseGrid(DoseGrid):
         return dose_kernels.proton_spot_cuda(beam_model, rlsp_object, wet_object, beam, gpu_id)
 
     def writeWETNRRD(self, plan, wet_path, gpu_id=0):
-        """Write each beam's WET volume to ``<wet_path>_beamNN.nrrd``
-        (plan_impt.py:259-287); needs SimpleITK."""
+        """Write each beam's WET volume (cm) to ``<wet_path>_beamNN.nrrd``
+        (plan_impt.py:259-287)."""
         if not wet_path.endswith(".nrrd"):
             raise Exception("WET path must have .nrrd extension")
-        sitk = self._sitk()
         wets = self.computeIMPTWET(plan, gpu_id)
-        fw = sitk.ImageFileWriter()
         for i, wet in enumerate(wets):
-            img = sitk.GetImageFromArray(wet.voxel_data)
-            img.SetOrigin(self.origin)
-            img.SetSpacing(self.spacing)
-            fw.SetFileName(wet_path.replace(".nrrd", "_beam%02i.nrrd" % (i + 1)))
-            fw.Execute(img)
+            imageio.write_nrrd(wet_path.replace(".nrrd", "_beam%02i.nrrd" % (i + 1)),
+                               np.asarray(wet.voxel_data, dtype=np.single), self.origin, self.spacing)
 
 
 class IMPTBeam(Beam):

@@ -1,0 +1,154 @@
+"""NRRD files and linear CT resampling (DoseCUDA.imageio, DoseGrid methods).
+
+SimpleITK, which the reference uses for these (plan.py:33-41, 59-97,
+173-212), is not installed, so parity with it is unpinned.  The checks
+here pin what fixes the results: the NRRD format fields, exact round trips,
+linear interpolation reproducing linear fields exactly inside the grid, the
+[-0.5, n - 0.5) inside-buffer rule with edge clamping, and a brute-force
+eight-corner loop written the way ITK's LinearInterpolateImageFunction
+evaluates a point.
+"""
+
+import gzip
+
+import numpy as np
+import pytest
+
+from DoseCUDA import imageio
+from DoseCUDA.plan import DoseGrid
+
+
+def _header(path):
+    raw = open(path, "rb").read()
+    end = raw.find(b"\n\n")
+    return raw[:end].decode("ascii").split("\n"), raw[end + 2:]
+
+
+@pytest.mark.parametrize("encoding", ["raw", "gzip"])
+def test_nrrd_round_trip_and_header(tmp_path, encoding):
+    rng = np.random.default_rng(0)
+    vol = rng.normal(size=(5, 7, 9)).astype(np.float32)
+    p = str(tmp_path / "v.nrrd")
+    imageio.write_nrrd(p, vol, (-10.5, 3.0, 7.25), (1.5, 2.0, 2.5), encoding=encoding)
+    lines, data = _header(p)
+    assert lines[0] == "NRRD0004"
+    assert "type: float" in lines and "dimension: 3" in lines and "sizes: 9 7 5" in lines
+    assert "space directions: (1.5,0.0,0.0) (0.0,2.0,0.0) (0.0,0.0,2.5)" in lines
+    assert "space origin: (-10.5,3.0,7.25)" in lines and "encoding: %s" % encoding in lines
+    payload = gzip.decompress(data) if encoding == "gzip" else data
+    assert np.array_equal(np.frombuffer(payload, "<f4").reshape(5, 7, 9), vol)   # x fastest
+    got, origin, spacing = imageio.read_nrrd(p)
+    assert got.dtype == np.float32 and np.array_equal(got, vol)
+    assert np.array_equal(origin, [-10.5, 3.0, 7.25]) and np.array_equal(spacing, [1.5, 2.0, 2.5])
+
+
+def test_nrrd_reader_big_endian_spacings_field(tmp_path):
+    vol = np.arange(2 * 3 * 4, dtype=np.int16).reshape(2, 3, 4) - 7
+    p = tmp_path / "s.nrrd"
+    hdr = "NRRD0004\ntype: short\ndimension: 3\nsizes: 4 3 2\nspacings: 0.5 0.75 3\nendian: big\nencoding: raw\n\n"
+    p.write_bytes(hdr.encode("ascii") + vol.astype(">i2").tobytes())
+    got, origin, spacing = imageio.read_nrrd(str(p))
+    assert got.dtype == np.int16 and np.array_equal(got, vol)
+    assert np.array_equal(origin, [0, 0, 0]) and np.array_equal(spacing, [0.5, 0.75, 3.0])
+
+
+def test_nrrd_rejects_what_it_cannot_read(tmp_path):
+    p = tmp_path / "d.nrrd"
+    p.write_bytes(b"NRRD0004\ntype: float\ndimension: 3\nsizes: 2 2 2\ndata file: x.raw\nencoding: raw\n\n")
+    with pytest.raises(NotImplementedError):
+        imageio.read_nrrd(str(p))
+    q = tmp_path / "r.nrrd"
+    q.write_bytes(b"NRRD0004\ntype: float\ndimension: 3\nsizes: 2 2 2\n"
+                  b"space directions: (0,1,0) (1,0,0) (0,0,1)\nencoding: raw\n\n" + bytes(32))
+    with pytest.raises(NotImplementedError):
+        imageio.read_nrrd(str(q))
+    with pytest.raises(ValueError):
+        imageio.write_nrrd(str(tmp_path / "w.nrrd"), np.zeros((2, 2), np.float32), (0, 0, 0), (1, 1, 1))
+
+
+def _itk_point(vol, ci, default):
+    """One output voxel as ITK evaluates it: inside test on the continuous
+    index, then the 2^3 corners with neighbour indices clamped to the image."""
+    n = (vol.shape[2], vol.shape[1], vol.shape[0])
+    if any(not (-0.5 <= ci[a] < n[a] - 0.5) for a in range(3)):
+        return default
+    base = [int(np.floor(ci[a])) for a in range(3)]
+    dist = [ci[a] - base[a] for a in range(3)]
+    val = 0.0
+    for corner in range(8):
+        idx, w = [], 1.0
+        for a in range(3):
+            if corner >> a & 1:
+                idx.append(min(base[a] + 1, n[a] - 1))
+                w *= dist[a]
+            else:
+                idx.append(max(base[a], 0))
+                w *= 1.0 - dist[a]
+        val += w * float(vol[idx[2], idx[1], idx[0]])
+    return val
+
+
+def test_resample_matches_brute_force_and_edges():
+    rng = np.random.default_rng(3)
+    vol = rng.uniform(-900, 1500, size=(4, 5, 6)).astype(np.float32)
+    origin, spacing = np.array([-3.0, 2.0, 1.0]), np.array([1.0, 1.5, 2.0])
+    new_origin, new_spacing, new_size = np.array([-3.9, 1.1, -0.4]), np.array([0.7, 0.9, 1.3]), (10, 9, 7)
+    got = imageio.resample_linear(vol, origin, spacing, new_origin, new_spacing, new_size)
+    assert got.shape == (7, 9, 10) and got.dtype == np.float32
+    n_out = n_edge = 0
+    for k in range(7):
+        for j in range(9):
+            for i in range(10):
+                p = new_origin + np.array([i, j, k]) * new_spacing
+                ci = (p - origin) / spacing
+                want = _itk_point(vol, ci, -1000.0)
+                assert abs(float(got[k, j, i]) - want) <= 1e-4 * max(1.0, abs(want)), (i, j, k)
+                n_out += want == -1000.0
+                n_edge += any(c < 0 or c > nn - 1 for c, nn in zip(ci, (6, 5, 4))) and want != -1000.0
+    assert n_out > 0 and n_edge > 0       # the grid covers outside points and the clamped border band
+
+
+def test_resample_linear_field_and_identity():
+    nz, ny, nx = 6, 7, 8
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    origin, spacing = np.array([10.0, -5.0, 2.0]), np.array([2.0, 1.0, 3.0])
+    px, py, pz = origin[0] + x * spacing[0], origin[1] + y * spacing[1], origin[2] + z * spacing[2]
+    field = lambda a, b, c: 3.0 * a - 2.0 * b + 0.5 * c + 7.0   # noqa: E731
+    vol = field(px, py, pz).astype(np.float64)
+    same = imageio.resample_linear(vol, origin, spacing, origin, spacing, (nx, ny, nz))
+    assert np.array_equal(same, vol)
+    # a grid strictly inside the input: linear interpolation is exact for a linear field
+    no, ns, nsz = origin + np.array([0.3, 0.2, 0.7]), np.array([1.1, 0.45, 1.7]), (12, 12, 8)
+    got = imageio.resample_linear(vol, origin, spacing, no, ns, nsz)
+    k, j, i = np.meshgrid(np.arange(nsz[2]), np.arange(nsz[1]), np.arange(nsz[0]), indexing="ij")
+    want = field(no[0] + i * ns[0], no[1] + j * ns[1], no[2] + k * ns[2])
+    assert np.allclose(got, want, rtol=0, atol=1e-9)
+
+
+def test_dosegrid_nrrd_and_resampling(tmp_path):
+    g = DoseGrid()
+    g.createCubePhantom(size=[14, 12, 10], spacing=3.0)
+    g.HU = np.random.default_rng(1).uniform(-1000, 1000, size=(14, 12, 10)).astype(np.float32)
+    g.spacing = np.array([1.0, 1.5, 2.0])
+    ct = str(tmp_path / "ct.nrrd")
+    g.writeCTNRRD(ct)
+    h = DoseGrid()
+    h.loadCTNRRD(ct)
+    assert np.array_equal(h.HU, g.HU) and np.array_equal(h.origin, g.origin) and np.array_equal(h.spacing, g.spacing)
+    assert tuple(h.size) == (14, 12, 10)
+    h.resampleCTfromSpacing(2.0)          # sizes int(nx sx / 2), int(ny sy / 2), int(nz sz / 2)
+    assert h.HU.shape == (14, 9, 5) and h.spacing == (2.0, 2.0, 2.0) and h.HU.dtype == np.float32
+    assert np.array_equal(h.origin, g.origin)
+    h.resampleCT((1.0, 1.0, 1.0), (4, 5, 6), g.origin - 50.0)   # entirely outside the CT
+    assert h.HU.shape == (6, 5, 4) and np.all(h.HU == -1000.0)
+    with pytest.raises(Exception):
+        g.writeCTNRRD(str(tmp_path / "ct.nii"))
+    # dose, RBE 1.1 / 1.0, per-beam files
+    g.dose = np.full((14, 12, 10), 2.0, np.float32)
+    g.beam_doses = [np.full((14, 12, 10), 1.0, np.float32), np.full((14, 12, 10), 0.5, np.float32)]
+    g.writeDoseNRRD(str(tmp_path / "d.nrrd"))
+    assert np.allclose(imageio.read_nrrd(str(tmp_path / "d.nrrd"))[0], 2.2)
+    g.writeDoseNRRD(str(tmp_path / "p.nrrd"), individual_beams=True, dose_type="PHYSICAL")
+    assert np.all(imageio.read_nrrd(str(tmp_path / "p_beam02.nrrd"))[0] == 0.5)
+    with pytest.raises(Exception):
+        g.writeDoseNRRD(str(tmp_path / "x.nrrd"), dose_type="RBE")
