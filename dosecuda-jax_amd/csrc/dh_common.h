@@ -25,6 +25,7 @@ struct Geom {
     float src[3];
     float sg, cg, sc, cc;
     float vsadx, vsady;
+    float ivsadx, ivsady;   // 1 / vsad: correctly rounded on the host, as a device division rounds
 };
 
 inline Grid make_grid(const dh_grid &g) {
@@ -40,6 +41,7 @@ inline Geom make_geom(const dh_beam_geom &b) {
     for (int a = 0; a < 3; ++a) { r.iso[a] = b.iso[a]; r.src[a] = b.src[a]; }
     r.sg = b.sin_g; r.cg = b.cos_g; r.sc = b.sin_c; r.cc = b.cos_c;
     r.vsadx = b.vsadx; r.vsady = b.vsady;
+    r.ivsadx = 1.0f / b.vsadx; r.ivsady = 1.0f / b.vsady;
     return r;
 }
 
@@ -161,7 +163,7 @@ struct LayerC {
     int start, n, eid, ok;
     float r80, A, B, C;
     float xmin, xmax, ymin, ymax;
-    float lam_min, sms_max, lim, pad;
+    float lam_min, sms_max, lim, vx;  // vx: vertex -B / (2A) of sigma_air's quadratic (0 when A = 0)
 };
 
 // Smoothing neighbour stencil of a beam: per (direction, dr) entry, direction

@@ -178,10 +178,7 @@ __device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, 
     const float dlo = B.dsrc - rbrick - whi + d07;
     const float dhi = B.dsrc + rbrick - B.wlo + d07;
     float sa = fmaxf(fabsf(fmaf(fmaf(c.A, dlo, c.B), dlo, c.C)), fabsf(fmaf(fmaf(c.A, dhi, c.B), dhi, c.C)));
-    if (c.A != 0.0f) {
-        const float vx = -c.B / (2.0f * c.A);
-        if (vx > dlo && vx < dhi) sa = fmaxf(sa, fabsf(fmaf(fmaf(c.A, vx, c.B), vx, c.C)));
-    }
+    if (c.A != 0.0f && c.vx > dlo && c.vx < dhi) sa = fmaxf(sa, fabsf(fmaf(fmaf(c.A, c.vx, c.B), c.vx, c.C)));
     const float st_ub = sa + c.sms_max;
     const float sh_ub = 1.001f * fast_sqrt(st_ub * st_ub + sn_ub * sn_ub);
     return dlb > cut * sh_ub;
@@ -259,7 +256,7 @@ __device__ __forceinline__ void spot_prep_one(const dh_spot *__restrict__ in, Sp
 __global__ __launch_bounds__(256) void k_spot_prep(BeamCtx c) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= c.n_spots) return;
-    spot_prep_one(c.spots_in, c.sp, s, 1.0f / c.b.vsadx, 1.0f / c.b.vsady);
+    spot_prep_one(c.spots_in, c.sp, s, c.b.ivsadx, c.b.ivsady);
 }
 
 // One wave per layer.  A record the kernels cannot use safely (energy id
@@ -279,7 +276,7 @@ __device__ __forceinline__ void layer_prep_one(const BeamCtx &cx, int l, int lan
     c.A = d[2];
     c.B = d[3];
     c.C = d[4];
-    c.pad = 0.0f;
+    c.vx = c.A != 0.0f ? -c.B / (2.0f * c.A) : 0.0f;
     if (c.ok) {
         reach_of(cx.sp, c.start, c.n, cx.m, c.eid, lane, c);
     } else {
@@ -318,7 +315,7 @@ __global__ __launch_bounds__(256) void k_spot_prep_batch(const BeamCtx *__restri
     if (t >= total) return;
     const int bm = find_owner(off, n_beams, t);
     const BeamCtx &c = load_ctx(ctx, bm);
-    spot_prep_one(c.spots_in, c.sp, (int)(t - off[bm]), 1.0f / c.b.vsadx, 1.0f / c.b.vsady);
+    spot_prep_one(c.spots_in, c.sp, (int)(t - off[bm]), c.b.ivsadx, c.b.ivsady);
 }
 
 __global__ __launch_bounds__(64) void k_layer_prep_batch(const BeamCtx *__restrict__ ctx,
@@ -386,7 +383,7 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
     const float3 h = image_to_head(c.b, v);
     const float sx = c.b.src[0] - v.x, sy = c.b.src[1] - v.y, sz = c.b.src[2] - v.z;
     const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
-    const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
+    const float ivx = c.b.ivsadx, ivy = c.b.ivsady;
     const BrickInfo B = brick_info_at(c.b, brick_centre(c.g, c.b, bi, bj, bk), wave_min(wet),
                                       wave_max(vo.valid ? wet : -3.0e38f));
     const LayerC *__restrict__ lc = c.lc;
@@ -670,7 +667,7 @@ __device__ __forceinline__ void wet_need_lanes(const BeamCtx &c, long long brick
         int bi, bj, bk;
         brick_of(c.bg, mine ? brick : 0, bi, bj, bk);
         const float3 vc = brick_centre(c.g, c.b, bi, bj, bk);
-        const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
+        const float ivx = c.b.ivsadx, ivy = c.b.ivsady;
         for (int l = 0; l < c.n_layers; ++l) {
             if (!__ballot(!done)) break;
             const LayerC L = cld(c.lc, l);
@@ -799,7 +796,7 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
     else q = SpotExt{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 1.f};
     const float itt = 1.0f / (q.tx * q.tx + q.ty * q.ty + 1.0f);
     const float ttx = q.tx * itt, tty = q.ty * itt;  // t_xy / |t|^2
-    const float ivx = 1.0f / c.b.vsadx, ivy = 1.0f / c.b.vsady;
+    const float ivx = c.b.ivsadx, ivy = c.b.ivsady;
     const unsigned long long chunk_n = (unsigned long long)min(CS, L.n - c0);
     LayerC R = L;  // reach of this chunk's spots only
     if (CULL) reach_of(c.sp, L.start + c0, (int)chunk_n, c.m, L.eid, lane, R);
