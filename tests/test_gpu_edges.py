@@ -8,6 +8,8 @@
   zero dose and zero gradients, like the oracle;
 * spots far outside the volume: dose equals the oracle's (about zero) and
   the gradients are finite;
+* beams on the range-shifter model (negative sigma_ms, an energy whose R80 is
+  negative): dose within 1e-3 of D_max of the oracle;
 * a batch whose plans include a beam without spots, and a batch of the
   degenerate grids: WET, dose and gradients bit-identical to the single-beam
   path.
@@ -118,6 +120,36 @@ def test_beam_without_usable_layers(dev, machine_dir, ids):
     assert float(np.abs(ref).max()) == 0.0
     assert float(dose.detach().abs().max()) == 0.0
     assert float(params[0][0].grad.abs().max()) == 0.0 and float(params[0][1].grad.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("cutoff", [0.0, 6.0])
+def test_range_shifter_model(dev, machine_dir, cutoff):
+    """Beams on the second beam model (range-shifter label '4'): negative
+    sigma_ms, and energy 0 with R80 = -0.65 mm (no voxel in range).  Dose within
+    1e-3 of D_max of the oracle, which picks the model by the same label."""
+    from DoseCUDA import IMPTPlan
+    from DoseCUDA.engine import IMPTDoseEngine
+    from DoseCUDA.workloads import _beam, _grid
+    _say("range-shifter model, cutoff %g" % cutoff)
+    g = _grid((20, 26, 22), 2.5)
+    g.HU = np.random.default_rng(6).uniform(-300.0, 400.0, size=(20, 26, 22)).astype(np.float32)
+    plan = IMPTPlan()
+    for ga, ca in ((60.0, 10.0), (200.0, 0.0)):
+        b = _beam(ga, np.asarray(_spots([0, 3, 30, 60]), np.float32), couch=ca)
+        b.dicom_rangeshifter_label = '4'
+        plan.addBeam(b)
+    eng = IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=cutoff)
+    params = eng.parameters()
+    dose = eng.differentiable_dose(params)
+    (dose * dose).sum().backward()
+    ref, _ = _oracle_dose_wet(g, plan, machine_dir)
+    D = dose.detach().cpu().numpy().astype(np.float64)
+    dmax = float(ref.max())
+    err = float(np.abs(D - ref).max())
+    _say("  dose err %.2e (D_max %.3g)" % (err, dmax))
+    assert dmax > 0.0 and err <= 1e-3 * dmax
+    for mu, xy in params:
+        assert torch.isfinite(mu.grad).all() and torch.isfinite(xy.grad).all()
 
 
 def test_spots_out_of_reach(dev, machine_dir):
