@@ -45,3 +45,41 @@ def test_wet_cache_and_wet_nrrd(tmp_path):
         vol, origin, spacing = read_nrrd(str(tmp_path / ("wet_beam%02d.nrrd" % (k + 1))))
         assert np.array_equal(vol, w.voxel_data)
         assert np.allclose(origin, g.origin) and np.allclose(spacing, g.spacing)
+
+
+def test_reference_phantom_script(tmp_path, machine_dir):
+    """The reference's own IMPT example, tests/test_phantom_impt.py: the
+    default cube phantom (138^3 @ 3 mm), one beam with a ring of 98 single
+    spots of increasing energy id, computeIMPTPlan, then the PHYSICAL dose and
+    the CT written as NRRD.  Dose within 1e-3 of D_max of the oracle (gamma
+    2%/2 mm 1.0), files equal to the arrays."""
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from DoseCUDA import IMPTBeam, IMPTDoseGrid, IMPTPlan
+    from DoseCUDA.imageio import read_nrrd
+    from oracle import impt_oracle as O
+    print("[gpu-dropin] reference phantom script", flush=True)
+    dose = IMPTDoseGrid()
+    plan = IMPTPlan()
+    beam = IMPTBeam()
+    dose.createCubePhantom()
+    beam.dicom_rangeshifter_label = '0'
+    n_spots = 98
+    for energy_id in range(n_spots):
+        theta = 2.0 * 3.14159 * energy_id / n_spots
+        beam.addSingleSpot(100.0 * np.cos(theta), 100.0 * np.sin(theta), 0.2, energy_id)
+    plan.addBeam(beam)
+    dose.computeIMPTPlan(plan)
+    dose.writeDoseNRRD(str(tmp_path / "cube_impt_dose.nrrd"), individual_beams=False, dose_type="PHYSICAL")
+    dose.writeCTNRRD(str(tmp_path / "cube_phantom_ct.nrrd"))
+
+    d, origin, spacing = read_nrrd(str(tmp_path / "cube_impt_dose.nrrd"))
+    assert np.array_equal(d, dose.dose) and np.allclose(origin, dose.origin) and np.allclose(spacing, 3.0)
+    ct, _, _ = read_nrrd(str(tmp_path / "cube_phantom_ct.nrrd"))
+    assert np.array_equal(ct, dose.HU)
+
+    ref, _ = O.compute_plan(dose, plan, machine_dir)
+    m = O.compare(dose.dose, ref)
+    gamma = O.gamma_pass_rate(dose.dose, ref, 3.0)
+    print("[gpu-dropin]   max|dD|/Dmax %.2e, gamma %.4f, D_max %.4g" % (m["max_rel_to_dmax"], gamma, float(ref.max())),
+          flush=True)
+    assert float(ref.max()) > 0.0 and m["max_rel_to_dmax"] <= 1e-3 and gamma == 1.0
