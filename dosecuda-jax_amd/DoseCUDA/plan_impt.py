@@ -10,17 +10,20 @@ same class, method and attribute names:
 
 The dose calls go through :mod:`DoseCUDA.dose_kernels`, whose functions keep
 the reference extension's names and signatures but run hand-written gfx950
-HIP kernels.  The RT-Ion-Plan DICOM reader/writer of the reference
-(plan_impt.py:353-674) is not part of the dose path and is not provided.
+HIP kernels.  The RT Ion Plan reader / writer (``readPlanDicom``,
+``writePlanDicom``, plan_impt.py:17-60, 344-674) go through
+:mod:`DoseCUDA.dicomio` instead of pydicom.
 """
 
+import datetime
 import os
 import sys
 
 import numpy as np
 import pandas as pd
 
-from .plan import Plan, Beam, DoseGrid, VolumeObject  # noqa: F401  (re-exported names)
+from .plan import Plan, Beam, DoseGrid, VolumeObject, Prescription  # noqa: F401  (re-exported names)
+from . import dicomio
 from . import imageio
 from . import dose_kernels
 
@@ -259,3 +262,253 @@ class IMPTPlan(Plan):
         self.folder_rangeshifter_label = rs["folder_rangeshifter_label"]
         self.beam_models = [IMPTBeamModel(d, model_dir, f)
                             for d, f in zip(self.dicom_rangeshifter_label, self.folder_rangeshifter_label)]
+
+    # -- RT Ion Plan DICOM (plan_impt.py:353-674), through DoseCUDA.dicomio --
+
+    def load_ct_series(self, ct_dir):
+        """Every ``.dcm`` file of ct_dir, sorted by InstanceNumber (plan_impt.py:344-351)."""
+        files = [os.path.join(ct_dir, f) for f in os.listdir(ct_dir) if f.lower().endswith(".dcm")]
+        series = [dicomio.dcmread(f, force=True) for f in files]
+        series.sort(key=lambda ds: ds.InstanceNumber)
+        return series
+
+    def _beam_model_for_label(self, label):
+        labels = list(np.asarray(self.dicom_rangeshifter_label).astype(str))
+        if label not in labels:
+            raise ValueError(f"Beam model not found for rangeshifter ID {label}")
+        return self.beam_models[labels.index(label)]
+
+    def writePlanDicom(self, ct_dir, output_file, rt_struct_file=None):
+        """Write the plan as an RT Ion Plan (plan_impt.py:353-633).
+
+        Patient and study identifiers come from the first CT slice of ct_dir.
+        Each energy layer of a beam (descending energy id) becomes a pair of
+        ion control points: the first carries the spot weights, the second
+        the same positions with zero weights; cumulative meterset weights run
+        over the layers.  Dose references come from the prescriptions (one
+        default 2 Gy x n_fractions PTV prescription when there are none),
+        with ROI numbers from rt_struct_file when it is given.
+        """
+        meta = dicomio.Dataset()
+        meta.MediaStorageSOPClassUID = dicomio.RT_ION_PLAN_STORAGE
+        meta.MediaStorageSOPInstanceUID = dicomio.generate_uid()
+        meta.TransferSyntaxUID = dicomio.EXPLICIT_VR_LE
+
+        ds = dicomio.Dataset()
+        first_ct = self.load_ct_series(ct_dir)[0]
+        for kw in ("PatientName", "PatientID", "StudyInstanceUID", "FrameOfReferenceUID"):
+            setattr(ds, kw, getattr(first_ct, kw))
+        now = datetime.datetime.now()
+        day, clock = now.strftime("%Y%m%d"), now.strftime("%H%M%S.%f")
+        _fill(ds, [("SeriesInstanceUID", dicomio.generate_uid()),
+                   ("SOPClassUID", dicomio.RT_ION_PLAN_STORAGE),
+                   ("SOPInstanceUID", meta.MediaStorageSOPInstanceUID),
+                   ("Modality", "RTPLAN"),
+                   ("RTPlanLabel", self.RTPlanLabel if self.RTPlanLabel else "IMPT Plan"),
+                   ("RTPlanDate", day), ("RTPlanTime", clock),
+                   ("InstanceCreationDate", day), ("InstanceCreationTime", clock),
+                   ("NumberOfBeams", len(self.beam_list))] + _PLAN_CONSTANTS)
+        rss = dicomio.Dataset()
+        rss.ReferencedSOPClassUID = dicomio.RT_STRUCTURE_SET_STORAGE
+        rss.ReferencedSOPInstanceUID = dicomio.generate_uid()
+        ds.ReferencedStructureSetSequence = [rss]
+
+        prescriptions = list(self.Prescriptions)
+        if not prescriptions:
+            rx = Prescription()
+            rx.TargetPrescriptionDose = self.n_fractions * 2.0
+            rx.ROIName = "PTV"
+            rx.TargetUnderdoseVolumeFraction = 2.0
+            prescriptions = [rx]
+        ds = create_dose_reference(ds, prescriptions, rt_struct_file)
+
+        fg = dicomio.Dataset()
+        _fill(fg, [("FractionGroupNumber", 1), ("NumberOfFractionsPlanned", self.n_fractions),
+                   ("NumberOfBeams", len(self.beam_list)), ("NumberOfBrachyApplicationSetups", 0),
+                   ("ReferencedBeamSequence", [])])
+        ds.FractionGroupSequence = [fg]
+        setup = dicomio.Dataset()
+        setup.PatientPosition = "HFS"
+        setup.PatientSetupNumber = 1
+        ds.PatientSetupSequence = [setup]
+
+        ds.IonBeamSequence = []
+        for number, beam in enumerate(self.beam_list, start=1):
+            model = self._beam_model_for_label(beam.dicom_rangeshifter_label)
+            ds.IonBeamSequence.append(self._ion_beam(number, beam, model, fg))
+        dicomio.dcmwrite(output_file, ds, meta)
+
+    def _ion_beam(self, number, beam, model, fg):
+        """One Ion Beam Sequence item; appends the beam's meterset to fg."""
+        spots = np.asarray(beam.spot_list)
+        rs_label = beam.dicom_rangeshifter_label
+        item = dicomio.Dataset()
+        _fill(item, [("BeamNumber", number), ("BeamName", beam.BeamName),
+                     ("BeamDescription", beam.BeamDescription)] + _BEAM_CONSTANTS +
+              [("VirtualSourceAxisDistances", [float(model.VSADX), float(model.VSADY)]),
+               ("SnoutSequence", [_dataset([("SnoutID", "0")])]),
+               ("NumberOfRangeShifters", 1 if rs_label else 0)])
+        if rs_label:
+            item.RangeShifterSequence = [_dataset([("RangeShifterNumber", 1), ("RangeShifterID", rs_label),
+                                                   ("RangeShifterType", "BINARY"),
+                                                   ("AccessoryCode", rs_label)])]
+        item.NumberOfLateralSpreadingDevices = 1
+        item.LateralSpreadingDeviceSequence = [_dataset([("LateralSpreadingDeviceNumber", 1),
+                                                         ("LateralSpreadingDeviceID", "0"),
+                                                         ("LateralSpreadingDeviceType", "SCATTERER")])]
+        item.NumberOfRangeModulators = 1
+        item.RangeModulatorSequence = [_dataset([("RangeModulatorNumber", 0), ("RangeModulatorID", "0"),
+                                                 ("RangeModulatorType", "FIXED"),
+                                                 ("RangeModulatorDescription", "")])]
+        total_mu = float(np.sum(spots[:, 2]))
+        fg.ReferencedBeamSequence.append(_dataset([("ReferencedBeamNumber", number), ("BeamMeterset", total_mu)]))
+        item.FinalCumulativeMetersetWeight = total_mu
+        for tag, vr, value in _BEAM_PRIVATE:
+            item.add_new(tag, vr, value)
+
+        cps = []
+        cum = 0.0
+        fwhm_per_sigma_cm = 2.0 * np.sqrt(2.0 * np.log(2.0)) * 10.0
+        for eid in np.unique(spots[:, 3]).astype(int)[::-1]:
+            layer = spots[spots[:, 3] == eid]
+            layer_mu = np.sum(layer[:, 2])
+            width = model.reference_sigmas[eid] * fwhm_per_sigma_cm
+            xy = layer[:, 0:2].flatten().tolist()
+            lsd = [_dataset([("LateralSpreadingDeviceSetting", "IN"),
+                             ("IsocenterToLateralSpreadingDeviceDistance", 2275.0),
+                             ("LateralSpreadingDeviceWaterEquivalentThickness", 0.0),
+                             ("ReferencedLateralSpreadingDeviceNumber", 1)])]
+            rms = [_dataset([("ReferencedRangeModulatorNumber", 0)])]
+            rdr = [_dataset([("ReferencedDoseReferenceNumber", k),
+                             ("CumulativeDoseReferenceCoefficient", None)]) for k in (1, 2)]
+            # the layer's two control points: weights, then zero weights
+            for cum_w, weights in ((cum, layer[:, 2].tolist()), (cum + layer_mu, [0.0] * len(layer))):
+                cp = dicomio.Dataset()
+                _fill(cp, [("ControlPointIndex", len(cps)), ("NominalBeamEnergyUnit", "MEV"),
+                           ("NominalBeamEnergy", float(model.energy_labels[eid])),
+                           ("GantryAngle", float(beam.gantry_angle)),
+                           ("PatientSupportAngle", float(beam.couch_angle)),
+                           ("TableTopEccentricAngle", float(beam.couch_angle)),
+                           ("IsocenterPosition", [float(c) for c in beam.iso]),
+                           ("CumulativeMetersetWeight", float(cum_w)),
+                           ("NumberOfScanSpotPositions", len(layer)),
+                           ("ScanSpotPositionMap", xy), ("ScanSpotMetersetWeights", weights),
+                           ("ScanningSpotSize", [width, width])] + _CP_CONSTANTS)
+                if rs_label:
+                    cp.RangeShifterSetting = "ON"
+                cp.LateralSpreadingDeviceSettingsSequence = lsd
+                cp.RangeModulatorSettingsSequence = rms
+                cp.ReferencedDoseReferenceSequence = rdr
+                cps.append(cp)
+            cum += layer_mu
+        item.IonControlPointSequence = cps
+        item.NumberOfControlPoints = len(cps)
+        return item
+
+    def readPlanDicom(self, plan_path):
+        """Replace the beams with those of an RT Ion Plan (plan_impt.py:635-674).
+
+        Per beam: range-shifter label from the first Range Shifter Sequence
+        item ('0' without one), gantry / couch angle and isocentre from the
+        first control point, and the spots of every even-numbered control
+        point at the energy id of its nominal energy in the first beam model.
+        """
+        ds = dicomio.dcmread(plan_path, force=True)
+        self.n_fractions = float(ds.FractionGroupSequence[0].NumberOfFractionsPlanned)
+        self.beam_list = []
+        self.n_beams = 0
+        for ibs in ds.IonBeamSequence:
+            beam = IMPTBeam()
+            if ibs.NumberOfRangeShifters and ibs.RangeShifterSequence[0].RangeShifterID is not None:
+                beam.dicom_rangeshifter_label = ibs.RangeShifterSequence[0].RangeShifterID
+            else:
+                beam.dicom_rangeshifter_label = "0"
+            cp0 = ibs.IonControlPointSequence[0]
+            beam.gantry_angle = float(cp0.GantryAngle)
+            beam.couch_angle = float(cp0.PatientSupportAngle)
+            beam.iso = np.array(cp0.IsocenterPosition, dtype=np.single)
+            if "BeamName" in ibs:
+                beam.BeamName = ibs.BeamName
+            if "BeamDescription" in ibs:
+                beam.BeamDescription = ibs.BeamDescription
+            for cp in ibs.IonControlPointSequence[::2]:
+                beam.addSpotData(cp, self.beam_models[0].energyIDFromLabel(float(cp.NominalBeamEnergy)))
+            self.addBeam(beam)
+
+
+# Fixed attributes of the RT Ion Plan the writer emits (plan_impt.py:386-395,
+# 459-475, 536-555).
+_PLAN_CONSTANTS = [("RTPlanGeometry", "PATIENT"), ("ApprovalStatus", "UNAPPROVED"),
+                   ("Manufacturer", "RaySearch Laboratories"), ("PlanIntent", "CURATIVE"),
+                   ("TreatmentProtocols", ["PencilBeamScanning"]), ("PatientIdentityRemoved", "YES"),
+                   ("DeidentificationMethod", "RayStation Custom Export"),
+                   ("NumberOfBrachyApplicationSetups", 0)]
+_BEAM_CONSTANTS = [("BeamType", "STATIC"), ("RadiationType", "PROTON"), ("ScanMode", "MODULATED_SPEC"),
+                   ("ModulatedScanModeType", "STATIONARY"), ("TreatmentMachineName", "ProBeat_noMRF_"),
+                   ("Manufacturer", "Generic"), ("PrimaryDosimeterUnit", "MU"),
+                   ("TreatmentDeliveryType", "TREATMENT"), ("NumberOfWedges", 0), ("NumberOfCompensators", 0),
+                   ("NumberOfBoli", 0), ("NumberOfBlocks", 0), ("PatientSupportType", "TABLE"),
+                   ("PatientSupportID", "TABLE")]
+_BEAM_PRIVATE = [((0x4001, 0x0010), "LO", "RAYSEARCHLABS 2.0"),
+                 ((0x4001, 0x1002), "UN", b"Constant 1.1"),
+                 ((0x4001, 0x1012), "UN", b"ProBeat_noMRF_"),
+                 ((0x4001, 0x1033), "UN", b"RBE_NOT_INCLUDED"),
+                 ((0x4001, 0x1003), "UN", b"20181030034922.000000 ")]
+_CP_CONSTANTS = [("GantryRotationDirection", "NONE"), ("BeamLimitingDeviceAngle", 0.0),
+                 ("BeamLimitingDeviceRotationDirection", "NONE"), ("PatientSupportRotationDirection", "NONE"),
+                 ("TableTopEccentricRotationDirection", "NONE"), ("TableTopPitchAngle", 0.0),
+                 ("TableTopPitchRotationDirection", "NONE"), ("TableTopRollAngle", 0.0),
+                 ("TableTopRollRotationDirection", "NONE"), ("GantryPitchAngle", 0.0),
+                 ("GantryPitchRotationDirection", "NONE"), ("TableTopVerticalPosition", ""),
+                 ("TableTopLongitudinalPosition", ""), ("TableTopLateralPosition", ""),
+                 ("SnoutPosition", 391.0), ("MetersetRate", 480.0), ("ScanSpotTuneID", "Standard"),
+                 ("NumberOfPaintings", 1)]
+
+
+def _fill(ds, pairs):
+    for kw, value in pairs:
+        setattr(ds, kw, value)
+    return ds
+
+
+def _dataset(pairs):
+    return _fill(dicomio.Dataset(), pairs)
+
+
+def get_roi_num(rt_ds, ROIName):
+    """ROINumber of the structure named ROIName in an RT Structure Set, or
+    None (plan_impt.py:17-21)."""
+    for roi in rt_ds.StructureSetROISequence:
+        if roi.ROIName == ROIName:
+            return roi.ROINumber
+    return None
+
+
+def create_dose_reference(ds, prescriptions, rt_struct_file=None):
+    """Append a target Dose Reference item per prescription (plan_impt.py:24-60).
+
+    ROI numbers are the prescriptions' positions (1, 2, ...) or, with an RT
+    Structure Set file, the numbers of the structures named by the
+    prescriptions (ValueError for a missing name), whose SOP instance UID then
+    becomes the plan's referenced structure set.
+    """
+    if "DoseReferenceSequence" not in ds:
+        ds.DoseReferenceSequence = []
+    rt_ds = dicomio.dcmread(rt_struct_file, force=True) if rt_struct_file else None
+    for pos, rx in enumerate(prescriptions, start=1):
+        roi = pos
+        if rt_ds is not None:
+            roi = get_roi_num(rt_ds, rx.ROIName)
+            if roi is None:
+                raise ValueError(f"No structure labelled '{rx.ROIName}' found in RT structure set.")
+        dr = _dataset([("DoseReferenceNumber", len(ds.DoseReferenceSequence) + 1), ("ReferencedROINumber", roi),
+                       ("DoseReferenceStructureType", "VOLUME"),
+                       ("DoseReferenceDescription", f"{rx.ROIName} Prescription"),
+                       ("DoseReferenceType", "TARGET"), ("TargetPrescriptionDose", rx.TargetPrescriptionDose),
+                       ("TargetUnderdoseVolumeFraction", rx.TargetUnderdoseVolumeFraction)])
+        dr.add_new((0x4001, 0x0010), "LO", "RAYSEARCHLABS 2.0")
+        dr.add_new((0x4001, 0x1011), "UN", str(rx.TargetPrescriptionDose).encode())
+        ds.DoseReferenceSequence.append(dr)
+    if rt_ds is not None:
+        ds.ReferencedStructureSetSequence[0].ReferencedSOPInstanceUID = rt_ds.SOPInstanceUID
+    return ds
