@@ -83,3 +83,63 @@ def test_reference_phantom_script(tmp_path, machine_dir):
     print("[gpu-dropin]   max|dD|/Dmax %.2e, gamma %.4f, D_max %.4g" % (m["max_rel_to_dmax"], gamma, float(ref.max())),
           flush=True)
     assert float(ref.max()) > 0.0 and m["max_rel_to_dmax"] <= 1e-3 and gamma == 1.0
+
+
+def test_plan_and_ct_through_dicom(tmp_path):
+    """A CT series and an RT Ion Plan written as DICOM and read back
+    (loadCTDCM, writePlanDicom, readPlanDicom; plan.py:43-57,
+    plan_impt.py:353-674) give the dose of the in-memory plan exactly: the
+    files hold the same HU, spots, angles and isocentres, and the dose path
+    sorts spots by energy id and position, so the layer order of the file does
+    not matter."""
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from DoseCUDA import IMPTDoseGrid, IMPTPlan, dicomio
+    from DoseCUDA.workloads import _beam, _grid, grid_spots
+    print("[gpu-dropin] CT series and plan through DICOM", flush=True)
+    g = _grid((24, 30, 28), 2.5)
+    rng = np.random.default_rng(6)
+    g.HU = np.round(rng.uniform(-300.0, 600.0, size=(24, 30, 28))).astype(np.float32)
+    ct_dir = tmp_path / "ct"
+    ct_dir.mkdir()
+    for k in range(g.HU.shape[0]):
+        ds = dicomio.Dataset()
+        ds.SOPClassUID = dicomio.CT_IMAGE_STORAGE
+        ds.SOPInstanceUID = dicomio.generate_uid()
+        ds.Modality = "CT"
+        ds.PatientName, ds.PatientID = "Phantom^Head", "PH1"
+        ds.StudyInstanceUID, ds.SeriesInstanceUID, ds.FrameOfReferenceUID = "1.2.3", "1.2.3.1", "1.2.3.9"
+        ds.InstanceNumber = g.HU.shape[0] - k
+        ds.ImagePositionPatient = [float(g.origin[0]), float(g.origin[1]), float(g.origin[2] + k * g.spacing[2])]
+        ds.PixelSpacing = [float(g.spacing[1]), float(g.spacing[0])]
+        ds.Rows, ds.Columns = g.HU.shape[1], g.HU.shape[2]
+        ds.SamplesPerPixel, ds.BitsAllocated, ds.BitsStored, ds.HighBit, ds.PixelRepresentation = 1, 16, 16, 15, 1
+        ds.RescaleSlope, ds.RescaleIntercept = 1.0, -1024.0
+        ds.PixelData = (g.HU[k].astype(np.int32) + 1024).astype("<i2").tobytes()
+        meta = dicomio.Dataset()
+        meta.MediaStorageSOPClassUID = dicomio.CT_IMAGE_STORAGE
+        meta.MediaStorageSOPInstanceUID = ds.SOPInstanceUID
+        dicomio.dcmwrite(str(ct_dir / ("ct%03d.dcm" % k)), ds, meta)
+
+    plan = IMPTPlan()
+    plan.n_fractions = 2
+    c = np.arange(-9.0, 9.0 + 1e-6, 3.0)
+    for ga, ca, rs in ((0.0, 0.0, "0"), (250.0, 15.0, "4")):
+        sl = grid_spots([20, 31, 26], c, c, disk_radius=9.0)
+        sl[:, 2] = rng.uniform(0.2, 1.0, size=sl.shape[0]).astype(np.float32)
+        b = _beam(ga, sl, couch=ca)
+        b.dicom_rangeshifter_label = rs
+        b.iso = np.array([1.25, -2.5, 0.75], dtype=np.single)
+        plan.addBeam(b)
+    plan.writePlanDicom(str(ct_dir), str(tmp_path / "plan.dcm"))
+    back = IMPTPlan()
+    back.readPlanDicom(str(tmp_path / "plan.dcm"))
+
+    h = IMPTDoseGrid()
+    h.loadCTDCM(str(ct_dir))
+    assert np.array_equal(h.HU, g.HU) and np.allclose(h.origin, g.origin) and np.allclose(h.spacing, g.spacing)
+    g.computeIMPTPlan(plan)
+    h.computeIMPTPlan(back)
+    assert float(g.dose.max()) > 0.0
+    assert np.array_equal(h.dose, g.dose)
+    for a, b in zip(g.beam_doses, h.beam_doses):
+        assert np.array_equal(a, b)
