@@ -76,7 +76,7 @@ class dh_batch(ctypes.Structure):
                 ("n_bricks", ctypes.c_int64), ("n_smooth_blocks", ctypes.c_int64),
                 ("n_spots", ctypes.c_int64), ("n_layers", ctypes.c_int64),
                 ("n_sum_blocks", ctypes.c_int64), ("n_partials", ctypes.c_int64),
-                ("n_adj32", ctypes.c_int64), ("n_adj64", ctypes.c_int64),
+                ("n_adj32", ctypes.c_int64), ("n_adj64", ctypes.c_int64), ("n_adj128", ctypes.c_int64),
                 ("n_reduce_blocks", ctypes.c_int64), ("tab", ctypes.c_int64 * 16),
                 ("fwd_blocks", ctypes.c_int32), ("ray_blocks", ctypes.c_int32),
                 ("prepared", ctypes.c_int32), ("n_summary_rows", ctypes.c_int32),

@@ -7,9 +7,10 @@ What is tuned, per batch shape:
   persistent grid of n 256-thread blocks striding over the bricks of all
   beams.  Results do not depend on it.
 * ``adj_cs``, ``adj_wg_target``, ``adj_max_slabs``, ``adj_min_bricks``: the
-  adjoint's spot-to-workgroup schedule -- spots per workgroup (32 or 64) and
-  how the bricks of a beam are cut into slabs (about wg_target workgroups per
-  beam, at most max_slabs slabs, at least min_bricks bricks per slab).  Only
+  adjoint's spot-to-workgroup schedule -- spots per workgroup (32, 64, or 128
+  packed) and how the bricks of a beam are cut into slabs (about wg_target
+  workgroups per beam, at most max_slabs slabs, at least min_bricks bricks
+  per slab).  Only
   the order of partial sums changes.
 
 Search: coordinate descent, one parameter at a time in the order above.  A
@@ -40,7 +41,7 @@ _LOCK = threading.Lock()
 
 RAY_BLOCKS = (0, 2048, 8192)
 FWD_BLOCKS = (0, 2048, 8192)
-ADJ_CS = (32, 64)
+ADJ_CS = (32, 64, 128)
 ADJ_WG_TARGET = (16384, 32768, 65536, 131072, 262144)
 ADJ_MAX_SLABS = (512, 1024, 2048, 4096)
 ADJ_MIN_BRICKS = (16, 32, 64, 128)

@@ -237,7 +237,7 @@ class BatchedDose:
         b = self.batch
         return {"bricks": b.n_bricks, "smooth_blocks": b.n_smooth_blocks, "spots": b.n_spots,
                 "layers": b.n_layers, "sum_blocks": b.n_sum_blocks, "partials": b.n_partials,
-                "adjoint_wg_32": b.n_adj32, "adjoint_wg_64": b.n_adj64, "reduce_blocks": b.n_reduce_blocks,
+                "adjoint_wg_32": b.n_adj32, "adjoint_wg_64": b.n_adj64, "adjoint_wg_128": b.n_adj128, "reduce_blocks": b.n_reduce_blocks,
                 "workspace_bytes": self.ws_bytes}
 
 

@@ -11,8 +11,8 @@
 //            fixed-order sum of the beam doses where beams write volumes of
 //            their own (plans with several beams)
 //   adjoint  brick summaries (one workgroup per row of bricks), (chunk, slab)
-//            workgroups of all beams (one launch per chunk size), per-spot
-//            reduction
+//            workgroups of all beams (one launch per chunk size: 32, 64, or
+//            packed 128), per-spot reduction
 // A work item finds its beam by binary search in a table of item offsets and
 // reads the beam's context (pointers, geometry, schedule) from a device table,
 // so the launches need no host-side state beyond item counts and can be
@@ -39,7 +39,7 @@ enum Table {
     T_SMOOTH_OFF,   // smoothing tiles (F_SMOOTH_TILE) or 256-voxel blocks
     T_SPOT_OFF,     // spot records
     T_LAYER_OFF,    // layer records
-    T_PART_OFF,     // adjoint partial records per beam (bookkeeping; no kernel reads it)
+    T_ADJ128_OFF,   // adjoint workgroups, packed 128-spot chunks
     T_ADJ32_OFF,    // adjoint workgroups, 32-spot chunks
     T_ADJ64_OFF,    // adjoint workgroups, 64-spot chunks
     T_REDUCE_OFF,   // reduction blocks
@@ -76,12 +76,12 @@ static AdjTune tune_from(const dh_batch_tune *t) {
 struct Layout {
     std::vector<BeamCtx> ctx;
     std::vector<PlanCtx> plans;
-    std::vector<long long> brick, smooth, smooth_tiles, spot, layer, part, adj32, adj64, reduce, sum, rows;
+    std::vector<long long> brick, smooth, smooth_tiles, spot, layer, adj128, adj32, adj64, reduce, sum, rows;
     std::vector<Stencil> stencil;
     std::vector<int> touch;
     long long tab[N_TABLES];
     long long n_bricks = 0, n_smooth = 0, n_spots = 0, n_layers = 0, n_sum = 0, n_part = 0, n_adj32 = 0,
-              n_adj64 = 0, n_reduce = 0, n_rows = 0, n_tiles = 0;
+              n_adj64 = 0, n_adj128 = 0, n_reduce = 0, n_rows = 0, n_tiles = 0;
     int box_max = 0;  // largest tiled-smoothing box of the batch (floats)
     long long bytes = 0;
     int flags = 0;
@@ -222,12 +222,14 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
             L.n_spots += S;
             L.layer.push_back(L.n_layers);
             L.n_layers += NL;
-            L.part.push_back(L.n_part);
             L.n_part += S > 0 ? (long long)w.nslab * S : 0;
             const long long items = S > 0 ? adjoint_chunks(S, NL, w.cs) * w.nslab : 0;
             L.adj32.push_back(L.n_adj32);
             L.adj64.push_back(L.n_adj64);
-            if (w.cs == 32) L.n_adj32 += items; else L.n_adj64 += items;
+            L.adj128.push_back(L.n_adj128);
+            if (w.cs == 32) L.n_adj32 += items;
+            else if (w.cs == 64) L.n_adj64 += items;
+            else L.n_adj128 += items;
             L.reduce.push_back(L.n_reduce);
             L.n_reduce += (S + 63) / 64;
             // brick summary: one workgroup per row of bricks along x; a beam
@@ -316,7 +318,7 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     put(T_SMOOTH_OFF, D.smooth.data(), D.smooth.size() * 8);
     put(T_SPOT_OFF, D.spot.data(), D.spot.size() * 8);
     put(T_LAYER_OFF, D.layer.data(), D.layer.size() * 8);
-    put(T_PART_OFF, D.part.data(), D.part.size() * 8);
+    put(T_ADJ128_OFF, D.adj128.data(), D.adj128.size() * 8);
     put(T_ADJ32_OFF, D.adj32.data(), D.adj32.size() * 8);
     put(T_ADJ64_OFF, D.adj64.data(), D.adj64.size() * 8);
     put(T_REDUCE_OFF, D.reduce.data(), D.reduce.size() * 8);
@@ -345,6 +347,7 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     batch->n_partials = D.n_part;
     batch->n_adj32 = D.n_adj32;
     batch->n_adj64 = D.n_adj64;
+    batch->n_adj128 = D.n_adj128;
     batch->n_reduce_blocks = D.n_reduce;
     for (int i = 0; i < N_TABLES; ++i) batch->tab[i] = D.tab[i];
     batch->tab[T_FLAGS] = D.flags | (D.flags & F_SMOOTH_TILE ? (long long)D.box_max << 32 : 0LL);
@@ -417,7 +420,8 @@ extern "C" int dh_batch_adjoint(dh_batch *b, int32_t flags, void *stream) {
     if (rc) return rc;
     return launch_adjoint_batch(table<BeamCtx>(b, T_CTX), b->n_beams, table<long long>(b, T_ROW_OFF),
                                 b->n_summary_rows, table<long long>(b, T_ADJ32_OFF), b->n_adj32,
-                                table<long long>(b, T_ADJ64_OFF), b->n_adj64, table<long long>(b, T_REDUCE_OFF),
+                                table<long long>(b, T_ADJ64_OFF), b->n_adj64, table<long long>(b, T_ADJ128_OFF),
+                                b->n_adj128, table<long long>(b, T_REDUCE_OFF),
                                 b->n_reduce_blocks, b->cutoff_sigma > 0.0f, st, b->events);
 }
 

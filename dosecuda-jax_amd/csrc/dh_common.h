@@ -301,8 +301,9 @@ int launch_forward_batch(const BeamCtx *ctx, const long long *brick_off, int n_b
                          bool cull, int persist_blocks, hipStream_t st, void *const *ev = nullptr);
 int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *row_off, long long total_rows,
                          const long long *item_off32, long long total_items32, const long long *item_off64,
-                         long long total_items64, const long long *reduce_off, long long total_reduce, bool cull,
-                         hipStream_t st, void *const *ev = nullptr);
+                         long long total_items64, const long long *item_off128, long long total_items128,
+                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st,
+                         void *const *ev);
 
 // ---------------------------------------------------------------------------
 // wave64 helpers

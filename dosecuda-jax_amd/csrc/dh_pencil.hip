@@ -911,11 +911,263 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
     if (sval && group == 0) gst(c.partial, (long long)slab * c.n_spots + spot, make_float4(K, FX, FY, 0.0f));
 }
 
+// Packed adjoint (CS = PACK_CS): a chunk holds up to 128 spots of one layer
+// (a layer of n spots splits into ceil(n / 128) near-equal chunks), two per
+// lane.  Per brick, the chunk's spots within reach are listed and run in
+// passes of np spots: lane = group * np + slot, G = min(64 / np, 16) voxel
+// subsets per pass, so the lanes carry pairs of reachable spots only.  The
+// pass count minimises the loop trips (plus a fixed cost per pass).  Each
+// pass folds its G subsets through LDS into the spot's owner lane in a fixed
+// order (deterministic).
+constexpr int PACK_CS = 128;
+constexpr int PACK_GMAX = 16;
+constexpr int PACK_VOX = WAVE + PACK_GMAX;  // voxel list incl. the zero tail a pass may read
+
+struct PackLds {
+    LdsVox vox[PACK_VOX];
+    float4 part[WAVE];
+    int list[PACK_CS];
+};
+
+// floor(a / b) for 0 <= a < 256, 1 <= b <= 128: a + 0.5 keeps the quotient
+// at least 0.5 / b from an integer (2^-16 relative), far beyond the 2^-22
+// relative error of the hardware reciprocal and the product.
+__device__ __forceinline__ int small_div(int a, int b) {
+    return (int)(((float)a + 0.5f) * __builtin_amdgcn_rcpf((float)b));
+}
+
+// Squared distance from (hx, hy, hz) to the axis of the spot at (x, y) with
+// slopes (tx, ty) and itt = 1 / |t|^2: the same terms as the adjoint loop.
+__device__ __forceinline__ float axis_d2_t(float hx, float hy, float hz, float x, float y, float tx, float ty,
+                                           float itt) {
+    const float a = hx - x, b = hy - y;
+    const float u = fmaf(hz, tx, a), w = fmaf(hz, ty, b);
+    const float qr = fmaf(u, tx * itt, fmaf(w, ty * itt, -hz));
+    return fmaf(u, fmaf(-qr, tx, a), w * fmaf(-qr, ty, b));
+}
+
+__device__ __forceinline__ int pack_groups(int np) { return min(small_div(WAVE, np), PACK_GMAX); }
+
+// Loop trips of nrel spots in P passes over na voxels, plus 2 per pass.
+__device__ __forceinline__ int pack_cost(int nrel, int na, int P) {
+    const int np = small_div(nrel + P - 1, P);
+    const int G = pack_groups(np);
+    return P * (small_div(na + G - 1, G) + 2);
+}
+
+template <bool CULL, bool COUNT>
+__device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, int slab, int lane, PackLds &S,
+                                                    unsigned long long &n_dense, unsigned long long &n_exec) {
+    const LayerC *__restrict__ lc = c.lc;
+    int l = 0, c0 = -1, cnt = 0;
+    for (; l < c.n_layers; ++l) {
+        const int n = gld(lc, l).n;
+        const int nch = (n + PACK_CS - 1) / PACK_CS;
+        if (cid < nch) {
+            c0 = (int)((long long)cid * n / nch);
+            cnt = (int)((long long)(cid + 1) * n / nch) - c0;
+            break;
+        }
+        cid -= nch;
+    }
+    if (c0 < 0) return;  // surplus item of the upper-bound grid
+    const LayerC L = gld(lc, l);
+    if (!L.ok) return;
+    // lane owns chunk spots lane and lane + 64
+    const bool v0 = lane < cnt, v1 = lane + WAVE < cnt;
+    float x0 = 0.0f, y0 = 0.0f, tx0 = 0.0f, ty0 = 0.0f, x1 = 0.0f, y1 = 0.0f, tx1 = 0.0f, ty1 = 0.0f;
+    if (v0) {
+        const SpotExt q = gld(c.sp, L.start + c0 + lane);
+        x0 = q.x;
+        y0 = q.y;
+        tx0 = q.tx;
+        ty0 = q.ty;
+    }
+    if (v1) {
+        const SpotExt q = gld(c.sp, L.start + c0 + lane + WAVE);
+        x1 = q.x;
+        y1 = q.y;
+        tx1 = q.tx;
+        ty1 = q.ty;
+    }
+    const float it0 = 1.0f / (tx0 * tx0 + ty0 * ty0 + 1.0f);
+    const float it1 = 1.0f / (tx1 * tx1 + ty1 * ty1 + 1.0f);
+    const float ivx = c.b.ivsadx, ivy = c.b.ivsady;
+    LayerC R = L;
+    if (CULL) reach_of(c.sp, L.start + c0, cnt, c.m, L.eid, lane, R);
+    if (lane < PACK_VOX - WAVE) S.vox[WAVE + lane] = LdsVox{};  // permanent zero tail
+
+    const long long nb = c.bg.count();
+    const long long b0 = nb * slab / c.nslab, b1 = nb * (slab + 1) / c.nslab;
+    float K0 = 0.0f, FX0 = 0.0f, FY0 = 0.0f, K1 = 0.0f, FX1 = 0.0f, FY1 = 0.0f;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+
+    for (long long base = b0; base < b1; base += WAVE) {
+        bool brel = false;
+        if (base + lane < b1) {
+            const float2 ws = gld(c.bsum, base + lane);
+            brel = ws.x <= ws.y && ws.x <= L.lim;
+            if (CULL && brel && !COUNT) {
+                int ti, tj, tk;
+                brick_of(c.bg, base + lane, ti, tj, tk);
+                brel = !reach_skip(R, brick_info_at(c.b, brick_centre(c.g, c.b, ti, tj, tk), ws.x, ws.y), ivx, ivy,
+                                   c.cut, c.rbrick);
+            }
+        }
+        unsigned long long fm = __ballot(brel);
+        while (fm) {
+            const long long brick = base + __builtin_ctzll(fm);
+            fm &= fm - 1ull;
+            int bi, bj, bk, i, j, k;
+            const VoxelOf vo = brick_voxel(c.g, c.bg, brick, lane, bi, bj, bk, i, j, k);
+            const float gv = vo.valid ? gld(c.gdose, vo.idx) : 0.0f;
+            const float wet = vo.valid ? gld(c.wet, vo.idx) * 10.0f : 3.0e38f;
+            const bool act = vo.valid && gv != 0.0f && wet <= L.lim;
+            const unsigned long long am = __ballot(act);
+            if (!am) continue;
+            if (COUNT) n_dense += (unsigned long long)__popcll(am) * (unsigned)cnt;
+
+            const float3 v = voxel_xyz(c.g, c.b, i, j, k);
+            const float3 h = image_to_head(c.b, v);
+            const float sx = c.b.src[0] - v.x, sy = c.b.src[1] - v.y, sz = c.b.src[2] - v.z;
+            const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
+            const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
+
+            bool r0 = v0, r1 = v1;
+            if (CULL) {
+                const float rc = fmaf(c.cut, wave_max_nonneg(act ? T.sh : 0.0f), c.rbrick);
+                const float3 hc = image_to_head(c.b, brick_centre(c.g, c.b, bi, bj, bk));
+                r0 = v0 && axis_d2_t(hc.x, hc.y, hc.z, x0, y0, tx0, ty0, it0) < rc * rc;
+                r1 = v1 && axis_d2_t(hc.x, hc.y, hc.z, x1, y1, tx1, ty1, it1) < rc * rc;
+            }
+            const unsigned long long m0 = __ballot(r0), m1 = __ballot(r1);
+            const int n0 = __popcll(m0), nrel = n0 + __popcll(m1);
+            if (nrel == 0) continue;
+            if (COUNT) {
+                n_exec += (unsigned long long)__popcll(am) * (unsigned)nrel;
+                continue;
+            }
+            // list positions: reachable first-set spots, then second-set spots
+            const int p0 = __popcll(m0 & lt_mask), p1 = n0 + __popcll(m1 & lt_mask);
+            if (r0) S.list[p0] = lane;
+            if (r1) S.list[p1] = lane + WAVE;
+            const int na = __popcll(am);
+            if (act) {
+                LdsVox e;
+                e.hx = h.x;
+                e.hy = h.y;
+                e.hz = h.z;
+                e.a1 = T.a1;
+                e.a2 = T.a2;
+                e.GA = gv * T.Af;
+                e.GB = gv * T.Bf;
+                e.GAa = e.GA * (T.a1 * (1.0f / kLog2e));
+                e.GBb = e.GB * (T.a2 * (1.0f / kLog2e));
+                e.pad0 = e.pad1 = e.pad2 = 0.0f;
+                S.vox[__popcll(am & lt_mask)] = e;
+            }
+            // zero voxels past na: every term of a zero voxel is an exact +0
+            if (lane >= na) S.vox[lane] = LdsVox{};
+            // passes: fewest loop trips, from the minimum count up
+            const int p_min = (nrel + WAVE - 1) / WAVE;
+            int P = p_min, best = pack_cost(nrel, na, P);
+            for (int Q = p_min + 1; Q <= p_min + 3 && Q <= nrel; ++Q) {
+                const int cq = pack_cost(nrel, na, Q);
+                if (cq < best) {
+                    best = cq;
+                    P = Q;
+                }
+            }
+            const int np_full = small_div(nrel + P - 1, P);
+            __syncthreads();
+            for (int s0 = 0; s0 < nrel; s0 += np_full) {
+                const int np = min(np_full, nrel - s0);
+                const int G = pack_groups(np);
+                const int group = small_div(lane, np), slot = lane - group * np;
+                const bool on = group < G;
+                const int js = on ? S.list[s0 + slot] : 0;
+                const int src = js & (WAVE - 1);
+                const bool hi = js >= WAVE;
+                // the pass's spot from its owner lane (every lane takes part)
+                const float ax = __shfl(x0, src), bx = __shfl(x1, src);
+                const float ay = __shfl(y0, src), by = __shfl(y1, src);
+                const float atx = __shfl(tx0, src), btx = __shfl(tx1, src);
+                const float aty = __shfl(ty0, src), bty = __shfl(ty1, src);
+                const float ait = __shfl(it0, src), bit = __shfl(it1, src);
+                const float qx = hi ? bx : ax, qy = hi ? by : ay, qtx = hi ? btx : atx, qty = hi ? bty : aty;
+                const float qit = hi ? bit : ait;
+                const float ttx = qtx * qit, tty = qty * qit;
+                const int nit = small_div(na + G - 1, G);
+                float K = 0.0f, X0 = 0.0f, X1 = 0.0f, Y0 = 0.0f, Y1 = 0.0f;
+                if (on) {
+                    for (int it = 0; it < nit; ++it) {
+                        const LdsVox e = S.vox[it * G + group];
+                        // same terms as adjoint_item
+                        const float a = e.hx - qx, b = e.hy - qy;
+                        const float u = fmaf(e.hz, qtx, a), w = fmaf(e.hz, qty, b);
+                        const float qr = fmaf(u, ttx, fmaf(w, tty, -e.hz));
+                        const float px = fmaf(-qr, qtx, a), py = fmaf(-qr, qty, b);
+                        const float d2 = fmaf(u, px, w * py);
+                        const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
+                        const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
+                        K = fmaf(e.GA, e1, K);
+                        K = fmaf(e.GB, e2, K);
+                        const float F = fmaf(e.GAa, e1, e.GBb * e2);
+                        const float Fq = F * qr;
+                        X0 = fmaf(F, px, X0);
+                        X1 = fmaf(Fq, px, X1);
+                        Y0 = fmaf(F, py, Y0);
+                        Y1 = fmaf(Fq, py, Y1);
+                    }
+                }
+                S.part[lane] = make_float4(K, fmaf(X1, ivx, X0), fmaf(Y1, ivy, Y0), 0.0f);
+                __syncthreads();
+                // owners add their spot's G subset sums in group order
+                const int o0 = p0 - s0, o1 = p1 - s0;
+                if (r0 && o0 >= 0 && o0 < np) {
+                    float4 t = S.part[o0];
+                    for (int g = 1; g < G; ++g) {
+                        const float4 y = S.part[g * np + o0];
+                        t.x += y.x;
+                        t.y += y.y;
+                        t.z += y.z;
+                    }
+                    K0 += t.x;
+                    FX0 += t.y;
+                    FY0 += t.z;
+                }
+                if (r1 && o1 >= 0 && o1 < np) {
+                    float4 t = S.part[o1];
+                    for (int g = 1; g < G; ++g) {
+                        const float4 y = S.part[g * np + o1];
+                        t.x += y.x;
+                        t.y += y.y;
+                        t.z += y.z;
+                    }
+                    K1 += t.x;
+                    FX1 += t.y;
+                    FY1 += t.z;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (COUNT) return;
+    const long long row = (long long)slab * c.n_spots + L.start + c0;
+    if (v0) gst(c.partial, row + lane, make_float4(K0, FX0, FY0, 0.0f));
+    if (v1) gst(c.partial, row + lane + WAVE, make_float4(K1, FX1, FY1, 0.0f));
+}
+
 template <bool CULL, int CS, bool COUNT = false>
 __global__ __launch_bounds__(64) void k_pb_adjoint(BeamCtx c, unsigned long long *__restrict__ counts = nullptr) {
-    __shared__ LdsVox s_vox[WAVE];
     unsigned long long nd = 0, ne = 0;
-    adjoint_item<CULL, CS, COUNT>(c, blockIdx.x, blockIdx.y, threadIdx.x, s_vox, nd, ne);
+    if constexpr (CS == PACK_CS) {
+        __shared__ PackLds s_pack;
+        adjoint_item_packed<CULL, COUNT>(c, blockIdx.x, blockIdx.y, threadIdx.x, s_pack, nd, ne);
+    } else {
+        __shared__ LdsVox s_vox[WAVE];
+        adjoint_item<CULL, CS, COUNT>(c, blockIdx.x, blockIdx.y, threadIdx.x, s_vox, nd, ne);
+    }
     if (COUNT && threadIdx.x == 0) {
         atomicAdd(&counts[2], nd);
         atomicAdd(&counts[3], ne);
@@ -928,7 +1180,6 @@ template <bool CULL, int CS>
 __global__ __launch_bounds__(64) void k_pb_adjoint_batch(const BeamCtx *__restrict__ ctx,
                                                         const long long *__restrict__ off, int n_beams,
                                                         long long base, long long total) {
-    __shared__ LdsVox s_vox[WAVE];
     const long long item = base + blockIdx.x;
     if (item >= total) return;
     const int bm = find_owner(off, n_beams, item);
@@ -938,7 +1189,13 @@ __global__ __launch_bounds__(64) void k_pb_adjoint_batch(const BeamCtx *__restri
     const unsigned lu = (unsigned)local, ns = (unsigned)c.nslab;
     const int cid = (int)(lu / ns), slab = (int)(lu - (unsigned)(lu / ns) * ns);
     unsigned long long nd = 0, ne = 0;
-    adjoint_item<CULL, CS, false>(c, cid, slab, threadIdx.x, s_vox, nd, ne);
+    if constexpr (CS == PACK_CS) {
+        __shared__ PackLds s_pack;
+        adjoint_item_packed<CULL, false>(c, cid, slab, threadIdx.x, s_pack, nd, ne);
+    } else {
+        __shared__ LdsVox s_vox[WAVE];
+        adjoint_item<CULL, CS, false>(c, cid, slab, threadIdx.x, s_vox, nd, ne);
+    }
 }
 
 // 256 threads = 64 spots x 4 slab groups; each group sums every 4th slab,
@@ -1009,16 +1266,11 @@ __global__ __launch_bounds__(256) void k_pb_reduce_batch(const BeamCtx *__restri
 static constexpr long long kAlign = 256;
 static long long align_up(long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-// Spots per adjoint workgroup: 64, or 32 (with two voxel subsets per wave) when
-// the average layer is small enough that 64-spot chunks would leave many lanes
-// idle.  Scores: lane use / (1 + per-brick overhead share), the overhead share
-// growing as chunks shrink.
-int adjoint_chunk(int n_spots, int n_layers) {
-    if (n_layers <= 0) return 64;
-    const double n = (double)n_spots / n_layers;
-    const double u64 = n / (ceil(n / 64.0) * 64.0), u32 = n / (ceil(n / 32.0) * 32.0);
-    return (u32 / 1.38 > u64 / 1.23) ? 32 : 64;
-}
+// Spots per adjoint workgroup when the schedule leaves it open: the packed
+// chunks, which fill the lanes per brick with the reachable spots (the
+// autotuner chose them over 32- and 64-spot chunks on the head-phantom, the
+// uneven-plan and the 1 mm body workloads).  32 and 64 stay selectable.
+int adjoint_chunk(int, int) { return PACK_CS; }
 
 // Slabs of bricks per spot chunk: enough (chunk, slab) workgroups (wg_target)
 // that the few heavy ones (chunks and slabs in the beam path) do not leave the
@@ -1044,7 +1296,7 @@ AdjTune default_tune() {
 }
 
 bool tune_ok(const AdjTune &t) {
-    return (t.cs == 0 || t.cs == 32 || t.cs == 64) && t.wg_target >= 1 && t.min_bricks >= 1 && t.max_slabs >= 1 &&
+    return (t.cs == 0 || t.cs == 32 || t.cs == 64 || t.cs == PACK_CS) && t.wg_target >= 1 && t.min_bricks >= 1 && t.max_slabs >= 1 &&
            t.max_slabs <= 65535;
 }
 
@@ -1166,7 +1418,10 @@ static int launch_adjoint(hipStream_t st, const BeamCtx &c, unsigned long long *
     if (rc) return rc;
     const dim3 grid((unsigned)adjoint_chunks(c.n_spots, c.n_layers, c.cs), c.nslab);
     const bool cull = c.cut > 0.0f;
-    if (c.cs == 32) {
+    if (c.cs == PACK_CS) {
+        if (cull) hipLaunchKernelGGL((k_pb_adjoint<true, PACK_CS, COUNT>), grid, dim3(64), 0, st, c, counts);
+        else hipLaunchKernelGGL((k_pb_adjoint<false, PACK_CS, COUNT>), grid, dim3(64), 0, st, c, counts);
+    } else if (c.cs == 32) {
         if (cull) hipLaunchKernelGGL((k_pb_adjoint<true, 32, COUNT>), grid, dim3(64), 0, st, c, counts);
         else hipLaunchKernelGGL((k_pb_adjoint<false, 32, COUNT>), grid, dim3(64), 0, st, c, counts);
     } else {
@@ -1273,8 +1528,9 @@ static int launch_adjoint_items(const BeamCtx *ctx, int n_beams, const long long
 
 int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *row_off, long long total_rows,
                          const long long *item_off32, long long total_items32, const long long *item_off64,
-                         long long total_items64, const long long *reduce_off, long long total_reduce, bool cull,
-                         hipStream_t st, void *const *ev) {
+                         long long total_items64, const long long *item_off128, long long total_items128,
+                         const long long *reduce_off, long long total_reduce, bool cull, hipStream_t st,
+                         void *const *ev) {
     int rc;
     for (long long base = 0; base < total_rows; base += kMaxBlocks) {
         const long long n = total_rows - base < kMaxBlocks ? total_rows - base : kMaxBlocks;
@@ -1282,6 +1538,7 @@ int launch_adjoint_batch(const BeamCtx *ctx, int n_beams, const long long *row_o
         if ((rc = check_launch("k_brick_summary_batch"))) return rc;
     }
     mark(ev, 2 * DH_TIME_ADJOINT, st);
+    if ((rc = launch_adjoint_items<PACK_CS>(ctx, n_beams, item_off128, total_items128, cull, st))) return rc;
     if ((rc = launch_adjoint_items<64>(ctx, n_beams, item_off64, total_items64, cull, st))) return rc;
     if ((rc = launch_adjoint_items<32>(ctx, n_beams, item_off32, total_items32, cull, st))) return rc;
     mark(ev, 2 * DH_TIME_ADJOINT + 1, st);

@@ -239,7 +239,8 @@ typedef struct dh_batch_beam {
 } dh_batch_beam;
 
 /* Schedules, chosen per batch (e.g. by an autotuner):
- *   adj_cs          spots per adjoint workgroup: 32, 64, or 0 (chosen per beam from its layer sizes)
+ *   adj_cs          spots per adjoint workgroup: 32, 64, 128 (packed: up to 128 spots of a layer, each
+ *                   brick's reachable spots compacted onto the lanes), or 0 (= 128)
  *   adj_wg_target   workgroups per beam to aim for (bricks are cut into slabs to reach it)
  *   adj_min_bricks  fewest bricks per slab
  *   adj_max_slabs   most slabs per beam (<= 65535)
@@ -272,6 +273,7 @@ typedef struct dh_batch {
     int64_t n_sum_blocks;      /* 1024-voxel blocks of plans with several beams */
     int64_t n_partials;        /* adjoint partial records */
     int64_t n_adj32, n_adj64;  /* adjoint workgroups of 32- and 64-spot chunks */
+    int64_t n_adj128;          /* adjoint workgroups of packed chunks (up to 128 spots, lanes filled per brick) */
     int64_t n_reduce_blocks;   /* 64-spot reduction blocks */
     int64_t tab[16];           /* byte offsets of the device tables in ws; tab[13] = end of the tables
                                   uploaded at setup (the rest of ws is scratch the phases rewrite);

@@ -203,12 +203,14 @@ def test_batch_graph_replay(dev):
         _say("replay %d ok" % rep)
 
 
-@pytest.mark.parametrize("tune", [(32, 0, 0, 0), (64, 0, 0, 0), (0, 512, 16, 8), (32, 200000, 8, 4000),
-                                  (0, 0, 0, 0, 7, 3)])
-def test_batch_adjoint_schedules(dev, tune):
+@pytest.mark.parametrize("tune,cutoff", [((32, 0, 0, 0), 6.0), ((64, 0, 0, 0), 6.0), ((0, 512, 16, 8), 6.0),
+                                         ((32, 200000, 8, 4000), 6.0), ((0, 0, 0, 0, 7, 3), 6.0),
+                                         ((128, 0, 0, 0), 6.0), ((128, 512, 16, 8), 6.0), ((128, 0, 0, 0), 0.0),
+                                         ((64, 0, 0, 0), 0.0)])
+def test_batch_adjoint_schedules(dev, tune, cutoff):
     from DoseCUDA.batch import BatchedDose, make_tune
-    _say("adjoint schedule %s" % (tune,))
-    engines = _engines(dev)[:3]
+    _say("adjoint schedule %s cutoff %g" % (tune, cutoff))
+    engines = _engines(dev, cutoff=cutoff)[:3]
     grads = _grads(engines, seed=3)
     ref = BatchedDose(engines)
     ref.compute_wet()
@@ -319,7 +321,7 @@ def test_autotuner_picks_and_caches(dev, tmp_path, monkeypatch):
     bd = BatchedDose(engines, tune="auto", wet_mode="reach")
     t = tune_dict(bd.tune)
     _say("tuned %s" % t)
-    assert t["adj_cs"] in (32, 64) and t["fwd_blocks"] in A.FWD_BLOCKS and t["ray_blocks"] in A.RAY_BLOCKS
+    assert t["adj_cs"] in (32, 64, 128) and t["fwd_blocks"] in A.FWD_BLOCKS and t["ray_blocks"] in A.RAY_BLOCKS
     assert A.lookup(A.signature(engines, 6.0, "reach")) == t
 
     def boom(*a, **k):

@@ -293,7 +293,8 @@ def test_oblique_multibeam_end_to_end(dev, machine_dir):
 
 
 def test_adjoint_81_spot_layers(dev):
-    """Layers of 81 spots take the 32-spot / two-voxel-subset adjoint path."""
+    """Layers of 81 spots, oblique beam: the packed adjoint runs each brick's
+    reachable spots in passes with several voxel subsets per pass."""
     from DoseCUDA import device as D
     _say("adjoint, 81-spot layers, oblique beam")
     g = _phantom_small(shape=(40, 56, 48), spacing=2.5, seed=4)
