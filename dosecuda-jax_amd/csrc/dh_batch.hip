@@ -182,7 +182,7 @@ static bool plan_layout(const dh_batch_plan *plans, int n_plans, const dh_batch_
             w.nslab = 1;
             w.cs = 64;
             if (S > 0) {
-                w = carve_beam(at(off), nb, S, NL, tune);
+                w = carve_beam(at(off), nb, S, NL, tune, cutoff_sigma > 0.0f);
                 off += w.bytes;
             }
             BeamCtx &c = L.ctx[b];
@@ -276,9 +276,12 @@ using namespace dh;
 
 extern "C" int64_t dh_batch_workspace_size(const dh_batch_plan *plans, int32_t n_plans, const dh_batch_beam *beams,
                                            int32_t n_beams, const dh_batch_tune *tune) {
-    Layout L;
+    // the adjoint schedule (and so the partial records) depends on whether the
+    // cutoff culls, which this query does not know: the larger layout
+    Layout L, Lc;
     if (!plan_layout(plans, n_plans, beams, n_beams, tune_from(tune), 0.0f, nullptr, L)) return DH_ERR_ARG;
-    return L.bytes;
+    if (!plan_layout(plans, n_plans, beams, n_beams, tune_from(tune), 1.0f, nullptr, Lc)) return DH_ERR_ARG;
+    return L.bytes > Lc.bytes ? L.bytes : Lc.bytes;
 }
 
 extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32_t n_plans,

@@ -281,11 +281,11 @@ int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long 
                      long long total_bricks, long long total_blocks, bool use_need, int persist_blocks,
                      int smooth_box_floats, hipStream_t st, void *const *ev = nullptr);
 // dh_pencil.hip
-int adjoint_chunk(int n_spots, int n_layers);
+int adjoint_chunk(int n_spots, int n_layers, bool cull);
 int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t);
 AdjTune default_tune();
 bool tune_ok(const AdjTune &t);
-BeamWs carve_beam(void *base, long long nbricks, int S, int L, const AdjTune &t);
+BeamWs carve_beam(void *base, long long nbricks, int S, int L, const AdjTune &t, bool cull);
 bool fill_model(const dh_beam_model *bm, Model &m);
 bool pencil_grid_ok(const dh_grid &g);
 bool spot_set_ok(const dh_spot_set *ss);

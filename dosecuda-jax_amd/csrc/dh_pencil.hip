@@ -1266,11 +1266,22 @@ __global__ __launch_bounds__(256) void k_pb_reduce_batch(const BeamCtx *__restri
 static constexpr long long kAlign = 256;
 static long long align_up(long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-// Spots per adjoint workgroup when the schedule leaves it open: the packed
-// chunks, which fill the lanes per brick with the reachable spots (the
-// autotuner chose them over 32- and 64-spot chunks on the head-phantom, the
-// uneven-plan and the 1 mm body workloads).  32 and 64 stay selectable.
-int adjoint_chunk(int, int) { return PACK_CS; }
+// Spots per adjoint workgroup when the schedule leaves it open.  With culling,
+// the packed chunks, which fill the lanes per brick with the spots in reach
+// (the autotuner chose them over 32- and 64-spot chunks on the head-phantom,
+// the uneven-plan and the 1 mm body workloads).  Without culling every spot
+// reaches every brick, the fixed chunks fill their lanes as well and cost
+// less per brick: 64, or 32 (two voxel subsets per wave) when the average
+// layer is small enough that 64-spot chunks would leave many lanes idle.
+// Scores: lane use / (1 + per-brick overhead share), the overhead share
+// growing as chunks shrink.
+int adjoint_chunk(int n_spots, int n_layers, bool cull) {
+    if (cull) return PACK_CS;
+    if (n_layers <= 0) return 64;
+    const double n = (double)n_spots / n_layers;
+    const double u64 = n / (ceil(n / 64.0) * 64.0), u32 = n / (ceil(n / 32.0) * 32.0);
+    return (u32 / 1.38 > u64 / 1.23) ? 32 : 64;
+}
 
 // Slabs of bricks per spot chunk: enough (chunk, slab) workgroups (wg_target)
 // that the few heavy ones (chunks and slabs in the beam path) do not leave the
@@ -1302,10 +1313,10 @@ bool tune_ok(const AdjTune &t) {
 
 // Workspace of one beam: [SpotExt x S][LayerC x L][adjoint partials float4 x
 // nslab*S][brick summaries float2 x bricks], each part 256-byte aligned.
-BeamWs carve_beam(void *base, long long nbricks, int S, int L, const AdjTune &t) {
+BeamWs carve_beam(void *base, long long nbricks, int S, int L, const AdjTune &t, bool cull) {
     BeamWs w;
     char *p = reinterpret_cast<char *>(base);
-    w.cs = t.cs ? t.cs : adjoint_chunk(S, L);
+    w.cs = t.cs ? t.cs : adjoint_chunk(S, L, cull);
     w.nslab = adjoint_slabs(nbricks, S, L, w.cs, t);
     long long off = 0;
     w.ext = reinterpret_cast<SpotExt *>(p + off);
@@ -1574,7 +1585,7 @@ static bool common_args(const dh_grid &g, const dh_beam_geom *geom, const dh_spo
 static BeamCtx single_ctx(const dh_grid &g, const dh_beam_geom *geom, const Model &m, const dh_spot_set *ss,
                           void *ws, float cutoff_sigma) {
     BeamCtx c = {};
-    const BeamWs w = carve_beam(ws, grid_bricks(g), ss->n_spots, ss->n_layers, default_tune());
+    const BeamWs w = carve_beam(ws, grid_bricks(g), ss->n_spots, ss->n_layers, default_tune(), cutoff_sigma > 0.0f);
     fill_pencil_ctx(c, g, *geom, m, *ss, w, cutoff_sigma);
     return c;
 }
@@ -1593,7 +1604,10 @@ extern "C" int dh_spot_params(dh_spot *spots, const int64_t *order, const float 
 
 extern "C" int64_t dh_pb_workspace_size(int32_t n_spots, int32_t n_layers, dh_grid g) {
     if (n_spots <= 0) return 0;
-    return carve_beam(nullptr, grid_bricks(g), n_spots, n_layers, default_tune()).bytes;
+    // the schedule depends on the cutoff, which this query does not know: the larger layout
+    const long long a = carve_beam(nullptr, grid_bricks(g), n_spots, n_layers, default_tune(), true).bytes;
+    const long long b = carve_beam(nullptr, grid_bricks(g), n_spots, n_layers, default_tune(), false).bytes;
+    return a > b ? a : b;
 }
 
 extern "C" int dh_pb_forward(const float *wet, float *dose, dh_grid g, const dh_beam_geom *geom,

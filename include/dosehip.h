@@ -240,7 +240,8 @@ typedef struct dh_batch_beam {
 
 /* Schedules, chosen per batch (e.g. by an autotuner):
  *   adj_cs          spots per adjoint workgroup: 32, 64, 128 (packed: up to 128 spots of a layer, each
- *                   brick's reachable spots compacted onto the lanes), or 0 (= 128)
+ *                   brick's reachable spots compacted onto the lanes), or 0 (128 with a cutoff, else 32
+ *                   or 64 from the average layer size)
  *   adj_wg_target   workgroups per beam to aim for (bricks are cut into slabs to reach it)
  *   adj_min_bricks  fewest bricks per slab
  *   adj_max_slabs   most slabs per beam (<= 65535)

@@ -360,3 +360,28 @@ def test_reuse_prep_requires_prepared_records(dev):
     a = bd.adjoint(_grads(engines), reuse_prep=True)
     torch.cuda.synchronize()
     assert all(torch.isfinite(g).all() for g, _ in a)
+
+
+@pytest.mark.parametrize("cutoff", [6.0, 0.0])
+def test_batch_workspace_covers_both_schedules(dev, cutoff):
+    """The workspace size query does not know the cutoff, which picks the
+    adjoint chunking (packed 128-spot chunks when culling, 32 / 64 otherwise)
+    and with it the number of partial records.  On a 160^3 grid with 40 small
+    layers the two layouts differ; set-up must accept the queried size for
+    either cutoff, and the gradients must match the single-beam path."""
+    from DoseCUDA.batch import BatchedDose
+    from DoseCUDA.engine import IMPTDoseEngine
+    _say("workspace size vs schedule, cutoff %g" % cutoff)
+    g = _phantom((160, 160, 160), 2.0, 12)
+    eng = IMPTDoseEngine(g, _plan([0.0], [0.0], list(range(20, 60)), radius=4.0, pitch=3.0), cutoff_sigma=cutoff)
+    grads = _grads([eng], seed=8)
+    eng.compute_wet(force=True)
+    want = [(m.clone(), x.clone()) for m, x in eng.adjoint(grads[0])]
+    bd = BatchedDose([eng])
+    _say("items %s" % bd.item_counts())
+    bd.compute_wet(force=True)
+    got = bd.adjoint(grads)
+    torch.cuda.synchronize()
+    assert float(want[0][0].abs().max()) > 0.0
+    for (a0, a1), (b0, b1) in zip(got, want):
+        assert torch.equal(a0, b0) and torch.equal(a1, b1)
