@@ -30,7 +30,7 @@
 // Skipping work (cutoff_sigma > 0): a spot whose axis stays further than
 // cutoff_sigma halo sigmas from every voxel of a brick contributes less than
 // exp(-cutoff^2/2) of its peak there and is not evaluated.  Tests run
-// lane-parallel: the 64 lanes of a wave judge 64 candidates (layers, spots or
+// lane-parallel: the 64 lanes of a wave test 64 candidates (layers, spots or
 // bricks) at once and the wave then walks only the survivors.
 #include "dh_common.h"
 
@@ -328,10 +328,10 @@ __global__ __launch_bounds__(64) void k_layer_prep_batch(const BeamCtx *__restri
 
 // ---------------------------------------------------------------------------
 // Forward.  One wave per 4x4x4 brick, one voxel per lane; a lane owns its
-// voxel's sum (deterministic, no atomics).  The lanes first judge 64 layers at
+// voxel's sum (deterministic, no atomics).  The lanes first test 64 layers at
 // a time (reaches the brick's WET range, and with culling: some spot of the
 // layer within reach of the brick); the wave then walks the surviving layers.
-// Per layer, each 64-spot chunk is judged lane-parallel against the brick
+// Per layer, each 64-spot chunk is tested lane-parallel against the brick
 // (distance from the brick centre to the spot axis vs cutoff * max sigma +
 // brick radius, once with the halo sigma and once with the narrower primary
 // sigma) and the surviving spots are evaluated four at a time, their records
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(256) void k_wet_dilate_batch(const BeamCtx *__restr
 // ---------------------------------------------------------------------------
 // Adjoint.  Workgroup = one wave = one CS-spot chunk of one layer x one slab
 // of bricks.  Lane = (voxel subset, spot); a spot's three sums stay in
-// registers across the whole slab.  The lanes judge 64 bricks at a time
+// registers across the whole slab.  The lanes test 64 bricks at a time
 // (holds voxels with dL/dD != 0 within the layer's range, and with culling:
 // within reach of the chunk); per surviving brick the wave acts as lanes =
 // voxels to compute the voxel terms, compacts the voxels that matter into LDS,
