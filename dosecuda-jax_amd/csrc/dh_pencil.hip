@@ -157,14 +157,36 @@ __device__ __forceinline__ BrickInfo brick_info_at(const Geom &b, float3 vc, flo
     return B;
 }
 
+// Upper bound of |sigma_ms| from layer c's lookup row for lookups at WET in
+// [wlo, whi] (mm): the largest |sigma| of the rows lut_lookup interpolates
+// between there (on a uniform depth grid rows floor(q) and floor(q) + 1 of
+// q = (wet - depth0) / ddepth, clamped to the row), else over the whole row
+// (also when the range spans more than SMS_ROWS rows).  The lerp stays within
+// its two rows up to rounding, which the 1.001 margin of reach_skip covers.
+constexpr int SMS_ROWS = 24;
+__device__ __forceinline__ float sms_bound(const Model &m, const LayerC &c, float wlo, float whi) {
+    if (!m.uniform) return c.sms_max;
+    const float qlo = (wlo - m.depth0) * m.inv_ddepth, qhi = (whi - m.depth0) * m.inv_ddepth;
+    if (!(qhi - qlo < (float)SMS_ROWS)) return c.sms_max;
+    // clamped before truncation (positive: truncation is floor); qhi <= 0 reads row 0 only, rows 0..1
+    // are taken
+    const int i0 = (int)fminf(fmaxf(qlo, 0.0f), (float)(m.lut_len - 2));
+    const int i1 = (int)fminf(fmaxf(qhi, 0.0f), (float)(m.lut_len - 2)) + 1;
+    const float2 *row = gptr(m.idd_sigma, (long long)c.eid * m.lut_len);
+    float s = 0.0f;
+    for (int t = i0; t <= i1; ++t) s = fmaxf(s, fabsf(row[t].y));
+    return s;
+}
+
 // True when every spot of the reach record c is negligible for the whole
 // brick: the brick centre's distance to the nearest spot axis, less the brick
 // radius, exceeds cut * (an upper bound of the halo sigma over the brick).  The
 // bound takes sigma_air over the range of d = |src - v| - wet + 0.7 R80 (a
-// quadratic: ends and vertex) plus max |sigma_ms|, and the halo width without
-// its negative terms, over the brick's WET range.
-__device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, float ivx, float ivy, float cut,
-                                           float rbrick) {
+// quadratic: ends and vertex) plus max |sigma_ms| over the lookup rows of the
+// brick's WET range, and the halo width without its negative terms, over that
+// range.
+__device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, const Model &m, float ivx, float ivy,
+                                           float cut, float rbrick) {
     if (!(c.r80 >= 5.0f)) return false;
     const float fx = fmaf(-B.hc.z, ivx, 1.0f), fy = fmaf(-B.hc.z, ivy, 1.0f);
     const float ex = fmaxf(0.0f, fmaxf(c.xmin * fx - B.hc.x, B.hc.x - c.xmax * fx));
@@ -179,7 +201,7 @@ __device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, 
     const float dhi = B.dsrc + rbrick - B.wlo + d07;
     float sa = fmaxf(fabsf(fmaf(fmaf(c.A, dlo, c.B), dlo, c.C)), fabsf(fmaf(fmaf(c.A, dhi, c.B), dhi, c.C)));
     if (c.A != 0.0f && c.vx > dlo && c.vx < dhi) sa = fmaxf(sa, fabsf(fmaf(fmaf(c.A, c.vx, c.B), c.vx, c.C)));
-    const float st_ub = sa + c.sms_max;
+    const float st_ub = sa + sms_bound(m, c, B.wlo, whi);
     const float sh_ub = 1.001f * fast_sqrt(st_ub * st_ub + sn_ub * sn_ub);
     return dlb > cut * sh_ub;
 }
@@ -396,7 +418,7 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
         if (l0 + lane < c.n_layers) {
             const LayerC L = gld(lc, l0 + lane);
             lrel = L.ok && B.wlo <= L.lim;
-            if (CULL && lrel) lrel = !reach_skip(L, B, ivx, ivy, c.cut, c.rbrick);
+            if (CULL && lrel) lrel = !reach_skip(L, B, c.m, ivx, ivy, c.cut, c.rbrick);
         }
         unsigned long long lmask = __ballot(lrel);
         while (lmask) {
@@ -671,7 +693,7 @@ __device__ __forceinline__ void wet_need_lanes(const BeamCtx &c, long long brick
         for (int l = 0; l < c.n_layers; ++l) {
             if (!__ballot(!done)) break;
             const LayerC L = cld(c.lc, l);
-            if (!done && L.ok && !reach_skip(L, brick_info_at(c.b, vc, -1.0f, L.lim), ivx, ivy, c.cut, c.rbrick))
+            if (!done && L.ok && !reach_skip(L, brick_info_at(c.b, vc, -1.0f, L.lim), c.m, ivx, ivy, c.cut, c.rbrick))
                 need = done = true;
         }
     }
@@ -815,7 +837,7 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
             if (CULL && brel && !COUNT) {
                 int ti, tj, tk;
                 brick_of(c.bg, base + lane, ti, tj, tk);
-                brel = !reach_skip(R, brick_info_at(c.b, brick_centre(c.g, c.b, ti, tj, tk), ws.x, ws.y), ivx, ivy,
+                brel = !reach_skip(R, brick_info_at(c.b, brick_centre(c.g, c.b, ti, tj, tk), ws.x, ws.y), c.m, ivx, ivy,
                                    c.cut, c.rbrick);
             }
         }
@@ -1010,7 +1032,7 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
             if (CULL && brel && !COUNT) {
                 int ti, tj, tk;
                 brick_of(c.bg, base + lane, ti, tj, tk);
-                brel = !reach_skip(R, brick_info_at(c.b, brick_centre(c.g, c.b, ti, tj, tk), ws.x, ws.y), ivx, ivy,
+                brel = !reach_skip(R, brick_info_at(c.b, brick_centre(c.g, c.b, ti, tj, tk), ws.x, ws.y), c.m, ivx, ivy,
                                    c.cut, c.rbrick);
             }
         }
