@@ -124,7 +124,22 @@ def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=3, grad_doses=None, 
     """Search the schedules for this batch; returns (tune dict, timings).
     The adjoint candidates run on dL/dD = grad_doses, else grad_fn(plan doses)
     (the caller's loss gradient), else a fixed pseudo-random field over the
-    voxels with dose."""
+    voxels with dose.
+
+    The candidate batches write the engines' own WET volumes.  In "reach" mode
+    those then hold WET only where the current spots reach (1e30 elsewhere),
+    so every engine beam's cached WET is marked stale on return: a later
+    engine call recomputes the full WET instead of reading that volume."""
+    try:
+        return _search(engines, cutoff_sigma, wet_mode, reps, grad_doses, log, grad_fn)
+    finally:
+        if wet_mode == "reach":
+            for e in engines:
+                for b in e.beams:
+                    b.wet_valid = False
+
+
+def _search(engines, cutoff_sigma, wet_mode, reps, grad_doses, log, grad_fn):
     from .batch import BatchedDose, make_tune
     cutoff = float(engines[0].cutoff_sigma if cutoff_sigma is None else cutoff_sigma)
     say = log or (lambda m: None)

@@ -76,12 +76,15 @@ def proton_raytrace_cuda(model, vol, beam, gpu_id):
     return np.ascontiguousarray(out)
 
 
-def proton_spot_cuda(model, vol, wet, beam, gpu_id, cutoff_sigma=D.DEFAULT_CUTOFF_SIGMA):
+def proton_spot_cuda(model, vol, wet, beam, gpu_id, cutoff_sigma=0.0):
     """Pencil-beam dose of one beam from its WET volume (proton_spot,
     dosemodule.cu:243-374).  Not multiplied by the number of fractions.
 
-    ``cutoff_sigma`` (keyword, not in the reference) bounds the beam's-eye-view
-    reach of a spot; 0 evaluates every spot for every voxel."""
+    By default every spot is evaluated for every voxel with WET <= 1.1 R80, as
+    the reference's pencilBeamKernel does (IMPTClasses.cu:255-267).
+    ``cutoff_sigma`` (keyword, not in the reference) > 0 skips spot/voxel
+    pairs farther than that many halo sigmas apart (each skipped term is below
+    exp(-cutoff_sigma^2 / 2) of its spot's peak)."""
     vsadx = _getfloat(model, "VSADX")
     vsady = _getfloat(model, "VSADY")
     dvp = _getarray(model, "divergence_params", 2)

@@ -281,14 +281,35 @@ __global__ __launch_bounds__(256) void k_spot_prep(BeamCtx c) {
     spot_prep_one(c.spots_in, c.sp, s, c.b.ivsadx, c.b.ivsady);
 }
 
+// A layer record whose energy id and spot range the kernels can use.
+__device__ __forceinline__ bool layer_in_range(const dh_layer &L, int n_energies, int n_spots) {
+    return (unsigned)L.energy_id < (unsigned)n_energies && L.spot_start >= 0 && L.n_spots > 0 &&
+           L.spot_start <= n_spots - L.n_spots;
+}
+
 // One wave per layer.  A record the kernels cannot use safely (energy id
-// outside the tables, spot range outside the spot array) gets ok = 0 and is
-// skipped rather than read out of bounds.
+// outside the tables, spot range outside the spot array, or a spot range that
+// intersects an earlier in-range layer's) gets ok = 0 and is skipped rather
+// than read out of bounds.  The usable layers are then disjoint runs, so their
+// spot counts sum to at most n_spots: the adjoint's chunk grid (adjoint_chunks)
+// covers them, and no spot's partials are written by two layers.
 __device__ __forceinline__ void layer_prep_one(const BeamCtx &cx, int l, int lane) {
     const dh_layer L = gld(cx.layers_in, l);
     LayerC c;
-    c.ok = (unsigned)L.energy_id < (unsigned)cx.m.n_energies && L.spot_start >= 0 && L.n_spots > 0 &&
-           L.spot_start <= cx.n_spots - L.n_spots;
+    bool overlap = false;
+    if (layer_in_range(L, cx.m.n_energies, cx.n_spots)) {
+        for (int k0 = 0; k0 < l; k0 += WAVE) {
+            const int k = k0 + lane;
+            bool hit = false;
+            if (k < l) {
+                const dh_layer P = gld(cx.layers_in, k);
+                hit = layer_in_range(P, cx.m.n_energies, cx.n_spots) && P.spot_start < L.spot_start + L.n_spots &&
+                      L.spot_start < P.spot_start + P.n_spots;
+            }
+            overlap = overlap || __ballot(hit) != 0;
+        }
+    }
+    c.ok = layer_in_range(L, cx.m.n_energies, cx.n_spots) && !overlap;
     c.start = c.ok ? L.spot_start : 0;
     c.n = c.ok ? L.n_spots : 0;
     c.eid = c.ok ? L.energy_id : 0;

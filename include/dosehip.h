@@ -70,7 +70,12 @@ typedef struct dh_spot {
 } dh_spot;
 
 /* One energy layer = a run of equal energy ids in the sorted spot array
- * (Layer, IMPTClasses.cuh:17-26; built like importLayers, IMPTClasses.cu:37-67). */
+ * (Layer, IMPTClasses.cuh:17-26; built like importLayers, IMPTClasses.cu:37-67).
+ * The kernels use a layer only if its energy id is inside the model, its
+ * range [spot_start, spot_start + n_spots) is non-empty and inside the spot
+ * array, and that range intersects no earlier usable-range layer of the
+ * array; other layers add no dose, and spots in no usable layer get zero
+ * gradients (importLayers only builds disjoint ascending runs). */
 typedef struct dh_layer {
     int32_t spot_start;
     int32_t n_spots;

@@ -16,6 +16,7 @@
  *                 layers as compute_dose_pure impt_jax.py:602-682
  *   orc_pb_grad   analytic dL/dmu, dL/dx, dL/dy of orc_pb_dose (no reference
  *                 counterpart: the reference differentiates with jax.grad)
+ *   orc_halo      the nuclear-halo fit alone (test hook)
  *
  * Decisions where the JAX code leaves float-level behaviour open:
  *  - trilinear sampling with zero padding per corner (scipy 'grid-constant'),
@@ -288,16 +289,11 @@ typedef struct {
     int p_on, h_on;
 } vterm_t;
 
-/* per (voxel, layer) terms (impt_jax.py:445-496) */
-static void voxel_terms(const pb_t *P, int l, real wet_mm, real dsrc, vterm_t *T) {
-    const int eid = P->layers[3 * l + 2];
-    const real r80 = (real)P->r80[l];
-    real idd, sms;
-    lut_interp(P->depths + (int64_t)eid * P->lut_len, P->sigmas + (int64_t)eid * P->lut_len,
-               P->idds + (int64_t)eid * P->lut_len, P->lut_len, wet_mm, &idd, &sms);
-    const float *c = P->dvp + (int64_t)eid * P->dvp_len;
-    real d = dsrc - wet_mm + R(0.7) * r80;
-    real st = (real)c[2] * d * d + (real)c[3] * d + (real)c[4] + sms;
+/* nuclear-halo width sigma_n (mm) and weight u_n at WET wet_mm of an energy
+ * with range r80 (impt_jax.py:469-483; nuclearHalo IMPTClasses.cu:104-121):
+ * the depth clamped to [0.1, r80 - 0.1] (fmin(fmax()) order, so r80 < 0.2
+ * gives r80 - 0.1), sigma_n >= 0, 0 <= u_n <= 0.9 */
+static void halo_fit(real r80, real wet_mm, real *o_sn, real *o_un) {
     real w = wet_mm;
     if (w < R(0.1)) w = R(0.1);
     if (w > r80 - R(0.1)) w = r80 - R(0.1);
@@ -309,6 +305,33 @@ static void voxel_terms(const pb_t *P, int l, real wet_mm, real dsrc, vterm_t *T
     if (sn < 0) sn = 0;
     if (un < 0) un = 0;
     if (un > R(0.9)) un = R(0.9);
+    *o_sn = sn;
+    *o_un = un;
+}
+
+/* test hook: halo_fit at n depths */
+int orc_halo(double r80, const double *wet_mm, int64_t n, double *sn, double *un) {
+    for (int64_t t = 0; t < n; ++t) {
+        real a, b;
+        halo_fit(R(r80), R(wet_mm[t]), &a, &b);
+        sn[t] = (double)a;
+        un[t] = (double)b;
+    }
+    return 0;
+}
+
+/* per (voxel, layer) terms (impt_jax.py:445-496) */
+static void voxel_terms(const pb_t *P, int l, real wet_mm, real dsrc, vterm_t *T) {
+    const int eid = P->layers[3 * l + 2];
+    const real r80 = (real)P->r80[l];
+    real idd, sms;
+    lut_interp(P->depths + (int64_t)eid * P->lut_len, P->sigmas + (int64_t)eid * P->lut_len,
+               P->idds + (int64_t)eid * P->lut_len, P->lut_len, wet_mm, &idd, &sms);
+    const float *c = P->dvp + (int64_t)eid * P->dvp_len;
+    real d = dsrc - wet_mm + R(0.7) * r80;
+    real st = (real)c[2] * d * d + (real)c[3] * d + (real)c[4] + sms;
+    real sn, un;
+    halo_fit(r80, wet_mm, &sn, &un);
     real st2 = st * st;
     real sh = SQRT(st2 + sn * sn);
     real sh2 = sh * sh;

@@ -19,8 +19,12 @@ Gaussian normalisation, axis distance by cross product), independent brute
 force (pure-Python loops, scipy map_coordinates 'grid-constant'), LUT values
 straight from the CSV files, invariants (symmetry, linearity) and central
 finite differences for the gradient.  The halo-width / halo-weight fit
-(IMPTClasses.cu:104-121) has no independent source; it is pinned only by its
-clamps and by transcription.
+(IMPTClasses.cu:104-121) has no independent source for its coefficients
+(parity unpinned for them); what the mathematics fixes is pinned: finite and
+continuous over the whole clamp range of all 196 energies (its poles at
+w = R80 + 5 and w^2 = (R80 + 3)^2 outside it), the clamp plateaus, the
+collapse of the range for R80 < 0.2 mm, and the two-Gaussian normalisation
+with a non-zero halo weight.
 """
 
 import ctypes
@@ -52,7 +56,8 @@ def lib(precision="f64"):
         L.orc_wet_at.argtypes = [P, i, i, i, d, P, P, i64, P]
         L.orc_pb_dose.argtypes = [P, i, i, i, d, P, P, P, P, i, P, i, P, P, P, i, P, i64, P]
         L.orc_pb_grad.argtypes = [P, P, i, i, i, d, P, P, i, P, P, i, P, i, P, P, P, i, P, P]
-        for fn in (L.orc_raymarch, L.orc_smooth, L.orc_wet_at, L.orc_pb_dose, L.orc_pb_grad):
+        L.orc_halo.argtypes = [d, P, i64, P, P]
+        for fn in (L.orc_raymarch, L.orc_smooth, L.orc_wet_at, L.orc_pb_dose, L.orc_pb_grad, L.orc_halo):
             fn.restype = ctypes.c_int
         _LIBS[name] = L
     return _LIBS[name]
@@ -213,6 +218,16 @@ def pb_grad(wet_cm, grad_dose, spacing, geom, spots, layers, r80, marr, precisio
                       _p(marr.dvp), marr.dvp.shape[1], _p(marr.depths), _p(marr.sigmas), _p(marr.idds),
                       marr.depths.shape[1], _p(gmu), _p(gxy))
     return gmu, gxy
+
+
+def halo(r80, wet_mm, precision="f64"):
+    """(sigma_n [mm], u_n) of the nuclear-halo fit at the depths wet_mm (mm) for
+    an energy of range r80 (mm): the term voxel_terms uses (test hook)."""
+    w = np.ascontiguousarray(np.atleast_1d(wet_mm), dtype=np.float64)
+    sn = np.empty(w.size)
+    un = np.empty(w.size)
+    lib(precision).orc_halo(float(r80), _p(w), w.size, _p(sn), _p(un))
+    return sn, un
 
 
 # ---------------------------------------------------------------------------

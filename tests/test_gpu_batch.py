@@ -344,6 +344,29 @@ def test_autotuner_picks_and_caches(dev, tmp_path, monkeypatch):
             assert float((x1 - y1).abs().max()) <= 1e-5 * float(y1.abs().max())
 
 
+def test_autotuner_leaves_engine_wet_cache_valid(dev, tmp_path, monkeypatch):
+    """Tuning a reach-mode batch writes reach-limited WET (1e30 outside the
+    spots' reach) into the engines' own WET volumes; the engines must not
+    reuse it as their full WET afterwards.  compute_wet, tune, then engine
+    doses with spots moved 8 mm must equal a fresh engine's bit for bit."""
+    from DoseCUDA.batch import BatchedDose
+    _say("autotuner vs the engines' WET cache")
+    monkeypatch.setenv("DOSEHIP_TUNE_CACHE", str(tmp_path / "tune.json"))
+    engines = _engines(dev)[:3]
+    for e in engines:
+        e.compute_wet()
+    BatchedDose(engines, tune="auto", wet_mode="reach")
+    fresh = _engines(dev)[:3]
+    for e, f in zip(engines, fresh):
+        xys = [b.xy0 + 8.0 for b in e.beams]
+        got = e.dose(None, xys).clone()
+        want = f.dose(None, [b.xy0 + 8.0 for b in f.beams])
+        torch.cuda.synchronize()
+        assert torch.equal(got, want)
+        for b, c in zip(e.beams, f.beams):
+            assert torch.equal(b.wet, c.wet)
+
+
 def test_reuse_prep_requires_prepared_records(dev):
     """DH_BATCH_REUSE_PREP on a batch whose spot records no call has prepared is
     an argument error (raised before any kernel runs); after a forward it works."""

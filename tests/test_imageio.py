@@ -152,3 +152,54 @@ def test_dosegrid_nrrd_and_resampling(tmp_path):
     assert np.all(imageio.read_nrrd(str(tmp_path / "p_beam02.nrrd"))[0] == 0.5)
     with pytest.raises(Exception):
         g.writeDoseNRRD(str(tmp_path / "x.nrrd"), dose_type="RBE")
+
+
+def _nrrd_header_fields(path):
+    """Header of an attached-data NRRD file parsed field by field as pynrrd
+    reports it (numbers and vectors as floats), plus the data byte count."""
+    raw = open(path, "rb").read()
+    end = raw.index(b"\n\n")
+    fields = {}
+    for line in raw[:end].decode("ascii").split("\n")[1:]:
+        if line.startswith("#"):
+            continue
+        key, val = line.split(": ", 1)
+        fields[key] = val
+    out = {"type": fields["type"], "dimension": int(fields["dimension"]), "space": fields["space"],
+           "sizes": [int(v) for v in fields["sizes"].split()],
+           "space directions": [[float(x) for x in v.strip("()").split(",")]
+                                for v in fields["space directions"].split()],
+           "kinds": fields["kinds"].split(), "endian": fields["endian"], "encoding": fields["encoding"],
+           "space origin": [float(x) for x in fields["space origin"].strip("()").split(",")]}
+    return out, len(raw) - end - 2
+
+
+def test_phantom_example_dose_header_matches_reference_output(tmp_path):
+    """The reference's phantom example (tests/test_phantom_impt.py: default
+    cube phantom, writeDoseNRRD PHYSICAL) produced a dose file whose header
+    the reference's tests/plot_nrrd.ipynb:61-64 stores as read back by pynrrd:
+    type float, dimension 3, space left-posterior-superior, sizes 138 138 138,
+    space directions 3 mm on the diagonal, kinds domain x 3, endian little,
+    encoding raw, space origin (-207, -207, -207).  The header depends only on
+    the grid, so a zero dose stands in for the GPU result here; the CT file
+    of the same example has the same grid fields."""
+    from DoseCUDA import IMPTDoseGrid
+    g = IMPTDoseGrid()
+    g.createCubePhantom()
+    g.dose = np.zeros(g.HU.shape, np.float32)
+    g.beam_doses = [g.dose]
+    expected = {"type": "float", "dimension": 3, "space": "left-posterior-superior", "sizes": [138, 138, 138],
+                "space directions": [[3.0, 0.0, 0.0], [0.0, 3.0, 0.0], [0.0, 0.0, 3.0]],
+                "kinds": ["domain", "domain", "domain"], "endian": "little", "encoding": "raw",
+                "space origin": [-207.0, -207.0, -207.0]}
+    dose_path = str(tmp_path / "cube_impt_dose.nrrd")
+    g.writeDoseNRRD(dose_path, individual_beams=False, dose_type="PHYSICAL")
+    hdr, nbytes = _nrrd_header_fields(dose_path)
+    assert hdr == expected
+    assert nbytes == 138 ** 3 * 4
+    ct_path = str(tmp_path / "cube_phantom_ct.nrrd")
+    g.writeCTNRRD(ct_path)
+    hdr, nbytes = _nrrd_header_fields(ct_path)
+    assert hdr == expected and nbytes == 138 ** 3 * 4
+    g.writeDoseNRRD(str(tmp_path / "d.nrrd"), individual_beams=True, dose_type="PHYSICAL")
+    assert _nrrd_header_fields(str(tmp_path / "d_beam01.nrrd"))[0] == expected

@@ -14,7 +14,13 @@
   autograd of an independent vectorised restatement.
 
 The halo fit (IMPTClasses.cu:104-121) has no source independent of its
-transcription; its clamps (0 <= weight <= 0.9, sigma >= 0) are checked.
+transcription, so its coefficients stay unpinned; what the mathematics fixes
+is checked: its clamps (0 <= weight <= 0.9, sigma >= 0), that it is finite and
+continuous over the whole clamp range [0.1, R80 - 0.1] of all 196 energies
+(the poles at w = R80 + 5 and w^2 = (R80 + 3)^2 lie outside it, so a sign
+slip that moves one inside fails), the plateaus outside the range, the
+collapse for R80 < 0.2 mm onto the continuation of the range end, and the
+planar normalisation (1 - u) + u = 1 at depths where the halo weight u > 0.
 """
 
 import math
@@ -252,6 +258,106 @@ def test_pb_halo_clamps(plan_models):
             _, spots, layers, r80 = O.spots_and_layers(sl, model.divergence_params)
             d = O.pb_dose(wet, 2.0, g, spots, layers, r80, marr)
             assert np.all(np.isfinite(d)) and np.all(d >= 0)
+
+
+def _energy_ranges(plan_models):
+    for model in plan_models.beam_models:
+        for e, r80 in enumerate(model.divergence_params[:, 1].astype(np.float64)):
+            yield model, e, float(r80)
+
+
+def test_halo_fit_finite_and_continuous_on_clamp_range(plan_models):
+    """On [0.1, R80 - 0.1] both fits are finite and bounded, and their largest
+    step between neighbouring depths shrinks in proportion to the grid step
+    (Lipschitz): a pole inside the range (e.g. (w - R80 + 5)^2 for
+    (w - R80 - 5)^2) blows the values up or stops the steps from shrinking."""
+    n_checked = 0
+    for _, e, r80 in _energy_ranges(plan_models):
+        if r80 < 0.2:
+            continue
+        n_checked += 1
+        coarse = np.linspace(0.1, r80 - 0.1, 401)
+        fine = np.linspace(0.1, r80 - 0.1, 4001)
+        sn_c, un_c = O.halo(r80, coarse)
+        sn_f, un_f = O.halo(r80, fine)
+        for v in (sn_c, un_c, sn_f, un_f):
+            assert np.all(np.isfinite(v))
+        assert sn_f.min() >= 0.0 and sn_f.max() < 40.0          # halo widths of millimetres to centimetres
+        assert un_f.min() >= 0.0 and un_f.max() <= 0.9
+        for c, f in ((sn_c, sn_f), (un_c, un_f)):
+            jc = np.abs(np.diff(c)).max()
+            jf = np.abs(np.diff(f)).max()
+            assert jf <= 0.15 * jc + 1e-12, (e, r80, jc, jf)   # step ratio 10: jumps shrink ~10x
+    assert n_checked == 195                                    # every energy but RS4's R80 = -0.65 mm
+
+
+def test_halo_fit_clamp_plateaus(plan_models):
+    """Depths below 0.1 mm give the value at 0.1 mm and depths beyond R80 - 0.1
+    the value at R80 - 0.1, exactly (the depth is clamped before the fit)."""
+    for _, e, r80 in _energy_ranges(plan_models):
+        if r80 < 0.2:
+            continue
+        lo = np.array([-5.0, 0.0, 0.05, 0.1])
+        hi = np.array([r80 - 0.1, r80, 1.1 * r80, 2.0 * r80 + 50.0])
+        for w in (lo, hi):
+            sn, un = O.halo(r80, w)
+            assert np.all(sn == sn[-1 if w is lo else 0]) and np.all(un == un[-1 if w is lo else 0])
+
+
+def test_halo_fit_collapsed_range(plan_models):
+    """RS4 energy 0 has R80 = -0.65 mm: the clamp [0.1, R80 - 0.1] is empty and
+    the depth collapses to R80 - 0.1 (fmin(fmax(w, 0.1), R80 - 0.1), as
+    jnp.clip): the fit is constant in the depth and equals the continuation of
+    its value at the end of the range, h(R) = fit(R - 0.1; R), extrapolated
+    smoothly from R in [0.2, 3] mm (the other clamp order would give
+    fit(0.1; R), a halo width about 0.05 mm larger)."""
+    rs4 = plan_models.beam_models[1]
+    r80 = float(rs4.divergence_params[0, 1])
+    assert r80 < 0.0
+    sn, un = O.halo(r80, np.linspace(-5.0, 400.0, 50))
+    assert np.all(sn == sn[0]) and np.all(un == un[0]) and np.isfinite(sn[0]) and np.isfinite(un[0])
+    assert 0.0 <= un[0] <= 0.9
+    # sigma_n stays positive near the range end, so its continuation is smooth
+    # (u_n sits on its clamp at 0 there, a kink, and is only range-checked)
+    rr = np.linspace(0.2, 3.0, 40)
+    end = np.array([O.halo(R, [R - 0.1])[0][0] for R in rr])      # sigma_n at the range end
+    start = np.array([O.halo(R, [0.1])[0][0] for R in rr])        # the other order's continuation
+    assert end.min() > 0.5
+    want = np.polyval(np.polyfit(rr, end, 5), r80)
+    other = np.polyval(np.polyfit(rr, start, 5), r80)
+    assert abs(want - sn[0]) < 2e-5, (want, sn[0])
+    assert abs(other - sn[0]) > 1e-2, (other, sn[0])               # the check tells the two orders apart
+
+
+@pytest.mark.parametrize("mi,eid,depths_cm", [(0, 10, (2.0, 4.5)), (0, 40, (5.0, 9.0)), (0, 80, (6.0, 15.0)),
+                                              (1, 30, (1.5, 3.0))])
+def test_pb_planar_normalisation_with_halo(plan_models, mi, eid, depths_cm):
+    """At depths where the halo weight u > 0 the primary and halo Gaussians
+    carry (1 - u) and u of the integral depth dose: one spot, WET constant
+    per beam's-eye-view plane, plane sum of D s^2 = mu * IDD(wet).  The
+    energy's sigma_air is made constant (A = B = 0, C = 4 mm above the row's
+    most negative sigma_ms): with the real coefficients sigma_air follows each
+    voxel's distance to the source, which grows off axis and widens the
+    Gaussians slightly (+3.9e-4 of the plane sum with the range shifter)."""
+    model = plan_models.beam_models[mi]
+    marr = O.ModelArrays(model)
+    marr.dvp = marr.dvp.copy()
+    marr.dvp[eid, 2:5] = (0.0, 0.0, 4.0 - min(0.0, float(marr.sigmas[eid].min())))
+    r80 = float(model.divergence_params[eid, 1])
+    _, un = O.halo(r80, np.array(depths_cm) * 10.0)
+    assert np.all(un > 0.02), un
+    n = (181, len(depths_cm), 181)
+    s = 1.0
+    wet = np.empty(n)
+    wet[:] = np.array(depths_cm)[None, :, None]
+    iso_adj = _centered_grid(n, s)
+    g = O.beam_geometry(np.zeros(3), -iso_adj, 0.0, 0.0, model.VSADX, model.VSADY)
+    sl = np.array([[0.0, 0.0, 1.3, eid], [0.0, 0.0, 0.0, eid]], np.float32)
+    _, spots, layers, r80s = O.spots_and_layers(sl, model.divergence_params)
+    dose = O.pb_dose(wet, s, g, spots, layers, r80s, marr)
+    for j, dcm in enumerate(depths_cm):
+        idd = np.interp(dcm * 10.0, marr.depths[eid], marr.idds[eid])
+        assert abs(dose[:, j, :].sum() * s * s / (1.3 * idd) - 1.0) < 2e-5
 
 
 # ---------------------------------------------------------------------------
