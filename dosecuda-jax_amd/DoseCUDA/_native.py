@@ -114,6 +114,9 @@ SIGNATURES = {
     "dh_pb_adjoint": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom),
                                      ctypes.POINTER(dh_beam_model), ctypes.POINTER(dh_spot_set),
                                      ctypes.c_float, _P, _P, _P, ctypes.c_int64, _P]),
+    "dh_pb_adjoint_noalloc": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom),
+                                             ctypes.POINTER(dh_beam_model), ctypes.POINTER(dh_spot_set),
+                                             ctypes.c_float, _P, _P, _P]),
     "dh_pb_count_pairs": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_beam_geom),
                                          ctypes.POINTER(dh_beam_model), ctypes.POINTER(dh_spot_set),
                                          ctypes.c_float, _P, _P, ctypes.c_int64, _P]),

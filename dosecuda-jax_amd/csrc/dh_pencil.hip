@@ -36,6 +36,8 @@
 
 #include <math.h>
 
+#include <mutex>
+
 namespace dh {
 
 constexpr float kLog2e = 1.4426950408889634f;
@@ -1719,4 +1721,103 @@ extern "C" int dh_pb_count_pairs(const float *wet, const float *grad_dose, dh_gr
     if ((rc = launch_forward<true>(st, c, 0, counts))) return rc;
     if (!grad_dose) return DH_OK;
     return launch_adjoint<true>(st, c, counts);
+}
+
+// ---------------------------------------------------------------------------
+// dh_pb_adjoint without a workspace argument, for callers that cannot
+// allocate device memory.  The beam's workspace (prepared spot and layer
+// records, adjoint partial sums, brick summaries) lives in `scratch`, one
+// array per device for the life of the process.  Nothing in it needs to start
+// at zero: every part is written by an earlier kernel of the same call before
+// any kernel reads it (prepare, k_brick_summary, every adjoint workgroup of a
+// usable layer writes its partial rows, k_pb_reduce reads only rows of spots
+// in usable layers).  Calls that overlap on one device share the array and
+// race.
+static __device__ __align__(256) unsigned char scratch[DH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept; nullptr on failure (error set).
+void *scratch_address() {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("dh_pb_adjoint_noalloc: no usable current device (%s, device %d)", hipGetErrorString(e), dev);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("dh_pb_adjoint_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" int dh_pb_adjoint_noalloc(const float *wet, const float *grad_dose, dh_grid g, const dh_beam_geom *geom,
+                                     const dh_beam_model *model, const dh_spot_set *spots, float cutoff_sigma,
+                                     float *grad_mu, float *grad_xy, void *stream) {
+    if (!pencil_grid_ok(g)) return DH_ERR_ARG;
+    if (!geom) {
+        set_error("null beam geometry");
+        return DH_ERR_ARG;
+    }
+    if (!spot_set_ok(spots)) return DH_ERR_ARG;
+    const int S = spots->n_spots;
+    if (S == 0) return DH_OK;
+    if (!wet || !grad_dose || !grad_mu || !grad_xy) {
+        set_error("dh_pb_adjoint_noalloc: null pointer");
+        return DH_ERR_ARG;
+    }
+    Model m;
+    if (!fill_model(model, m)) return DH_ERR_ARG;
+    // dh_pb_adjoint's schedule when its layout fits the array; otherwise the
+    // most slabs that fit (the partial sums then add in another order)
+    const long long nbricks = grid_bricks(g);
+    const bool cull = cutoff_sigma > 0.0f;
+    AdjTune t = default_tune();
+    BeamWs w = carve_beam(nullptr, nbricks, S, spots->n_layers, t, cull);
+    if (w.bytes > DH_NOALLOC_SCRATCH_BYTES) {
+        const long long row = (long long)S * 16LL;
+        const long long fixed = w.bytes - (long long)w.nslab * row;  // upper bound of the slab-free parts
+        const long long fit = (DH_NOALLOC_SCRATCH_BYTES - fixed - kAlign) / row;
+        if (fit >= 1) {
+            t.max_slabs = (int)(fit < t.max_slabs ? fit : t.max_slabs);
+            w = carve_beam(nullptr, nbricks, S, spots->n_layers, t, cull);
+        }
+        if (fit < 1 || w.bytes > DH_NOALLOC_SCRATCH_BYTES) {
+            set_error("dh_pb_adjoint_noalloc: %d spots in %d layers on a %d x %d x %d grid need %lld bytes of "
+                      "scratch with one slab, more than the %lld bytes the library holds per device; use "
+                      "dh_pb_adjoint with a workspace",
+                      S, spots->n_layers, g.nx, g.ny, g.nz, fixed + row, (long long)DH_NOALLOC_SCRATCH_BYTES);
+            return DH_ERR_ARG;
+        }
+    }
+    void *ws = scratch_address();
+    if (!ws) return DH_ERR_HIP;
+    w = carve_beam(ws, nbricks, S, spots->n_layers, t, cull);
+    hipStream_t st = (hipStream_t)stream;
+    BeamCtx c = {};
+    fill_pencil_ctx(c, g, *geom, m, *spots, w, cutoff_sigma);
+    c.wet = const_cast<float *>(wet);
+    c.gdose = grad_dose;
+    c.grad_mu = grad_mu;
+    c.grad_xy = grad_xy;
+    int rc = prepare(st, c);
+    if (rc) return rc;
+    if ((rc = launch_adjoint<false>(st, c, nullptr))) return rc;
+    hipLaunchKernelGGL(k_pb_reduce, dim3((S + 63) / 64), dim3(256), 0, st, c);
+    return check_launch("k_pb_reduce");
 }

@@ -176,6 +176,26 @@ int dh_pb_adjoint(const float *wet, const float *grad_dose, dh_grid g,
                   float *grad_mu, float *grad_xy,
                   void *workspace, int64_t workspace_bytes, void *stream);
 
+/* dh_pb_adjoint without a workspace argument.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It allocates nothing: the beam's workspace (spot and
+ * layer records, adjoint partial sums, brick summaries) lives in an array of
+ * DH_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs on
+ * until the process ends.  Same kernels and results as dh_pb_adjoint (bit for
+ * bit when dh_pb_adjoint's layout fits the array; otherwise fewer brick slabs,
+ * so the partial sums add in another order).  Returns DH_ERR_ARG, with a
+ * message, when even one slab does not fit (about
+ * 48 * n_spots + 64 * n_layers + 8 * bricks bytes, bricks = grid voxels / 64).
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (other streams, other threads) race and must be serialised by
+ * the caller. */
+#define DH_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int dh_pb_adjoint_noalloc(const float *wet, const float *grad_dose, dh_grid g,
+                          const dh_beam_geom *geom, const dh_beam_model *model,
+                          const dh_spot_set *spots, float cutoff_sigma,
+                          float *grad_mu, float *grad_xy, void *stream);
+
 /* Work accounting for dh_pb_forward / dh_pb_adjoint with the same arguments
  * (no dose or gradient is computed).  Adds to the DEVICE array counts[4]:
  *   counts[0]  forward pairs the reference evaluates: voxels with
