@@ -16,7 +16,11 @@ dL/d(x, y)) for every beam of every plan.
 
 Plain path (per plan, for the ratio): per beam eager calls of full WET
 (ray-march + smoothing), dense forward (every spot for every in-range voxel,
-cutoff 0) summed over beams, the same dL/dD, dense adjoint per beam.
+cutoff 0) summed over beams, the same dL/dD, dense adjoint per beam.  Two
+more lines separate the gains: the same eager per-beam path with the 6-sigma
+cutoff (culling alone), and one plan on the batched graph path (K = 1:
+batching of its beams, graph replay and reach-limited WET, without other
+plans to share the launches).
 
 value = spot.voxel evaluations per second counted as the reference does: per
 plan, every spot of a layer for every voxel with WET <= 1.1 R80 (forward) plus
@@ -129,8 +133,10 @@ def cpu_baseline(g, plan, eng, gvol_np, n_sample, seed=0):
         wet_s.reshape(-1)[idx] = wet_full.reshape(-1)[idx]
         gs = np.zeros_like(wet_full)
         gs.reshape(-1)[idx] = gvol_np.reshape(-1)[idx]
+        raw_full = b.wet_raw.detach().cpu().numpy() if b.wet_raw is not None else wet_full
         t0 = time.perf_counter()
         O.raymarch(rlsp, s, geom, idx=idx, precision="f32")
+        O.smooth(raw_full, s, geom, idx=idx, precision="f32")
         O.pb_dose(wet_s, s, geom, spots, layers, r80, marr, idx=idx, precision="f32")
         t1 = time.perf_counter()
         O.pb_grad(wet_s, gs, s, geom, spots, layers, r80, marr, precision="f32")
@@ -142,10 +148,24 @@ def cpu_baseline(g, plan, eng, gvol_np, n_sample, seed=0):
             inr = w_mm <= 1.1 * R
             pairs += int(inr.sum()) * int(ns) + int((inr & gnz).sum()) * int(ns)
         log("cpu baseline beam %d: fwd %.2fs adj %.2fs" % (k, t1 - t0, t2 - t1))
+    model_name = cpu_model()
     return {"value": pairs / t_total, "unit": "spot·voxel evals/s", "cores": threads, "kind": "port",
+            "cpu_model": model_name,
             "sample": "%d random voxels of the head phantom (plan copy 0), 3 beams: ray-march of the sampled "
-                      "voxels + forward dose + adjoint at them (float32 oracle build, OpenMP, dense: every "
-                      "spot); smoothing omitted; %d spot.voxel pairs in %.1f s" % (len(idx), pairs, t_total)}
+                      "voxels, smoothing of them (54 neighbour reads each), forward dose and adjoint at them "
+                      "(float32 oracle build, OpenMP on %d threads of %s, dense: every spot); %d spot.voxel pairs "
+                      "in %.1f s" % (len(idx), threads, model_name, pairs, t_total)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def load_traffic(kernel):
@@ -442,6 +462,11 @@ def main():
     nb = len(engines[0].beams)
     n_spots = sum(b.spots.n_spots for b in engines[0].beams)
 
+    # raw-WET volumes of the engines' own (rather than workspace scratch), so the voxels the reach pass
+    # marches can be counted
+    for e in engines:
+        for b in e.beams:
+            b.wet_raw = torch.zeros(e.shape, dtype=torch.float32, device=dev)
     t_tune = time.perf_counter()
     bd = BatchedDose(engines, wet_mode="reach", tune=None if args.no_tune else "auto",
                      tune_grad_fn=lambda doses: [lgrad(d) for d in doses], tune_grad_key="head-loss")
@@ -499,6 +524,33 @@ def main():
     kms = {N.TIMED_KERNELS[k]: float(np.mean(v)) for k, v in kt.items()}
     ms_step = ms_graph if ms_graph is not None else ms_eager
 
+    # voxels the reach-limited ray-march computes per step: raw WET poisoned with NaN, one step, count
+    for e in engines:
+        for b in e.beams:
+            b.wet_raw.fill_(float("nan"))
+    step()
+    torch.cuda.synchronize()
+    marched = int(sum(int(torch.isfinite(b.wet_raw).sum().item()) for e in engines for b in e.beams))
+    log("reach pass marched %d of %d voxel-beams (%.1f%%)" % (marched, nvox * nb * K, 100.0 * marched / (nvox * nb * K)))
+
+    # K = 1: one plan on the batched graph path (its own tuned schedule)
+    k1_ms = None
+    if not args.eager_only:
+        b1 = BatchedDose([engines[0]], wet_mode="reach", tune=None if args.no_tune else "auto",
+                         tune_grad_fn=lambda doses: [lgrad(d) for d in doses], tune_grad_key="head-loss")
+        mus1 = [b.mu0.clone() for b in engines[0].beams]
+        xys1 = [b.xy0.clone() for b in engines[0].beams]
+
+        def step1():
+            doses = b1.forward(mus1, xys1)
+            lgrad(doses[0], out=b1.grad_doses[0])
+            return doses, b1.adjoint(None, reuse_prep=True, input_order=False)
+
+        log("timing %d graph replays of one plan (K = 1)" % args.steps)
+        k1_ms, gs1 = _graph_ms(step1, dev, args.warmup, args.steps)
+        del gs1, b1
+        torch.cuda.empty_cache()
+
     # work counts per plan (plan copy 0): evaluated pairs with the culled path, reference pairs with the
     # plain path's dL/dD; both on the full WET (the dose kernels see the same values where they evaluate)
     e0 = engines[0]
@@ -514,28 +566,34 @@ def main():
     plain_ms = None
     gd_dense = None
 
-    def plain_step():
+    def plain_step(cutoff=0.0):
         for b in e0.beams:
             b.wet_raw, b.wet = D.wet_from_padded(e0.rlsp_padded, e0.grid, b.geo, dev, raw_out=b.wet_raw, out=b.wet)
         dose = torch.empty(e0.shape, dtype=torch.float32, device=dev)
         for k, b in enumerate(e0.beams):
             b.spots.set_weights_positions(b.mu0, b.xy0)
-            D.pb_forward(b.wet, dose, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0,
+            D.pb_forward(b.wet, dose, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=cutoff,
                          accumulate=k > 0)
         gd = lgrad(dose)
         for b in e0.beams:
-            D.pb_adjoint(b.wet, gd, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=0.0)
+            D.pb_adjoint(b.wet, gd, e0.grid, b.geo, b.dmodel, b.spots, b.ws, b.ws_bytes, cutoff_sigma=cutoff)
         return dose, gd
 
-    if args.plain_steps > 0:
-        log("plain path %d steps" % args.plain_steps)
-        _, gd_dense = plain_step()
+    def plain_time(cutoff):
+        plain_step(cutoff)
         torch.cuda.synchronize()
         tp = time.perf_counter()
         for _ in range(args.plain_steps):
-            _, gd_dense = plain_step()
+            out = plain_step(cutoff)
         torch.cuda.synchronize()
-        plain_ms = (time.perf_counter() - tp) / args.plain_steps * 1e3
+        return (time.perf_counter() - tp) / args.plain_steps * 1e3, out
+
+    plain_culled_ms = None
+    if args.plain_steps > 0:
+        log("plain path, culled (cutoff %g), %d steps" % (args.cutoff_sigma, args.plain_steps))
+        plain_culled_ms, _ = plain_time(args.cutoff_sigma)
+        log("plain path, dense, %d steps" % args.plain_steps)
+        plain_ms, (_, gd_dense) = plain_time(0.0)
     else:
         e0.compute_wet(force=True)
         dose_d = e0.dose()
@@ -572,18 +630,18 @@ def main():
                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": load_traffic(dom_name),
                     "avg_launch_ms": kms[dom_name], "flop_per_pair": fpp, "pairs_per_launch": pairs}
     else:
-        # WET kernels: compulsory HBM bytes = quad RLSP read (16 B per voxel) + raw WET write (4 B) per beam
-        # for the ray-march; raw read + smoothed write (8 B) per beam for smoothing; all voxels as bound
-        per_vox = 20.0 if dom_name == "k_wet_raymarch_batch" else 8.0
-        byts = per_vox * nvox * nb * K
+        # WET kernels, algorithmic bytes per voxel and beam: ray-march 8 (RLSP read 4 + raw WET write 4) over
+        # the voxels the reach pass marched; smoothing 8 (raw WET read + smoothed write) over every voxel
+        if dom_name == "k_wet_raymarch_batch":
+            units, per_vox, what = marched, 8.0, "marched voxel-beams (reach pass)"
+        else:
+            units, per_vox, what = nvox * nb * K, 8.0, "voxel-beams"
+        byts = per_vox * units
         achieved = byts / (kms[dom_name] * 1e-3) / 1e9
         roofline = {"kernel": dom_name, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(dom_name),
-                    "avg_launch_ms": kms[dom_name], "bytes_per_launch": byts}
-        if dom_name == "k_wet_raymarch_batch":
-            # measured with rocprofv3 counters (TA_TA_BUSY, TCP_TOTAL_CACHE_ACCESSES), not from this run
-            roofline["limiter"] = ("L1 tag lookups of the 16-byte trilinear gathers: TA busy 90-100% of cycles at "
-                                   "about one 64-byte segment per clock per CU; HBM is not the bound")
+                    "avg_launch_ms": kms[dom_name], "bytes_per_launch": byts, "bytes_per_unit": per_vox,
+                    "units_per_launch": units, "unit_kind": what}
     kernels = {}
     for name, ms in kms.items():
         r = {"avg_ms_per_step": ms}
@@ -622,6 +680,11 @@ def main():
         },
         "ms_per_plan": ms_plan,
         "plain_ms_per_plan": plain_ms,
+        "plain_culled_ms_per_plan": plain_culled_ms,
+        "batched_k1_ms_per_plan": k1_ms,
+        "speedup_vs_plain_culled_per_plan": (plain_culled_ms / ms_plan) if plain_culled_ms else None,
+        "speedup_k1_vs_plain_culled": (plain_culled_ms / k1_ms) if (plain_culled_ms and k1_ms) else None,
+        "marched_voxel_beams_per_step": marched,
         "dropin_forward_ms_per_plan_host_arrays": dropin_ms,
         "speedup_vs_plain_per_plan": (plain_ms / ms_plan) if plain_ms else None,
         "eager_ms_per_step": ms_eager,
