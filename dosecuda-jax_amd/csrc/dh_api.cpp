@@ -44,8 +44,9 @@ extern "C" int dh_last_error(char *buf, size_t len) {
     return (int)m.size();
 }
 
-// Quad RLSP volume plus its table of uniform 4x4x4 cell blocks.
+// Quad RLSP volume plus its table of 4x4x4 cell blocks (value and run radius)
+// and two ints of scratch per block.
 extern "C" int64_t dh_padded_len(dh_grid g) {
     return (int64_t)4 * g.nx * g.ny * (g.nz + 1) +
-           (int64_t)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
+           (int64_t)4 * ((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
 }

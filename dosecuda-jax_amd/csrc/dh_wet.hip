@@ -18,13 +18,17 @@
 // sample instead of four 8-byte ones halve the former and the bilinear form
 // trims the latter (8 instead of 14 operations per sample).
 //
-// Block table (after the quads): one float per 4x4x4 block of trilinear cells
-// (cells with floor(c) in [4b, 4b + 3] per axis): the common value when every
-// corner those cells touch (voxels 4b .. 4b + 4, zero past the high end) holds
-// the same rho, NaN otherwise.  Every sample whose cell lies in a uniform block
-// is exactly that value, so the march adds a run of such samples at once
-// (empty-space skipping; air around the patient is one value because the HU table clamps
-// at -1000 HU).
+// Block table (after the quads): one float2 per 4x4x4 block of trilinear cells
+// (cells with floor(c) in [4b, 4b + 3] per axis).  x: the common value when
+// every corner those cells touch (voxels 4b .. 4b + 4, zero past the high end)
+// holds the same rho, NaN otherwise.  y: for a uniform block, its run radius r
+// (0 .. RUN_RMAX, as a float): every block within Chebyshev distance r of it
+// that lies in the grid is uniform with the same value, so all cells of that
+// cube of (2r + 1)^3 blocks have that value.  Every sample whose cell lies in
+// the cube is exactly the value, so the march adds the run of samples that
+// stays in the cube at once (empty-space skipping; air around the patient is
+// one value because the HU table clamps at -1000 HU).  The table is followed by
+// one int per block of scratch for the radius passes.
 #include <algorithm>
 #include "dh_common.h"
 
@@ -36,11 +40,14 @@ __host__ __device__ inline long long block_count(int nx, int ny, int nz) {
     return (long long)((nx + 3) / 4) * ((ny + 3) / 4) * ((nz + 3) / 4);
 }
 
+// Largest run radius (in blocks) the table records.
+constexpr int RUN_RMAX = 15;
+
 // One thread per block of cells: compare every corner the block's cells touch
 // (voxels 4b .. min(4b + 4, n) per axis; rho(i, j, k) is the first component
 // of quad (i, j, k), zero at i = nx or j = ny, and quad (i, j, nz) is zero)
-// with the first one.
-__global__ __launch_bounds__(256) void k_rlsp_blocks(const float4 *__restrict__ q, float *__restrict__ table, Grid g,
+// with the first one.  The radius is filled in by k_block_radius.
+__global__ __launch_bounds__(256) void k_rlsp_blocks(const float4 *__restrict__ q, float2 *__restrict__ table, Grid g,
                                                     long long nblk) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nblk) return;
@@ -59,7 +66,47 @@ __global__ __launch_bounds__(256) void k_rlsp_blocks(const float4 *__restrict__ 
                 const float e = (i < g.nx && j < g.ny) ? q[i + (long long)g.nx * (j + (long long)g.ny * k)].x : 0.0f;
                 same = e == v;
             }
-    table[t] = same ? v : __int_as_float(0x7fc00000);
+    table[t] = make_float2(same ? v : __int_as_float(0x7fc00000), 0.0f);
+}
+
+// Run radii by three separable passes of the Chebyshev distance to the nearest
+// "obstacle" of a block B -- a block that is not uniform with B's value (a
+// non-uniform block is an obstacle to every block, itself included; blocks
+// outside the grid are none).  Pass a (x, y, z in turn) gives
+//     f_a(B) = min over |d| <= RUN_RMAX + 1 of max(|d|, same(B + d e_a) ? f_{a-1}(B + d e_a) : 0),
+// with f_{-1} = RUN_RMAX + 1 everywhere: f_x is the distance along x, f_y the
+// distance in the (x, y) plane, f_z the 3-D one (capped at RUN_RMAX + 1).  A
+// block whose value equals B's has the same obstacles as B, and one whose value
+// differs is itself an obstacle, which is why the term is 0 there.  Radius =
+// f_z - 1.  Blocks are indexed bx + nbx * (by + nby * bz).
+__global__ __launch_bounds__(256) void k_block_radius(const float2 *__restrict__ table, const int *__restrict__ fin,
+                                                     int *__restrict__ fout, int nbx, int nby, int nbz, int axis) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= nbx * nby * nbz) return;
+    const int b[3] = {t % nbx, (t / nbx) % nby, t / (nbx * nby)};
+    const int n = axis == 0 ? nbx : (axis == 1 ? nby : nbz);
+    const int stride = axis == 0 ? 1 : (axis == 1 ? nbx : nbx * nby);
+    const float v = table[t].x;
+    int best = RUN_RMAX + 1;
+    for (int d = 0; d <= RUN_RMAX + 1 && d < best; ++d) {
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+            if (d == 0 && sgn > 0) continue;
+            const int p = b[axis] + sgn * d;
+            if (p < 0 || p >= n) continue;
+            const int u = t + sgn * d * stride;
+            const int f = table[u].x == v ? (fin ? fin[u] : RUN_RMAX + 1) : 0;  // NaN never equals
+            best = min(best, max(d, f));
+        }
+    }
+    fout[t] = best;
+}
+
+// Radius = f_z - 1 into the table (0 for non-uniform blocks).
+__global__ __launch_bounds__(256) void k_block_radius_set(float2 *__restrict__ table, const int *__restrict__ f,
+                                                         long long nblk) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk) return;
+    table[t].y = (float)max(f[t] - 1, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,6 +229,7 @@ __device__ __forceinline__ float trilinear_buf(__amdgpu_buffer_rsrc_t rs, float 
 // (0 when du = 0), add = 0 (3e38 when du = 0).
 struct BlockExit {
     int offx, offy, offz;
+    float sgx, sgy, sgz;
     float rdx, rdy, rdz, addx, addy, addz;
 };
 
@@ -190,6 +238,9 @@ __device__ __forceinline__ BlockExit block_exit_setup(float dux, float duy, floa
     e.offx = dux > 0.0f ? 4 : 0;
     e.offy = duy > 0.0f ? 4 : 0;
     e.offz = duz > 0.0f ? 4 : 0;
+    e.sgx = dux > 0.0f ? 1.0f : -1.0f;
+    e.sgy = duy > 0.0f ? 1.0f : -1.0f;
+    e.sgz = duz > 0.0f ? 1.0f : -1.0f;
     e.rdx = dux != 0.0f ? 1.0f / dux : 0.0f;
     e.rdy = duy != 0.0f ? 1.0f / duy : 0.0f;
     e.rdz = duz != 0.0f ? 1.0f / duz : 0.0f;
@@ -199,11 +250,15 @@ __device__ __forceinline__ BlockExit block_exit_setup(float dux, float duy, floa
     return e;
 }
 
-__device__ __forceinline__ int block_run(const BlockExit &e, float cx, float cy, float cz) {
-    // c >= 0 here: truncation is floor
-    const float tx = fmaf((float)((((int)cx >> 2) << 2) + e.offx) - cx, e.rdx, e.addx);
-    const float ty = fmaf((float)((((int)cy >> 2) << 2) + e.offy) - cy, e.rdy, e.addy);
-    const float tz = fmaf((float)((((int)cz >> 2) << 2) + e.offz) - cz, e.rdz, e.addz);
+// Samples from n on that stay in the cube of blocks within Chebyshev distance
+// r of block (bx, by, bz) (cells [4 (b - r), 4 (b + r + 1)) per axis), minus the
+// step of margin: the far face along an axis is 4 b + off + 4 r sg with
+// sg = +1 moving up, -1 moving down.
+__device__ __forceinline__ int cube_run(const BlockExit &e, float cx, float cy, float cz, int bx, int by, int bz,
+                                        float r4) {
+    const float tx = fmaf(fmaf(r4, e.sgx, (float)((bx << 2) + e.offx)) - cx, e.rdx, e.addx);
+    const float ty = fmaf(fmaf(r4, e.sgy, (float)((by << 2) + e.offy)) - cy, e.rdy, e.addy);
+    const float tz = fmaf(fmaf(r4, e.sgz, (float)((bz << 2) + e.offz)) - cz, e.rdz, e.addz);
     return (int)fminf(fminf(fminf(tx, ty), tz), 1.0e9f) - 1;
 }
 
@@ -225,25 +280,31 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
     const int n_end = (int)floorf(t_exit) + 3;
 
     const int nxy = g.nx * g.ny;
+    // one buffer over the quads and the block table after them (byte offsets
+    // below 2^32: grid_ok)
+    const unsigned tab_off = (unsigned)nxy * (unsigned)(g.nz + 1) * 16u;
+    const unsigned nbx = (unsigned)(g.nx + 3) >> 2, nby = (unsigned)(g.ny + 3) >> 2;
+    const unsigned nbz = (unsigned)(g.nz + 3) >> 2;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)qv, (short)0, (int)((unsigned)nxy * (unsigned)(g.nz + 1) * 16u), 0x00020000);
+        (void *)qv, (short)0, (int)(tab_off + nbx * nby * nbz * 8u), 0x00020000);
     const int so_z = nxy * 16;
-    // block table after the quads: rho of uniform 4x4x4 cell blocks, NaN elsewhere
-    const float *ub = reinterpret_cast<const float *>(qv + quad_count(g.nx, g.ny, g.nz));
-    const int nbx = (g.nx + 3) >> 2, nby = (g.ny + 3) >> 2;
     const BlockExit ex = block_exit_setup(dux, duy, duz);
     float acc0 = -0.05f, acc1 = 0.0f;
     int n = 0;
     // Per step: the table entry of sample n's block; in a uniform block the run
-    // of samples n .. n + T that stay in it adds T + 1 times its value at once,
-    // otherwise samples n and n + 1 are read (no quad loads for skipped runs).
+    // of samples n .. n + T that stay in its cube of same-valued blocks adds
+    // T + 1 times its value at once, otherwise samples n and n + 1 are read (no
+    // quad loads for skipped runs).
     while (n < n_safe) {
         const float f0 = (float)n;
         const float cx = fmaf(dux, f0, c0x), cy = fmaf(duy, f0, c0y), cz = fmaf(duz, f0, c0z);
         const int bx = (int)cx >> 2, by = (int)cy >> 2, bz = (int)cz >> 2;  // c >= 0: truncation is floor
-        const float u = gld(ub, bx + nbx * (by + nby * bz));
+        const unsigned bidx = (unsigned)bx + __umul24(nbx, (unsigned)by + __umul24(nby, (unsigned)bz));
+        const auto te = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(tab_off + (bidx << 3)), 0, 0);
+        const float u = __uint_as_float(te[0]);
         if (u == u) {
-            const int T = min(block_run(ex, cx, cy, cz), n_safe - 1 - n);
+            const float r4 = 4.0f * __uint_as_float(te[1]);
+            const int T = min(cube_run(ex, cx, cy, cz, bx, by, bz, r4), n_safe - 1 - n);
             if (T >= 1) {
                 acc1 = fmaf(fmaxf(u, 0.0f) * (float)(T + 1), 0.1f, acc1);
                 n += T + 1;
@@ -756,9 +817,10 @@ bool grid_ok(const dh_grid &g) {
         set_error("invalid grid %d x %d x %d, spacing %g", g.nx, g.ny, g.nz, (double)g.spacing);
         return false;
     }
-    // 32-bit byte offsets of the ray-march buffer loads: 16 * nx * ny * (nz + 1) < 2^32
-    if ((double)g.nx * g.ny * (g.nz + 1) >= 268435456.0) {
-        set_error("grid %d x %d x %d too large (nx*ny*(nz+1) must stay below 2^28)", g.nx, g.ny, g.nz);
+    // 32-bit byte offsets of the ray-march buffer loads: the quads and the block
+    // table after them, 16 * nx * ny * (nz + 1) + 8 * blocks < 2^32
+    if (16.0 * g.nx * g.ny * (g.nz + 1) + 8.0 * (double)block_count(g.nx, g.ny, g.nz) >= 4294967296.0) {
+        set_error("grid %d x %d x %d too large (16*nx*ny*(nz+1) + 8*blocks must stay below 2^32)", g.nx, g.ny, g.nz);
         return false;
     }
     return true;
@@ -768,12 +830,29 @@ bool grid_ok(const dh_grid &g) {
 
 using namespace dh;
 
-// Block table of a quad volume (written right after its quads).
+// Block table of a quad volume (written right after its quads): values, then
+// run radii by three separable passes through the two scratch int arrays that
+// follow the table (x -> s0, y -> s1, z -> s0, radius = s0 - 1).
 static int launch_blocks(float *rlsp_padded, const dh_grid &g, hipStream_t st) {
     const long long nquad = quad_count(g.nx, g.ny, g.nz), nblk = block_count(g.nx, g.ny, g.nz);
-    hipLaunchKernelGGL(k_rlsp_blocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st,
-                       reinterpret_cast<const float4 *>(rlsp_padded), rlsp_padded + 4 * nquad, make_grid(g), nblk);
-    return check_launch("k_rlsp_blocks");
+    float2 *table = reinterpret_cast<float2 *>(rlsp_padded + 4 * nquad);
+    int *s0 = reinterpret_cast<int *>(rlsp_padded + 4 * nquad + 2 * nblk);
+    int *s1 = s0 + nblk;
+    const dim3 grid((unsigned)((nblk + 255) / 256));
+    hipLaunchKernelGGL(k_rlsp_blocks, grid, dim3(256), 0, st, reinterpret_cast<const float4 *>(rlsp_padded), table,
+                       make_grid(g), nblk);
+    int rc = check_launch("k_rlsp_blocks");
+    if (rc) return rc;
+    const int nbx = (g.nx + 3) / 4, nby = (g.ny + 3) / 4, nbz = (g.nz + 3) / 4;
+    int *const out[3] = {s0, s1, s0};
+    const int *const in[3] = {nullptr, s0, s1};
+    for (int axis = 0; axis < 3; ++axis) {
+        hipLaunchKernelGGL(k_block_radius, grid, dim3(256), 0, st, (const float2 *)table, in[axis], out[axis], nbx,
+                           nby, nbz, axis);
+        if ((rc = check_launch("k_block_radius"))) return rc;
+    }
+    hipLaunchKernelGGL(k_block_radius_set, grid, dim3(256), 0, st, table, (const int *)s0, nblk);
+    return check_launch("k_block_radius_set");
 }
 
 extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, const double *hu_knots,

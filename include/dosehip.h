@@ -19,12 +19,15 @@
  *    (a, b - a, c - a, (d - c) - (b - a)) of a = rho(i,j,k), b = rho(i+1,j,k),
  *    c = rho(i,j+1,k), d = rho(i+1,j+1,k), rho = 0 past the high end of each
  *    axis (the bilinear form of the slice-k cell face); 4*nx*ny*(nz+1)
- *    floats, 16-byte aligned, followed by a table of
- *    ceil(nx/4)*ceil(ny/4)*ceil(nz/4) floats, one per 4x4x4 block of
- *    trilinear cells: the rho every corner of the block's cells holds when
- *    they are all equal, NaN otherwise (the ray-march adds runs of samples in
- *    uniform blocks at once).  dh_rlsp_from_hu / dh_rlsp_pad write both
- *    parts; dh_padded_len gives the total.
+ *    floats, 16-byte aligned, followed by a table of B =
+ *    ceil(nx/4)*ceil(ny/4)*ceil(nz/4) float pairs, one per 4x4x4 block of
+ *    trilinear cells (index bx + nbx*(by + nby*bz)): the rho every corner of
+ *    the block's cells holds when they are all equal, NaN otherwise, and for
+ *    such a uniform block its run radius r (0..15, a float): every block of the
+ *    grid within Chebyshev distance r of it is uniform with the same rho (the
+ *    ray-march adds the run of samples that stays in that cube at once); then
+ *    2*B ints of scratch.  dh_rlsp_from_hu / dh_rlsp_pad write all of it;
+ *    dh_padded_len gives the total.
  *  - Return value 0 = success; negative = error, message via dh_last_error().
  *  - Entry points are re-entrant; per-thread error text.
  */
@@ -118,8 +121,8 @@ int dh_version(void);
  * truncated to len).  Returns the length of the full message. */
 int dh_last_error(char *buf, size_t len);
 
-/* Number of floats of a quad RLSP volume with its block table:
- * 4*nx*ny*(nz+1) + ceil(nx/4)*ceil(ny/4)*ceil(nz/4). */
+/* Number of floats of a quad RLSP volume with its block table and scratch:
+ * 4*nx*ny*(nz+1) + 4*ceil(nx/4)*ceil(ny/4)*ceil(nz/4). */
 int64_t dh_padded_len(dh_grid g);
 
 /* HU -> RLSP, piecewise linear through (hu_knots, rlsp_knots), clamped at both

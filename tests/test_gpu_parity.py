@@ -108,6 +108,29 @@ def _quads_np(rho):
     return np.stack([a, b - a, c - a, (d - c) - (b - a)], axis=-1)
 
 
+def _block_radius_np(table, rmax=15):
+    """Run radius per block by brute force: the largest r <= rmax such that every
+    block within Chebyshev distance r (inside the block grid) is uniform with
+    the block's value; 0 for non-uniform blocks."""
+    bz, by, bx = table.shape
+    out = np.zeros(table.shape, np.float32)
+    for k in range(bz):
+        for j in range(by):
+            for i in range(bx):
+                v = table[k, j, i]
+                if np.isnan(v):
+                    continue
+                r = 0
+                while r < rmax:
+                    q = r + 1
+                    cube = table[max(k - q, 0):k + q + 1, max(j - q, 0):j + q + 1, max(i - q, 0):i + q + 1]
+                    if not np.all(cube == v):
+                        break
+                    r = q
+                out[k, j, i] = r
+    return out
+
+
 def _block_table_np(rho):
     """Reference block table: per 4x4x4 block of cells, the common value of the
     corners its cells touch (zero past the high end), NaN if they differ."""
@@ -149,10 +172,17 @@ def test_block_table_and_uniform_skipping(dev, gantry, couch):
     grid = D.make_grid(rlsp.shape, float(np.float32(g.spacing[0])))
     padded = D.rlsp_pad(torch.from_numpy(rlsp).to(dev), grid, dev).cpu().numpy()
     nz, ny, nx = shape
-    table = padded[4 * nx * ny * (nz + 1):].reshape((nz + 3) // 4, (ny + 3) // 4, (nx + 3) // 4)
+    nb = ((nz + 3) // 4) * ((ny + 3) // 4) * ((nx + 3) // 4)
+    assert padded.size == 4 * nx * ny * (nz + 1) + 4 * nb
+    pairs = padded[4 * nx * ny * (nz + 1): 4 * nx * ny * (nz + 1) + 2 * nb].reshape(nb, 2)
+    table = pairs[:, 0].reshape((nz + 3) // 4, (ny + 3) // 4, (nx + 3) // 4)
+    radius = pairs[:, 1].reshape(table.shape)
     want = _block_table_np(rlsp)
     assert np.array_equal(np.isnan(table), np.isnan(want))
     assert np.array_equal(table[~np.isnan(want)], want[~np.isnan(want)])
+    want_r = _block_radius_np(want)
+    assert np.array_equal(radius, want_r), "run radii differ from the brute force"
+    assert want_r.max() >= 1
     # quads: the bilinear form, bit for bit
     q = padded[: 4 * nx * ny * (nz + 1)].reshape(nz + 1, ny, nx, 4)
     assert np.array_equal(q.view(np.uint32), _quads_np(rlsp).view(np.uint32))

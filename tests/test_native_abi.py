@@ -66,8 +66,9 @@ def test_argument_errors_without_gpu():
     with pytest.raises(ValueError):
         N.check(rc, "dh_wet_raymarch")
     good = N.dh_grid(8, 8, 8, 2.0)
-    assert lib.dh_padded_len(good) == 4 * 8 * 8 * 9 + 2 * 2 * 2
-    assert lib.dh_padded_len(N.dh_grid(9, 5, 4, 2.0)) == 4 * 9 * 5 * 5 + 3 * 2 * 1
+    # quads, then per 4x4x4 cell block a (value, run radius) pair and two ints of scratch
+    assert lib.dh_padded_len(good) == 4 * 8 * 8 * 9 + 4 * 2 * 2 * 2
+    assert lib.dh_padded_len(N.dh_grid(9, 5, 4, 2.0)) == 4 * 9 * 5 * 5 + 4 * 3 * 2 * 1
     huge = N.dh_grid(2000, 2000, 2000, 1.0)
     assert lib.dh_rlsp_pad(ctypes.c_void_p(16), ctypes.c_void_p(16), huge, None) == N.DH_ERR_ARG
     knots = (ctypes.c_double * 2)(0.0, 0.0)
