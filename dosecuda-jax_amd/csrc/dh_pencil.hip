@@ -214,6 +214,8 @@ struct VoxTerms {
     float a1, a2;    // exponent scales in the exp2 domain: -0.5 log2(e) / sigma^2 (0 if sigma <= 0)
     float sh;        // halo sigma (culling bound)
     float st;        // primary sigma where positive, else 0 (culling bound of the primary term)
+    float s2;        // sqrt(-a2)
+    float r12;       // a1 / a2 = sigma_h^2 / sigma_t^2 where both are positive, else 0
 };
 
 __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, float wet, float dsrc) {
@@ -244,6 +246,8 @@ __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, f
     t.a2 = pos2 ? (-0.5f * kLog2e) * fast_rcp(sh2) : 0.0f;
     t.sh = fast_sqrt(sh2);
     t.st = pos1 ? st : 0.0f;
+    t.s2 = fast_sqrt(-t.a2);
+    t.r12 = (pos1 && pos2) ? sh2 * fast_rcp(st2) : 0.0f;
     return t;
 }
 
@@ -388,16 +392,47 @@ __global__ __launch_bounds__(64) void k_layer_prep_batch(const BeamCtx *__restri
 // COUNT: no dose; *n_dense += in-range voxels x layer spots (the reference's
 // work), *n_exec += in-range voxels x spots evaluated here, a spot evaluated
 // with the halo term alone counting one half.
-__device__ __forceinline__ void spot_eval(const SpotExt &q, float hx, float hy, float hz, float a1, float a2,
-                                          float &P, float &H) {
-    const float d2 = axis_d2(hx, hy, hz, q);
-    P = fmaf(q.mu, __builtin_amdgcn_exp2f(a1 * d2), P);
-    H = fmaf(q.mu, __builtin_amdgcn_exp2f(a2 * d2), H);
+//
+// The exponents are evaluated on the in-plane offset scaled by s = sqrt(-a2)
+// (a2 = -0.5 log2(e) / sigma_h^2, the halo exponent scale): with
+// fx = 1 - hz / VSADX the offset from the spot axis is u = hx - x fx (as
+// hx - x + hz tx, tx = x / VSADX), so s u = s hx - x (s fx) is one fma with
+// per-(voxel, layer) factors, likewise s v, and the quadratic form of (s u, s v)
+// is qs = -a2 d2 directly.  The halo term is exp2(-qs), the primary term
+// exp2(-r12 qs) with r12 = a1 / a2 = sigma_h^2 / sigma_t^2 (0 without a
+// primary Gaussian).
+struct ScaledVox {
+    float shx, shy, sfx, sfy, r12;
+};
+
+__device__ __forceinline__ ScaledVox scaled_vox(float hx, float hy, float hz, float ivx, float ivy, float s,
+                                               float r12) {
+    ScaledVox z;
+    z.shx = s * hx;
+    z.shy = s * hy;
+    z.sfx = s * fmaf(-hz, ivx, 1.0f);
+    z.sfy = s * fmaf(-hz, ivy, 1.0f);
+    z.r12 = r12;
+    return z;
+}
+
+// -a2 d2 for the spot q
+__device__ __forceinline__ float scaled_d2(const SpotExt &q, const ScaledVox &z) {
+    const float us = fmaf(-q.x, z.sfx, z.shx);
+    const float vs = fmaf(-q.y, z.sfy, z.shy);
+    const float w = fmaf(q.m12x2, vs, q.m11 * us);
+    return fmaf(us, w, (q.m22 * vs) * vs);
+}
+
+__device__ __forceinline__ void spot_eval(const SpotExt &q, const ScaledVox &z, float &P, float &H) {
+    const float qs = scaled_d2(q, z);
+    P = fmaf(q.mu, __builtin_amdgcn_exp2f(-(z.r12 * qs)), P);
+    H = fmaf(q.mu, __builtin_amdgcn_exp2f(-qs), H);
 }
 
 // Halo term only: the spot is beyond reach of the (narrower) primary Gaussian.
-__device__ __forceinline__ void spot_eval_h(const SpotExt &q, float hx, float hy, float hz, float a2, float &H) {
-    H = fmaf(q.mu, __builtin_amdgcn_exp2f(a2 * axis_d2(hx, hy, hz, q)), H);
+__device__ __forceinline__ void spot_eval_h(const SpotExt &q, const ScaledVox &z, float &H) {
+    H = fmaf(q.mu, __builtin_amdgcn_exp2f(-scaled_d2(q, z)), H);
 }
 
 struct VoxelOf {
@@ -461,6 +496,7 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                 rcP2 = rp * rp;
             }
             float P = 0.0f, H = 0.0f;
+            const ScaledVox z = scaled_vox(h.x, h.y, h.z, ivx, ivy, T.s2, T.r12);
             const SpotExt *lsp = gptr(sp, L.start);
             for (int c0 = 0; c0 < L.n; c0 += WAVE) {
                 const int nin = min(WAVE, L.n - c0);
@@ -494,15 +530,15 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                     const int s3 = __builtin_ctzll(mask);
                     mask &= mask - 1ull;
                     const SpotExt q0 = csp[s0], q1 = csp[s1], q2 = csp[s2], q3 = csp[s3];
-                    spot_eval(q0, h.x, h.y, h.z, T.a1, T.a2, P, H);
-                    spot_eval(q1, h.x, h.y, h.z, T.a1, T.a2, P, H);
-                    spot_eval(q2, h.x, h.y, h.z, T.a1, T.a2, P, H);
-                    spot_eval(q3, h.x, h.y, h.z, T.a1, T.a2, P, H);
+                    spot_eval(q0, z, P, H);
+                    spot_eval(q1, z, P, H);
+                    spot_eval(q2, z, P, H);
+                    spot_eval(q3, z, P, H);
                 }
                 while (mask) {
                     const int s0 = __builtin_ctzll(mask);
                     mask &= mask - 1ull;
-                    spot_eval(csp[s0], h.x, h.y, h.z, T.a1, T.a2, P, H);
+                    spot_eval(csp[s0], z, P, H);
                 }
                 while (__popcll(hmask) >= 4) {
                     const int s0 = __builtin_ctzll(hmask);
@@ -514,15 +550,15 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                     const int s3 = __builtin_ctzll(hmask);
                     hmask &= hmask - 1ull;
                     const SpotExt q0 = csp[s0], q1 = csp[s1], q2 = csp[s2], q3 = csp[s3];
-                    spot_eval_h(q0, h.x, h.y, h.z, T.a2, H);
-                    spot_eval_h(q1, h.x, h.y, h.z, T.a2, H);
-                    spot_eval_h(q2, h.x, h.y, h.z, T.a2, H);
-                    spot_eval_h(q3, h.x, h.y, h.z, T.a2, H);
+                    spot_eval_h(q0, z, H);
+                    spot_eval_h(q1, z, H);
+                    spot_eval_h(q2, z, H);
+                    spot_eval_h(q3, z, H);
                 }
                 while (hmask) {
                     const int s0 = __builtin_ctzll(hmask);
                     hmask &= hmask - 1ull;
-                    spot_eval_h(csp[s0], h.x, h.y, h.z, T.a2, H);
+                    spot_eval_h(csp[s0], z, H);
                 }
             }
             if (act) acc = fmaf(T.Af, P, fmaf(T.Bf, H, acc));
