@@ -64,6 +64,39 @@ __device__ __forceinline__ float wave_max_nonneg(float x) {
     return __int_as_float(__builtin_amdgcn_readlane(v, 63));
 }
 
+// Exact wave minimum / maximum of any floats (no NaN), wave-uniform: the bit
+// pattern mapped to an int whose signed order is the float order, reduced with
+// DPP lane moves as wave_max_nonneg (all 64 lanes active).
+__device__ __forceinline__ int float_key(float x) {
+    const int b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float key_float(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+
+__device__ __forceinline__ float wave_min_dpp(float x) {
+    int v = float_key(x);
+    constexpr int kMax = 0x7fffffff;
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0xB1, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x4E, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x141, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x140, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x142, 0xA, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x143, 0xC, 0xF, false));
+    return key_float(__builtin_amdgcn_readlane(v, 63));
+}
+
+__device__ __forceinline__ float wave_max_dpp(float x) {
+    int v = float_key(x);
+    constexpr int kMin = (int)0x80000000;
+    v = max(v, __builtin_amdgcn_update_dpp(kMin, v, 0xB1, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(kMin, v, 0x4E, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(kMin, v, 0x141, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(kMin, v, 0x140, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(kMin, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(kMin, v, 0x143, 0xC, 0xF, false));
+    return key_float(__builtin_amdgcn_readlane(v, 63));
+}
+
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
@@ -188,13 +221,14 @@ __device__ __forceinline__ float sms_bound(const Model &m, const LayerC &c, floa
 // brick's WET range, and the halo width without its negative terms, over that
 // range.
 __device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, const Model &m, float ivx, float ivy,
-                                           float cut, float rbrick) {
+                                           float cut, float rbrick, float *sh_bound = nullptr) {
+    if (sh_bound) *sh_bound = 3.0e38f;
     if (!(c.r80 >= 5.0f)) return false;
     const float fx = fmaf(-B.hc.z, ivx, 1.0f), fy = fmaf(-B.hc.z, ivy, 1.0f);
     const float ex = fmaxf(0.0f, fmaxf(c.xmin * fx - B.hc.x, B.hc.x - c.xmax * fx));
     const float ey = fmaxf(0.0f, fmaxf(c.ymin * fy - B.hc.y, B.hc.y - c.ymax * fy));
     const float dlb = 0.9999f * fast_sqrt(c.lam_min * (ex * ex + ey * ey)) - rbrick;
-    if (!(dlb > 0.0f)) return false;
+    if (!(dlb > 0.0f) && !sh_bound) return false;
     const float whi = fminf(B.whi, c.lim);
     const float w_ub = fminf(fmaxf(whi, 0.1f), c.r80 - 0.1f);
     const float sn_ub = 2.85f + 0.0014f * c.r80 * fast_log(w_ub + 3.0f) + 0.06f * w_ub;
@@ -205,6 +239,7 @@ __device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, 
     if (c.A != 0.0f && c.vx > dlo && c.vx < dhi) sa = fmaxf(sa, fabsf(fmaf(fmaf(c.A, c.vx, c.B), c.vx, c.C)));
     const float st_ub = sa + sms_bound(m, c, B.wlo, whi);
     const float sh_ub = 1.001f * fast_sqrt(st_ub * st_ub + sn_ub * sn_ub);
+    if (sh_bound) *sh_bound = sh_ub;
     return dlb > cut * sh_ub;
 }
 
@@ -458,34 +493,69 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
     int bi, bj, bk, i, j, k;
     vo = brick_voxel(c.g, c.bg, brick, lane, bi, bj, bk, i, j, k);
     const float wet = vo.valid ? gld(c.wet, vo.idx) * 10.0f : 3.0e38f;
+    const LayerC *__restrict__ lc = c.lc;
+    const SpotExt *__restrict__ sp = c.sp;
+    float acc = 0.0f;
+    unsigned long long n_exec2 = 0;  // COUNT: twice the evaluated pairs
+
+    // Bricks no layer can use (every voxel deeper than 1.1 R80 of every layer:
+    // outside the spots' reach in reach-limited WET, or beyond the range) leave
+    // before the geometry: the dose there is 0.
+    const float wlo = wave_min_dpp(wet);
+    bool usable = false;
+    for (int l0 = 0; l0 < c.n_layers && !usable; l0 += WAVE) {
+        bool u = false;
+        if (l0 + lane < c.n_layers) {
+            const LayerC L = gld(lc, l0 + lane);
+            u = L.ok && wlo <= L.lim;
+        }
+        usable = __ballot(u) != 0ull;
+    }
+    if (!usable) return acc;  // (COUNT: no voxel is in range of any layer, nothing to count)
 
     const float3 v = voxel_xyz(c.g, c.b, i, j, k);
     const float3 h = image_to_head(c.b, v);
     const float sx = c.b.src[0] - v.x, sy = c.b.src[1] - v.y, sz = c.b.src[2] - v.z;
     const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
     const float ivx = c.b.ivsadx, ivy = c.b.ivsady;
-    const BrickInfo B = brick_info_at(c.b, brick_centre(c.g, c.b, bi, bj, bk), wave_min(wet),
-                                      wave_max(vo.valid ? wet : -3.0e38f));
-    const LayerC *__restrict__ lc = c.lc;
-    const SpotExt *__restrict__ sp = c.sp;
-
-    float acc = 0.0f;
-    unsigned long long n_exec2 = 0;  // COUNT: twice the evaluated pairs
+    const BrickInfo B = brick_info_at(c.b, brick_centre(c.g, c.b, bi, bj, bk), wlo,
+                                      wave_max_dpp(vo.valid ? wet : -3.0e38f));
     for (int l0 = 0; l0 < c.n_layers; l0 += WAVE) {
         bool lrel = false;
+        float shb = 3.0e38f;  // CULL: upper bound of the halo sigma over the brick for layer l0 + lane
         if (l0 + lane < c.n_layers) {
             const LayerC L = gld(lc, l0 + lane);
             lrel = L.ok && B.wlo <= L.lim;
-            if (CULL && lrel) lrel = !reach_skip(L, B, c.m, ivx, ivy, c.cut, c.rbrick);
+            if (CULL && lrel) lrel = !reach_skip(L, B, c.m, ivx, ivy, c.cut, c.rbrick, &shb);
         }
         unsigned long long lmask = __ballot(lrel);
         while (lmask) {
-            const int l = l0 + __builtin_ctzll(lmask);
+            const int lo = __builtin_ctzll(lmask);
+            const int l = l0 + lo;
             lmask &= lmask - 1ull;
             const LayerC L = gld(lc, l);
             const bool act = wet <= L.lim;
             const unsigned long long amask = __ballot(act);
             if (!amask) continue;
+            const SpotExt *lsp = gptr(sp, L.start);
+            // CULL: spots of the first chunk within the bound radius of the brick centre (cut times the
+            // halo-sigma bound of the brick plus the brick radius, at least the exact radius below); a
+            // visit with none there, and none in later chunks, evaluates nothing and stops before the
+            // voxel terms
+            float d2c0 = 3.0e38f;
+            if (CULL) {
+                const float rb = fmaf(c.cut, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(shb), lo)),
+                                      c.rbrick);
+                const float rb2 = rb * rb;
+                bool any = false;
+                for (int c0 = 0; c0 < L.n && !any; c0 += WAVE) {
+                    float d2 = 3.0e38f;
+                    if (lane < min(WAVE, L.n - c0)) d2 = axis_d2(B.hc.x, B.hc.y, B.hc.z, lsp[c0 + lane]);
+                    if (c0 == 0) d2c0 = d2;
+                    any = __ballot(d2 < rb2) != 0ull;
+                }
+                if (!any) continue;
+            }
             const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
             float rcH2 = 0.0f, rcP2 = 0.0f;
             if (CULL) {
@@ -497,14 +567,13 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
             }
             float P = 0.0f, H = 0.0f;
             const ScaledVox z = scaled_vox(h.x, h.y, h.z, ivx, ivy, T.s2, T.r12);
-            const SpotExt *lsp = gptr(sp, L.start);
             for (int c0 = 0; c0 < L.n; c0 += WAVE) {
                 const int nin = min(WAVE, L.n - c0);
                 unsigned long long mask, hmask;
                 if (CULL) {
                     bool relH = false, relP = false;
                     if (lane < nin) {
-                        const float d2c = axis_d2(B.hc.x, B.hc.y, B.hc.z, lsp[c0 + lane]);
+                        const float d2c = c0 == 0 ? d2c0 : axis_d2(B.hc.x, B.hc.y, B.hc.z, lsp[c0 + lane]);
                         relH = d2c < rcH2;
                         relP = d2c < rcP2;  // implies relH: the halo sigma is the wider
                     }
@@ -520,7 +589,8 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                     continue;
                 }
                 const SpotExt *csp = lsp + c0;
-                while (__popcll(mask) >= 4) {
+                // trip counts from scalar popcounts (a 64-bit compare would go to the VALU)
+                for (int nq = __builtin_amdgcn_readfirstlane(__popcll(mask)) >> 2; nq > 0; --nq) {
                     const int s0 = __builtin_ctzll(mask);
                     mask &= mask - 1ull;
                     const int s1 = __builtin_ctzll(mask);
@@ -540,7 +610,7 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                     mask &= mask - 1ull;
                     spot_eval(csp[s0], z, P, H);
                 }
-                while (__popcll(hmask) >= 4) {
+                for (int nq = __builtin_amdgcn_readfirstlane(__popcll(hmask)) >> 2; nq > 0; --nq) {
                     const int s0 = __builtin_ctzll(hmask);
                     hmask &= hmask - 1ull;
                     const int s1 = __builtin_ctzll(hmask);
