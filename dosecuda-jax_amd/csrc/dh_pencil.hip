@@ -1097,14 +1097,57 @@ __device__ __forceinline__ float axis_d2_t(float hx, float hy, float hz, float x
     return fmaf(u, fmaf(-qr, tx, a), w * fmaf(-qr, ty, b));
 }
 
-__device__ __forceinline__ int pack_groups(int np) { return min(small_div(WAVE, np), PACK_GMAX); }
+// Pass plans of the packed adjoint, fixed at compile time (the wave-uniform
+// counts then index a constant table by scalar loads instead of taking
+// reciprocals on the VALU per visit and pass):
+//   groups[np]            G = min(64 / np, 16) voxel subsets for np spots per pass;
+//   trips[na][np]         ceil(na / G) loop trips over na voxels (indexed by np, so
+//                         that both lookups of a pass issue together);
+//   passes[nrel-1][na-1]  the pass count P in [ceil(nrel / 64), +3] (and <= nrel) with the
+//                         fewest P (ceil(na / G(ceil(nrel / P))) + PACK_PASS_COST), the first on a tie;
+//   spots[nrel-1][na-1]   ceil(nrel / P), the spots of a full pass.
+// PACK_PASS_COST is the work outside the loop per pass (spot exchange,
+// subset fold) in loop trips.
+constexpr int PACK_PASS_COST = 3;
 
-// Loop trips of nrel spots in P passes over na voxels, plus 2 per pass.
-__device__ __forceinline__ int pack_cost(int nrel, int na, int P) {
-    const int np = small_div(nrel + P - 1, P);
-    const int G = pack_groups(np);
-    return P * (small_div(na + G - 1, G) + 2);
+struct PackPlanTables {
+    unsigned char groups[WAVE + 1];
+    unsigned char trips[WAVE + 1][WAVE + 1];
+    unsigned char passes[PACK_CS][WAVE];
+    unsigned char spots[PACK_CS][WAVE];
+};
+
+constexpr int pack_groups_of(int np) { return WAVE / np < PACK_GMAX ? WAVE / np : PACK_GMAX; }
+
+constexpr int pack_cost_of(int nrel, int na, int P) {
+    return P * ((na + pack_groups_of((nrel + P - 1) / P) - 1) / pack_groups_of((nrel + P - 1) / P) + PACK_PASS_COST);
 }
+
+constexpr PackPlanTables make_pack_plan_tables() {
+    PackPlanTables t{};
+    for (int np = 1; np <= WAVE; ++np) t.groups[np] = (unsigned char)pack_groups_of(np);
+    for (int na = 0; na <= WAVE; ++na)
+        for (int np = 1; np <= WAVE; ++np)
+            t.trips[na][np] = (unsigned char)((na + pack_groups_of(np) - 1) / pack_groups_of(np));
+    for (int nrel = 1; nrel <= PACK_CS; ++nrel) {
+        for (int na = 1; na <= WAVE; ++na) {
+            const int p_min = (nrel + WAVE - 1) / WAVE;
+            int P = p_min, best = pack_cost_of(nrel, na, P);
+            for (int Q = p_min + 1; Q <= p_min + 3 && Q <= nrel; ++Q) {
+                const int cq = pack_cost_of(nrel, na, Q);
+                if (cq < best) {
+                    best = cq;
+                    P = Q;
+                }
+            }
+            t.passes[nrel - 1][na - 1] = (unsigned char)P;
+            t.spots[nrel - 1][na - 1] = (unsigned char)((nrel + P - 1) / P);
+        }
+    }
+    return t;
+}
+
+__constant__ PackPlanTables kPackPlan = make_pack_plan_tables();
 
 template <bool CULL, bool COUNT>
 __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, int slab, int lane, PackLds &S,
@@ -1155,6 +1198,7 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
 
     for (long long base = b0; base < b1; base += WAVE) {
         bool brel = false;
+        float shb = 3.0e38f;  // CULL: upper bound of the halo sigma over brick base + lane
         if (base + lane < b1) {
             const float2 ws = gld(c.bsum, base + lane);
             brel = ws.x <= ws.y && ws.x <= L.lim;
@@ -1162,12 +1206,13 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
                 int ti, tj, tk;
                 brick_of(c.bg, base + lane, ti, tj, tk);
                 brel = !reach_skip(R, brick_info_at(c.b, brick_centre(c.g, c.b, ti, tj, tk), ws.x, ws.y), c.m, ivx, ivy,
-                                   c.cut, c.rbrick);
+                                   c.cut, c.rbrick, &shb);
             }
         }
         unsigned long long fm = __ballot(brel);
         while (fm) {
-            const long long brick = base + __builtin_ctzll(fm);
+            const int bl = __builtin_ctzll(fm);
+            const long long brick = base + bl;
             fm &= fm - 1ull;
             int bi, bj, bk, i, j, k;
             const VoxelOf vo = brick_voxel(c.g, c.bg, brick, lane, bi, bj, bk, i, j, k);
@@ -1178,6 +1223,18 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
             if (!am) continue;
             if (COUNT) n_dense += (unsigned long long)__popcll(am) * (unsigned)cnt;
 
+            // CULL: the chunk's spots within the bound radius of the brick centre (cut times the brick's
+            // halo-sigma bound plus the brick radius, at least the exact radius below); none: nothing to
+            // evaluate, leave before the voxel terms
+            float d2s0 = 3.0e38f, d2s1 = 3.0e38f;
+            if (CULL) {
+                const float3 hc = image_to_head(c.b, brick_centre(c.g, c.b, bi, bj, bk));
+                if (v0) d2s0 = axis_d2_t(hc.x, hc.y, hc.z, x0, y0, tx0, ty0, it0);
+                if (v1) d2s1 = axis_d2_t(hc.x, hc.y, hc.z, x1, y1, tx1, ty1, it1);
+                const float rb = fmaf(c.cut, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(shb), bl)),
+                                      c.rbrick);
+                if (!__ballot(d2s0 < rb * rb || d2s1 < rb * rb)) continue;
+            }
             const float3 v = voxel_xyz(c.g, c.b, i, j, k);
             const float3 h = image_to_head(c.b, v);
             const float sx = c.b.src[0] - v.x, sy = c.b.src[1] - v.y, sz = c.b.src[2] - v.z;
@@ -1187,9 +1244,8 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
             bool r0 = v0, r1 = v1;
             if (CULL) {
                 const float rc = fmaf(c.cut, wave_max_nonneg(act ? T.sh : 0.0f), c.rbrick);
-                const float3 hc = image_to_head(c.b, brick_centre(c.g, c.b, bi, bj, bk));
-                r0 = v0 && axis_d2_t(hc.x, hc.y, hc.z, x0, y0, tx0, ty0, it0) < rc * rc;
-                r1 = v1 && axis_d2_t(hc.x, hc.y, hc.z, x1, y1, tx1, ty1, it1) < rc * rc;
+                r0 = v0 && d2s0 < rc * rc;
+                r1 = v1 && d2s1 < rc * rc;
             }
             const unsigned long long m0 = __ballot(r0), m1 = __ballot(r1);
             const int n0 = __popcll(m0), nrel = n0 + __popcll(m1);
@@ -1219,21 +1275,12 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
             }
             // zero voxels past na: every term of a zero voxel is an exact +0
             if (lane >= na) S.vox[lane] = LdsVox{};
-            // passes: fewest loop trips, from the minimum count up
-            const int p_min = (nrel + WAVE - 1) / WAVE;
-            int P = p_min, best = pack_cost(nrel, na, P);
-            for (int Q = p_min + 1; Q <= p_min + 3 && Q <= nrel; ++Q) {
-                const int cq = pack_cost(nrel, na, Q);
-                if (cq < best) {
-                    best = cq;
-                    P = Q;
-                }
-            }
-            const int np_full = small_div(nrel + P - 1, P);
+            // passes: the plan with the fewest loop trips (1 <= nrel <= 128, 1 <= na <= 64)
+            const int np_full = kPackPlan.spots[nrel - 1][na - 1];
             __syncthreads();
             for (int s0 = 0; s0 < nrel; s0 += np_full) {
                 const int np = min(np_full, nrel - s0);
-                const int G = pack_groups(np);
+                const int G = kPackPlan.groups[np];
                 const int group = small_div(lane, np), slot = lane - group * np;
                 const bool on = group < G;
                 const int js = on ? S.list[s0 + slot] : 0;
@@ -1248,7 +1295,7 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
                 const float qx = hi ? bx : ax, qy = hi ? by : ay, qtx = hi ? btx : atx, qty = hi ? bty : aty;
                 const float qit = hi ? bit : ait;
                 const float ttx = qtx * qit, tty = qty * qit;
-                const int nit = small_div(na + G - 1, G);
+                const int nit = kPackPlan.trips[na][np];
                 float K = 0.0f, X0 = 0.0f, X1 = 0.0f, Y0 = 0.0f, Y1 = 0.0f;
                 if (on) {
                     for (int it = 0; it < nit; ++it) {
