@@ -169,6 +169,57 @@ __device__ __forceinline__ void lut_lookup(const Model &m, int eid, float wet, f
     sig = fmaf(c.y - a.y, f, a.y);
 }
 
+// The lookup above on a uniform depth grid with rows [i0, i0 + 64) of the
+// row staged in LDS (tab) by the wave: one coalesced 512-byte load per wave
+// and visit instead of two scattered 8-byte gathers per lane.  i0 is the row
+// of the smallest WET of the voxels that use the result, so their
+// interpolations read staged rows unless the wave's WET spans 63 rows or more
+// (those read the row from global memory).  Same values, same arithmetic:
+// bit-identical to lut_lookup.
+__device__ __forceinline__ int lut_stage(const Model &m, int eid, float wlo, int lane, float2 *tab) {
+    const float q = (wlo - m.depth0) * m.inv_ddepth;
+    const int i0 = (int)fminf(fmaxf(q, 0.0f), (float)(m.lut_len - 1));  // clamped before truncation
+    const float2 *row = gptr(m.idd_sigma, (long long)eid * m.lut_len);
+    tab[lane] = row[min(i0 + lane, m.lut_len - 1)];
+    // the wave's lanes read each other's rows
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return i0;
+}
+
+__device__ __forceinline__ void lut_lookup_staged(const Model &m, int eid, float wet, const float2 *tab, int i0,
+                                                  float &idd, float &sig) {
+    const float2 *row = gptr(m.idd_sigma, (long long)eid * m.lut_len);
+    const int L = m.lut_len;
+    const float q = (wet - m.depth0) * m.inv_ddepth;
+    if (!(q > 0.0f)) {
+        const float2 r = row[0];
+        idd = r.x;
+        sig = r.y;
+        return;
+    }
+    if (q > (float)(L - 1)) {
+        const float2 r = row[L - 1];
+        idd = r.x;
+        sig = r.y;
+        return;
+    }
+    const int lo = min((int)q, L - 2);
+    const float f = q - (float)lo;
+    const int t = lo - i0;
+    float2 a, c;
+    if (t >= 0 && t < WAVE - 1) {
+        a = tab[t];
+        c = tab[t + 1];
+    } else {
+        a = row[lo];
+        c = row[lo + 1];
+    }
+    idd = fmaf(c.x - a.x, f, a.x);
+    sig = fmaf(c.y - a.y, f, a.y);
+}
+
 // Brick summary for the reach test: centre in the beam's-eye view, distance
 // of the centre to the source, WET range (mm) of the voxels that matter.
 struct BrickInfo {
@@ -253,9 +304,9 @@ struct VoxTerms {
     float r12;       // a1 / a2 = sigma_h^2 / sigma_t^2 where both are positive, else 0
 };
 
-__device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, float wet, float dsrc) {
-    float idd, sms;
-    lut_lookup(m, c.eid, wet, idd, sms);
+// From the lookup's (IDD, sigma_ms) at wet.
+__device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, float wet, float dsrc, float idd,
+                                              float sms) {
     const float r80 = c.r80;
     const float d = dsrc - wet + 0.7f * r80;
     const float st = fmaf(fmaf(c.A, d, c.B), d, c.C) + sms;
@@ -284,6 +335,12 @@ __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, f
     t.s2 = fast_sqrt(-t.a2);
     t.r12 = (pos1 && pos2) ? sh2 * fast_rcp(st2) : 0.0f;
     return t;
+}
+
+__device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, float wet, float dsrc) {
+    float idd, sms;
+    lut_lookup(m, c.eid, wet, idd, sms);
+    return vox_terms(m, c, wet, dsrc, idd, sms);
 }
 
 // Squared distance from head-frame point (hx, hy, hz) to a spot axis.
@@ -556,7 +613,17 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                 }
                 if (!any) continue;
             }
-            const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
+            float idd, sms;
+            if (c.m.uniform) {
+                // the layer's LUT rows from the brick's smallest WET on, in this wave's LDS slot
+                __shared__ float2 s_lut[4][WAVE];
+                float2 *tab = s_lut[(threadIdx.x >> 6) & 3];
+                const int i0 = lut_stage(c.m, L.eid, B.wlo, lane, tab);
+                lut_lookup_staged(c.m, L.eid, wet, tab, i0, idd, sms);
+            } else {
+                lut_lookup(c.m, L.eid, wet, idd, sms);
+            }
+            const VoxTerms T = vox_terms(c.m, L, wet, dsrc, idd, sms);
             float rcH2 = 0.0f, rcP2 = 0.0f;
             if (CULL) {
                 // sigmas are non-negative (sh a square root, st 0 unless positive)
