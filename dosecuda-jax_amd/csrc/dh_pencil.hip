@@ -103,10 +103,10 @@ __device__ __forceinline__ float wave_min(float v) {
     return v;
 }
 
-// Reach of spots [start, start + n) (lane-parallel) into c.
-__device__ __forceinline__ void reach_of(const SpotExt *__restrict__ sp, int start, int n, const Model &m, int eid,
-                                         int lane, LayerC &c) {
-    float xmin = 3.0e38f, xmax = -3.0e38f, ymin = 3.0e38f, ymax = -3.0e38f, t2max = 0.0f, smax = 0.0f;
+// Spot part of a reach record: bounding box and smallest distance-form
+// eigenvalue of spots [start, start + n) (lane-parallel) into c.
+__device__ __forceinline__ void reach_spots(const SpotExt *__restrict__ sp, int start, int n, int lane, LayerC &c) {
+    float xmin = 3.0e38f, xmax = -3.0e38f, ymin = 3.0e38f, ymax = -3.0e38f, t2max = 0.0f;
     for (int s = lane; s < n; s += WAVE) {
         const SpotExt q = gld(sp, start + s);
         xmin = fminf(xmin, q.x);
@@ -115,14 +115,32 @@ __device__ __forceinline__ void reach_of(const SpotExt *__restrict__ sp, int sta
         ymax = fmaxf(ymax, q.y);
         t2max = fmaxf(t2max, q.tx * q.tx + q.ty * q.ty);
     }
-    const float2 *row = gptr(m.idd_sigma, (long long)eid * m.lut_len);
-    for (int t = lane; t < m.lut_len; t += WAVE) smax = fmaxf(smax, fabsf(row[t].y));
     c.xmin = wave_min(xmin);
     c.xmax = wave_max(xmax);
     c.ymin = wave_min(ymin);
     c.ymax = wave_max(ymax);
     c.lam_min = 1.0f / (1.0f + wave_max(t2max));
+}
+
+// Reach of spots [start, start + n) of energy eid (lane-parallel) into c.
+__device__ __forceinline__ void reach_of(const SpotExt *__restrict__ sp, int start, int n, const Model &m, int eid,
+                                         int lane, LayerC &c) {
+    reach_spots(sp, start, n, lane, c);
+    float smax = 0.0f;
+    const float2 *row = gptr(m.idd_sigma, (long long)eid * m.lut_len);
+    for (int t = lane; t < m.lut_len; t += WAVE) smax = fmaxf(smax, fabsf(row[t].y));
     c.sms_max = wave_max(smax);
+}
+
+// Reach record of a chunk [c0, c0 + n) of usable layer L: the layer's own record
+// when the chunk is the whole layer (the same computation over the same spots),
+// else the chunk's spot part with the layer's max |sigma_ms| (it depends on the
+// energy only).
+__device__ __forceinline__ LayerC chunk_reach(const SpotExt *__restrict__ sp, const LayerC &L, int c0, int n,
+                                              int lane) {
+    LayerC R = L;
+    if (c0 != 0 || n != L.n) reach_spots(sp, L.start + c0, n, lane, R);
+    return R;
 }
 
 // LUT lookup (interpolateProtonLUT, IMPTClasses.cu:69-92): i = first depth >= wet;
@@ -1016,8 +1034,8 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
     const float ttx = q.tx * itt, tty = q.ty * itt;  // t_xy / |t|^2
     const float ivx = c.b.ivsadx, ivy = c.b.ivsady;
     const unsigned long long chunk_n = (unsigned long long)min(CS, L.n - c0);
-    LayerC R = L;  // reach of this chunk's spots only
-    if (CULL) reach_of(c.sp, L.start + c0, (int)chunk_n, c.m, L.eid, lane, R);
+    // reach of this chunk's spots only
+    const LayerC R = CULL ? chunk_reach(c.sp, L, c0, (int)chunk_n, lane) : L;
 
     const long long nb = c.bg.count();
     const long long b0 = nb * slab / c.nslab, b1 = nb * (slab + 1) / c.nslab;
@@ -1254,8 +1272,7 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
     const float it0 = 1.0f / (tx0 * tx0 + ty0 * ty0 + 1.0f);
     const float it1 = 1.0f / (tx1 * tx1 + ty1 * ty1 + 1.0f);
     const float ivx = c.b.ivsadx, ivy = c.b.ivsady;
-    LayerC R = L;
-    if (CULL) reach_of(c.sp, L.start + c0, cnt, c.m, L.eid, lane, R);
+    const LayerC R = CULL ? chunk_reach(c.sp, L, c0, cnt, lane) : L;
     if (lane < PACK_VOX - WAVE) S.vox[WAVE + lane] = LdsVox{};  // permanent zero tail
 
     const long long nb = c.bg.count();
