@@ -287,8 +287,8 @@ __device__ __forceinline__ float sms_bound(const Model &m, const LayerC &c, floa
 // radius, exceeds cut * (an upper bound of the halo sigma over the brick).  The
 // bound takes sigma_air over the range of d = |src - v| - wet + 0.7 R80 (a
 // quadratic: ends and vertex) plus max |sigma_ms| over the lookup rows of the
-// brick's WET range, and the halo width without its negative terms, over that
-// range.
+// brick's WET range, and the halo width's increasing terms at the top of that
+// range plus its decreasing terms at the bottom.
 __device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, const Model &m, float ivx, float ivy,
                                            float cut, float rbrick, float *sh_bound = nullptr) {
     if (sh_bound) *sh_bound = 3.0e38f;
@@ -299,8 +299,16 @@ __device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, 
     const float dlb = 0.9999f * fast_sqrt(c.lam_min * (ex * ex + ey * ey)) - rbrick;
     if (!(dlb > 0.0f) && !sh_bound) return false;
     const float whi = fminf(B.whi, c.lim);
+    // sigma_n = (increasing terms) + (decreasing terms) in the clamped WET w (R80 >= 5 here: the log term
+    // grows; -7.4e-5 w^2 and -0.22 R80 / (w - R80 - 5)^2 fall for 0 < w < R80 + 5), so over the brick's
+    // range it is at most the increasing terms at the largest WET plus the decreasing ones at the
+    // smallest: a bound that only grows when the range widens
     const float w_ub = fminf(fmaxf(whi, 0.1f), c.r80 - 0.1f);
-    const float sn_ub = 2.85f + 0.0014f * c.r80 * fast_log(w_ub + 3.0f) + 0.06f * w_ub;
+    const float w_lb = fminf(fmaxf(B.wlo, 0.1f), c.r80 - 0.1f);
+    const float e5 = w_lb - c.r80 - 5.0f;
+    const float sn_ub = fmaxf(2.85f + 0.0014f * c.r80 * fast_log(w_ub + 3.0f) + 0.06f * w_ub - 7.4e-5f * w_lb * w_lb -
+                                  0.22f * c.r80 * fast_rcp(e5 * e5),
+                              0.0f);
     const float d07 = 0.7f * c.r80;
     const float dlo = B.dsrc - rbrick - whi + d07;
     const float dhi = B.dsrc + rbrick - B.wlo + d07;
