@@ -155,15 +155,19 @@ struct Model {
     float depth0, inv_ddepth;
 };
 
-// Layer record (64 bytes): the layer's spot range and energy, the constants of
+// Layer record (96 bytes): the layer's spot range and energy, the constants of
 // its sigma_air polynomial, and the reach of its spots: bounding box of the
 // positions at the isocentre plane, the smallest eigenvalue 1/(1 + |t_xy|^2)
-// of their distance forms and max |sigma_ms| of the energy's lookup row.
+// of their distance forms and max |sigma_ms| of the energy's lookup row; then
+// the factors of the halo fit and the range that depend on R80 alone, formed
+// once per layer exactly as the per-voxel expressions would form them.
 struct LayerC {
     int start, n, eid, ok;
     float r80, A, B, C;
     float xmin, xmax, ymin, ymax;
     float lam_min, sms_max, lim, vx;  // vx: vertex -B / (2A) of sigma_air's quadratic (0 when A = 0)
+    float sn_log, sn_pole, un_lin, un_pole;  // 0.0014 R80, 0.22 R80, 1 / (11.2 - 0.023 R80), 0.35 (0.0017 R80^2 - R80)
+    float r3, d07, w_top, pad;               // R80 + 3, 0.7 R80, R80 - 0.1
 };
 
 // Smoothing neighbour stencil of a beam: per (direction, dr) entry, direction

@@ -303,13 +303,13 @@ __device__ __forceinline__ bool reach_skip(const LayerC &c, const BrickInfo &B, 
     // grows; -7.4e-5 w^2 and -0.22 R80 / (w - R80 - 5)^2 fall for 0 < w < R80 + 5), so over the brick's
     // range it is at most the increasing terms at the largest WET plus the decreasing ones at the
     // smallest: a bound that only grows when the range widens
-    const float w_ub = fminf(fmaxf(whi, 0.1f), c.r80 - 0.1f);
-    const float w_lb = fminf(fmaxf(B.wlo, 0.1f), c.r80 - 0.1f);
+    const float w_ub = fminf(fmaxf(whi, 0.1f), c.w_top);
+    const float w_lb = fminf(fmaxf(B.wlo, 0.1f), c.w_top);
     const float e5 = w_lb - c.r80 - 5.0f;
-    const float sn_ub = fmaxf(2.85f + 0.0014f * c.r80 * fast_log(w_ub + 3.0f) + 0.06f * w_ub - 7.4e-5f * w_lb * w_lb -
-                                  0.22f * c.r80 * fast_rcp(e5 * e5),
+    const float sn_ub = fmaxf(2.85f + c.sn_log * fast_log(w_ub + 3.0f) + 0.06f * w_ub - 7.4e-5f * w_lb * w_lb -
+                                  c.sn_pole * fast_rcp(e5 * e5),
                               0.0f);
-    const float d07 = 0.7f * c.r80;
+    const float d07 = c.d07;
     const float dlo = B.dsrc - rbrick - whi + d07;
     const float dhi = B.dsrc + rbrick - B.wlo + d07;
     float sa = fmaxf(fabsf(fmaf(fmaf(c.A, dlo, c.B), dlo, c.C)), fabsf(fmaf(fmaf(c.A, dhi, c.B), dhi, c.C)));
@@ -334,17 +334,16 @@ struct VoxTerms {
 __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, float wet, float dsrc, float idd,
                                               float sms) {
     const float r80 = c.r80;
-    const float d = dsrc - wet + 0.7f * r80;
+    const float d = dsrc - wet + c.d07;
     const float st = fmaf(fmaf(c.A, d, c.B), d, c.C) + sms;
 
-    // nuclear halo (nuclearHalo, IMPTClasses.cu:104-121)
-    const float w = fminf(fmaxf(wet, 0.1f), r80 - 0.1f);
+    // nuclear halo (nuclearHalo, IMPTClasses.cu:104-121); the R80-only factors come from the layer record
+    const float w = fminf(fmaxf(wet, 0.1f), c.w_top);
     const float e5 = w - r80 - 5.0f;
-    const float r3 = r80 + 3.0f;
-    float sn = 2.85f + 0.0014f * r80 * fast_log(w + 3.0f) + 0.06f * w - 7.4e-5f * w * w -
-               0.22f * r80 * fast_rcp(e5 * e5);
-    float un = 0.052f * fast_log(1.13f + w * fast_rcp(11.2f - 0.023f * r80)) +
-               0.35f * (0.0017f * r80 * r80 - r80) * fast_rcp(r3 * r3 - w * w) - 1.61e-9f * w * r3 * r3;
+    const float r3 = c.r3;
+    float sn = 2.85f + c.sn_log * fast_log(w + 3.0f) + 0.06f * w - 7.4e-5f * w * w - c.sn_pole * fast_rcp(e5 * e5);
+    float un = 0.052f * fast_log(1.13f + w * c.un_lin) + c.un_pole * fast_rcp(r3 * r3 - w * w) -
+               1.61e-9f * w * r3 * r3;
     sn = fmaxf(sn, 0.0f);
     un = fminf(fmaxf(un, 0.0f), 0.9f);
 
@@ -444,6 +443,15 @@ __device__ __forceinline__ void layer_prep_one(const BeamCtx &cx, int l, int lan
     c.B = d[3];
     c.C = d[4];
     c.vx = c.A != 0.0f ? -c.B / (2.0f * c.A) : 0.0f;
+    const float r80 = c.r80;
+    c.sn_log = 0.0014f * r80;
+    c.sn_pole = 0.22f * r80;
+    c.un_lin = fast_rcp(11.2f - 0.023f * r80);
+    c.un_pole = 0.35f * (0.0017f * r80 * r80 - r80);
+    c.r3 = r80 + 3.0f;
+    c.d07 = 0.7f * r80;
+    c.w_top = r80 - 0.1f;
+    c.pad = 0.0f;
     if (c.ok) {
         reach_of(cx.sp, c.start, c.n, cx.m, c.eid, lane, c);
     } else {

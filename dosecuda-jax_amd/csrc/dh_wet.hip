@@ -285,6 +285,7 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
     const unsigned tab_off = (unsigned)nxy * (unsigned)(g.nz + 1) * 16u;
     const unsigned nbx = (unsigned)(g.nx + 3) >> 2, nby = (unsigned)(g.ny + 3) >> 2;
     const unsigned nbz = (unsigned)(g.nz + 3) >> 2;
+    const unsigned nbxy = nbx * nby;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)qv, (short)0, (int)(tab_off + nbx * nby * nbz * 8u), 0x00020000);
     const int so_z = nxy * 16;
@@ -299,7 +300,9 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
         const float f0 = (float)n;
         const float cx = fmaf(dux, f0, c0x), cy = fmaf(duy, f0, c0y), cz = fmaf(duz, f0, c0z);
         const int bx = (int)cx >> 2, by = (int)cy >> 2, bz = (int)cz >> 2;  // c >= 0: truncation is floor
-        const unsigned bidx = (unsigned)bx + __umul24(nbx, (unsigned)by + __umul24(nby, (unsigned)bz));
+        // two 24-bit products (the nested form compiled to a 64-bit multiply-add)
+        const unsigned bidx =
+            (unsigned)bx + (unsigned)__umul24(nbx, (unsigned)by) + (unsigned)__umul24(nbxy, (unsigned)bz);
         const auto te = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(tab_off + (bidx << 3)), 0, 0);
         const float u = __uint_as_float(te[0]);
         if (u == u) {
