@@ -12,7 +12,11 @@ One step (batched path) = ONE HIP-graph replay over a batch of K such plans:
 reach-limited WET (ray-march + smoothing where the spots can reach), plan dose
 (forward), dL/dD of L = mean_target (D - 2)^2 + 0.1 mean_body D^2 (target:
 25 mm sphere at the isocentre, body: HU > -500), adjoint (dL/dweight,
-dL/d(x, y)) for every beam of every plan.
+dL/d(x, y)) for every beam of every plan.  The timed graph runs the plans as
+--streams (default 2) sub-batches on concurrent streams (StreamedBatches:
+results bit-identical to one batch); the one-batch, one-stream graph is
+timed beside it, and the per-kernel times come from eager steps of the one
+batch (each kernel with the GPU to itself).
 
 Plain path (per plan, for the ratio): per beam eager calls of full WET
 (ray-march + smoothing), dense forward (every spot for every in-range voxel,
@@ -406,6 +410,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plans", type=int, default=4, help="head-phantom plans per batch (K)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="head workload: the timed step runs the plans as this many sub-batches on concurrent "
+                         "streams (1 = one batch on one stream)")
     ap.add_argument("--cutoff-sigma", type=float, default=6.0)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain (dense, per-beam) path; 0 = skip")
     ap.add_argument("--cpu-sample", type=int, default=800000,
@@ -441,7 +448,7 @@ def main():
     import torch
     from DoseCUDA import _native as N
     from DoseCUDA import device as D
-    from DoseCUDA.batch import BatchedDose, GraphedStep
+    from DoseCUDA.batch import BatchedDose, StreamedBatches
     from DoseCUDA.engine import IMPTDoseEngine
 
     assert torch.cuda.is_available(), "bench.py needs a HIP GPU"
@@ -483,19 +490,30 @@ def main():
         return doses, bd.adjoint(None, reuse_prep=True, input_order=False)
 
     log("batch items %s" % bd.item_counts())
-    ms_graph = None
+    ms_graph = ms_graph_1 = None
+    n_streams = max(1, min(args.streams, K))
     if not args.eager_only:
-        log("capturing the batched step")
-        gs = GraphedStep(step, dev, warmup=max(1, args.warmup))
-        for _ in range(args.warmup):
-            gs.replay()
-        torch.cuda.synchronize()
-        log("timing %d graph replays" % args.steps)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            gs.replay()
-        torch.cuda.synchronize()
-        ms_graph = (time.perf_counter() - t0) / args.steps * 1e3
+        log("capturing the batched step (one batch, one stream)")
+        ms_graph_1, gs = _graph_ms(step, dev, args.warmup, args.steps)
+        del gs
+        ms_graph = ms_graph_1
+        if n_streams > 1:
+            # the same step as sub-batches of the plans on concurrent streams (StreamedBatches)
+            t_s = time.perf_counter()
+            sb = StreamedBatches(engines, n_streams=n_streams, wet_mode="reach",
+                                 tune=None if args.no_tune else "auto",
+                                 tune_grad_fn=lambda doses: [lgrad(d) for d in doses], tune_grad_key="head-loss")
+            t_tune += time.perf_counter() - t_s
+            log("%d sub-batches %s on %d streams, schedules %s" %
+                (len(sb.subs), sb.groups, n_streams, [tune_dict(x.tune) for x in sb.subs]))
+
+            def step_streamed():
+                return sb.step(mus, xys, grad_fn=lambda p, d, out: lgrad(d, out=out), input_order=False)
+
+            log("timing %d graph replays of the streamed step" % args.steps)
+            ms_graph, gs = _graph_ms(step_streamed, dev, args.warmup, args.steps)
+            del gs, sb
+            torch.cuda.empty_cache()
 
     # eager batched steps with per-kernel HIP events (same kernels, same stream)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * N.DH_TIME_N)]
@@ -676,8 +694,10 @@ def main():
                         "WET, plan doses, dL/dD, adjoint (weights and positions) of every plan" % K,
             "plans": K, "voxels_per_plan": nvox, "spots_per_plan": n_spots, "beams_per_plan": nb,
             "cutoff_sigma": args.cutoff_sigma, "path": "graph" if ms_graph is not None else "eager",
+            "streams": n_streams if ms_graph is not None else 1,
             "schedule": tune_dict(bd.tune), "setup_and_autotune_s": t_tune,
         },
+        "one_stream_graph_ms_per_step": ms_graph_1,
         "ms_per_plan": ms_plan,
         "plain_ms_per_plan": plain_ms,
         "plain_culled_ms_per_plan": plain_culled_ms,

@@ -24,6 +24,9 @@ The phase calls only enqueue kernels that read tables prepared at set-up, so
 gradient written with torch element-wise ops, adjoint) in one HIP graph and
 replay it; spot weights and positions are read from device tensors at every
 replay.
+
+``StreamedBatches`` splits the plans into a few sub-batches whose phases run
+concurrently on streams of their own (same results, bit for bit).
 """
 
 import ctypes
@@ -239,6 +242,129 @@ class BatchedDose:
                 "layers": b.n_layers, "sum_blocks": b.n_sum_blocks, "partials": b.n_partials,
                 "adjoint_wg_32": b.n_adj32, "adjoint_wg_64": b.n_adj64, "adjoint_wg_128": b.n_adj128, "reduce_blocks": b.n_reduce_blocks,
                 "workspace_bytes": self.ws_bytes}
+
+
+def split_contiguous(costs, n):
+    """Split items with the given costs into min(n, len(costs)) contiguous,
+    non-empty groups of near-equal total cost: a group closes at the first item
+    whose running sum reaches its share (j + 1) / n of the total, or when every
+    later item is needed to give each remaining group one.  Returns a list of
+    index lists."""
+    costs = [max(float(c), 0.0) for c in costs]
+    m = len(costs)
+    n = max(1, min(int(n), m))
+    if sum(costs) <= 0.0:
+        costs = [1.0] * m
+    total = sum(costs)
+    groups, cur, run = [], [], 0.0
+    for i, c in enumerate(costs):
+        cur.append(i)
+        run += c
+        need = n - len(groups) - 1   # groups still to open after this one
+        if need > 0 and (run >= total * (len(groups) + 1) / n or m - i - 1 == need):
+            groups.append(cur)
+            cur = []
+    groups.append(cur)
+    return groups
+
+
+class StreamedBatches:
+    """A batch of plans as several ``BatchedDose`` sub-batches whose phases run
+    concurrently, each on a stream of its own.
+
+    The plans are split into ``n_streams`` contiguous groups of near-equal work
+    (voxels × spots of each plan).  Sub-batch 0 runs on the caller's current
+    stream, sub-batch j > 0 on a side stream that first waits for the current
+    stream, and the current stream waits for every side stream before a call
+    returns: calls order with the caller's other work, and ``GraphedStep``
+    captures them as one graph with parallel branches.  Kernels of different
+    sub-batches then share the GPU (a ray-march, bound by its texture-path
+    gathers, beside a dose kernel, bound by the VALU; each kernel's last waves
+    beside the next kernel of another sub-batch).  Each sub-batch computes what
+    one ``BatchedDose`` over all the plans computes for its plans, bit for bit.
+
+    Other keyword arguments go to every ``BatchedDose`` (with tune="auto" each
+    sub-batch takes the autotuner's schedule for its own shape).  ``doses``,
+    ``grad_doses`` and ``beam_grads`` list every plan and beam in the order of
+    ``engines``.
+    """
+
+    def __init__(self, engines, n_streams=2, **kw):
+        engines = list(engines)
+        if not engines:
+            raise ValueError("StreamedBatches needs at least one plan")
+        cost = [float(np.prod(e.shape)) * sum(b.spots.n_spots for b in e.beams) for e in engines]
+        self.groups = split_contiguous(cost, n_streams)
+        self.subs = [BatchedDose([engines[i] for i in g], **kw) for g in self.groups]
+        self.engines = engines
+        self.device = self.subs[0].device
+        self.streams = [None] + [torch.cuda.Stream(self.device) for _ in self.subs[1:]]
+        self.doses = [d for s in self.subs for d in s.doses]
+        self.grad_doses = [g for s in self.subs for g in s.grad_doses]
+        self.beam_grads = [bg for s in self.subs for bg in s.beam_grads]
+        self.n_beams = sum(s.n_beams for s in self.subs)
+        self._beam_off = np.cumsum([0] + [s.n_beams for s in self.subs]).tolist()
+        self._plan_off = np.cumsum([0] + [len(g) for g in self.groups]).tolist()
+
+    def _run(self, fn):
+        """fn(j, sub-batch) for every sub-batch, each on its stream; joined on the current stream."""
+        cur = torch.cuda.current_stream(self.device)
+        for j in range(1, len(self.subs)):
+            s = self.streams[j]
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                fn(j, self.subs[j])
+        fn(0, self.subs[0])
+        for s in self.streams[1:]:
+            cur.wait_stream(s)
+
+    def _beams(self, values, j):
+        return None if values is None else values[self._beam_off[j]:self._beam_off[j + 1]]
+
+    def _input_order(self):
+        out = []
+        k = 0
+        for e in self.engines:
+            for b in e.beams:
+                gmu, gxy = self.beam_grads[k]
+                out.append((b.spots.to_input_order(gmu), b.spots.to_input_order(gxy)))
+                k += 1
+        return out
+
+    def forward(self, mus=None, xys=None):
+        """Plan doses of every plan (as BatchedDose.forward)."""
+        self._run(lambda j, s: s.forward(self._beams(mus, j), self._beams(xys, j)))
+        return self.doses
+
+    def adjoint(self, grad_doses=None, reuse_prep=False, input_order=True):
+        """Spot gradients of every beam (as BatchedDose.adjoint)."""
+        def one(j, s):
+            g = None if grad_doses is None else grad_doses[self._plan_off[j]:self._plan_off[j + 1]]
+            s.adjoint(g, reuse_prep=reuse_prep, input_order=False)
+        self._run(one)
+        return self._input_order() if input_order else self.beam_grads
+
+    def step(self, mus=None, xys=None, grad_fn=None, input_order=True):
+        """One optimisation step, each sub-batch on its stream from start to end:
+        forward, dL/dD of every plan p by grad_fn(p, dose, out) (element-wise
+        device work that writes out = grad_doses[p]; None: grad_doses as they
+        are), adjoint.  Returns (doses, beam gradients) as forward / adjoint."""
+        def one(j, s):
+            doses = s.forward(self._beams(mus, j), self._beams(xys, j))
+            if grad_fn is not None:
+                for i, (d, g) in enumerate(zip(doses, s.grad_doses)):
+                    grad_fn(self._plan_off[j] + i, d, g)
+            s.adjoint(None, reuse_prep=True, input_order=False)
+        self._run(one)
+        return self.doses, (self._input_order() if input_order else self.beam_grads)
+
+    def item_counts(self):
+        """Work items of each phase, summed over the sub-batches."""
+        tot = {}
+        for s in self.subs:
+            for k, v in s.item_counts().items():
+                tot[k] = tot.get(k, 0) + v
+        return tot
 
 
 class GraphedStep:

@@ -408,3 +408,63 @@ def test_batch_workspace_covers_both_schedules(dev, cutoff):
     assert float(want[0][0].abs().max()) > 0.0
     for (a0, a1), (b0, b1) in zip(got, want):
         assert torch.equal(a0, b0) and torch.equal(a1, b1)
+
+
+@pytest.mark.parametrize("wet_mode,n_streams", [("reach", 2), ("reach", 3), ("full", 2)])
+def test_streamed_batches(dev, wet_mode, n_streams):
+    """Sub-batches on concurrent streams (StreamedBatches) vs one BatchedDose
+    over the same plans: doses and gradients bit-identical, eager (forward,
+    adjoint, step) and in HIP-graph replays with moving spots, with eager work
+    between the replays."""
+    from DoseCUDA.batch import BatchedDose, GraphedStep, StreamedBatches
+    _say("streamed batches: %s WET, %d streams" % (wet_mode, n_streams))
+    engines = _engines(dev)
+    grads = _grads(engines, seed=3)
+    mus = [b.mu0.clone() for e in engines for b in e.beams]
+    xys = [b.xy0.clone() for e in engines for b in e.beams]
+    one = BatchedDose(engines, wet_mode=wet_mode)
+    sb = StreamedBatches(engines, n_streams=n_streams, wet_mode=wet_mode)
+    assert len(sb.subs) == min(n_streams, len(engines))
+    assert [i for g in sb.groups for i in g] == list(range(len(engines)))
+
+    def ref_step():
+        doses = [d.clone() for d in one.forward(mus, xys)]
+        adj = [(a.clone(), b.clone()) for a, b in one.adjoint(grads, reuse_prep=wet_mode == "reach")]
+        return doses, adj
+
+    def check(doses, adj, want, what):
+        assert len(doses) == len(want[0]) and len(adj) == len(want[1])
+        for p, (a, b) in enumerate(zip(doses, want[0])):
+            assert torch.equal(a, b), "%s: plan %d dose" % (what, p)
+        for k, ((a0, a1), (b0, b1)) in enumerate(zip(adj, want[1])):
+            assert torch.equal(a0, b0) and torch.equal(a1, b1), "%s: beam %d gradients" % (what, k)
+
+    want = ref_step()
+    doses = [d.clone() for d in sb.forward(mus, xys)]
+    adj = sb.adjoint(grads, reuse_prep=wet_mode == "reach")
+    torch.cuda.synchronize()
+    check(doses, adj, want, "forward + adjoint")
+
+    gsrc = [g.clone() for g in grads]
+
+    def grad_fn(p, dose, out):
+        out.copy_(gsrc[p])
+
+    d2, a2 = sb.step(mus, xys, grad_fn=grad_fn)
+    torch.cuda.synchronize()
+    check(d2, a2, want, "step")
+
+    gs = GraphedStep(lambda: sb.step(mus, xys, grad_fn=grad_fn, input_order=False), dev)
+    rng = np.random.default_rng(17)
+    for rep in range(3):
+        for m, x in zip(mus, xys):
+            m.copy_(torch.from_numpy(rng.uniform(0.1, 1.0, size=m.shape[0]).astype(np.float32)))
+            x.add_(torch.from_numpy(rng.normal(scale=0.3, size=tuple(x.shape)).astype(np.float32)).to(dev))
+        doses_g, adj_g = gs.replay()
+        doses_g = [d.clone() for d in doses_g]
+        adj_g = sb._input_order()
+        adj_g = [(a.clone(), b.clone()) for a, b in adj_g]
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device=dev).sum().item()   # eager work between replays
+        check(doses_g, adj_g, ref_step(), "replay %d" % rep)
+        _say("replay %d ok" % rep)
