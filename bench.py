@@ -213,7 +213,7 @@ def run_uneven(args):
     import numpy as np
     import torch
     from DoseCUDA import device as D
-    from DoseCUDA.batch import BatchedDose, tune_dict
+    from DoseCUDA.batch import BatchedDose, StreamedBatches, tune_dict
     from DoseCUDA.engine import IMPTDoseEngine
     from DoseCUDA.workloads import uneven_plans
     n_plans = args.uneven_plans
@@ -251,6 +251,17 @@ def run_uneven(args):
     del gs, bd
     torch.cuda.empty_cache()
     log("batched step %.2f ms for %d plans" % (ms_batch, n_plans))
+    # the same step as sub-batches on concurrent streams
+    ms_streamed = None
+    if args.streams > 1:
+        sb = StreamedBatches(engines, n_streams=args.streams,
+                             grad_fn=lambda p, d, out: grad_into(d, out, masks[p], counts[p][0], counts[p][1]),
+                             wet_mode="reach", tune=None if args.no_tune else "auto", tune_grad_key="uneven-loss")
+        log("%d sub-batches of %s plans" % (len(sb.subs), [len(x) for x in sb.groups]))
+        ms_streamed, gs = _graph_ms(lambda: sb.step(input_order=False), dev, args.warmup, args.steps)
+        del gs, sb
+        torch.cuda.empty_cache()
+        log("streamed step %.2f ms for %d plans" % (ms_streamed, n_plans))
     # each plan alone, same batched path (its own graph, default schedule)
     single = []
     for k, e in enumerate(engines):
@@ -291,12 +302,15 @@ def run_uneven(args):
                                                                           min(spots), max(spots), sum(spots)),
         "batch_ms_per_step": ms_batch,
         "batch_ms_per_plan": ms_batch / n_plans,
+        "streams": args.streams,
+        "streamed_batch_ms_per_step": ms_streamed,
         "alone_batched_ms": {"p50": float(np.percentile(single, 50)), "p99": float(np.percentile(single, 99)),
                              "sum": float(single.sum())},
         "plain_ms": {"p50": float(np.percentile(plain, 50)), "p99": float(np.percentile(plain, 99)),
                      "sum": float(plain.sum())},
         "speedup_batch_vs_plain_per_plan": float(plain.sum() / ms_batch),
         "speedup_batch_vs_alone": float(single.sum() / ms_batch),
+        "speedup_streamed_vs_plain_per_plan": float(plain.sum() / ms_streamed) if ms_streamed else None,
         "setup_and_autotune_s": t_setup,
     }
     print(json.dumps(rec), flush=True)
@@ -500,15 +514,14 @@ def main():
         if n_streams > 1:
             # the same step as sub-batches of the plans on concurrent streams (StreamedBatches)
             t_s = time.perf_counter()
-            sb = StreamedBatches(engines, n_streams=n_streams, wet_mode="reach",
-                                 tune=None if args.no_tune else "auto",
-                                 tune_grad_fn=lambda doses: [lgrad(d) for d in doses], tune_grad_key="head-loss")
+            sb = StreamedBatches(engines, n_streams=n_streams, grad_fn=lambda p, d, out: lgrad(d, out=out),
+                                 wet_mode="reach", tune=None if args.no_tune else "auto", tune_grad_key="head-loss")
             t_tune += time.perf_counter() - t_s
             log("%d sub-batches %s on %d streams, schedules %s" %
                 (len(sb.subs), sb.groups, n_streams, [tune_dict(x.tune) for x in sb.subs]))
 
             def step_streamed():
-                return sb.step(mus, xys, grad_fn=lambda p, d, out: lgrad(d, out=out), input_order=False)
+                return sb.step(mus, xys, input_order=False)
 
             log("timing %d graph replays of the streamed step" % args.steps)
             ms_graph, gs = _graph_ms(step_streamed, dev, args.warmup, args.steps)

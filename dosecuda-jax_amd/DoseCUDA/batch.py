@@ -283,19 +283,28 @@ class StreamedBatches:
     beside the next kernel of another sub-batch).  Each sub-batch computes what
     one ``BatchedDose`` over all the plans computes for its plans, bit for bit.
 
-    Other keyword arguments go to every ``BatchedDose`` (with tune="auto" each
-    sub-batch takes the autotuner's schedule for its own shape).  ``doses``,
-    ``grad_doses`` and ``beam_grads`` list every plan and beam in the order of
-    ``engines``.
+    grad_fn(p, dose, out): the caller's dL/dD of plan p (index into
+    ``engines``) written into out, element-wise device work; ``step`` uses it,
+    and with tune="auto" (and no tune_grad_fn) each sub-batch's adjoint
+    schedule is tuned on it.  Other keyword arguments go to every
+    ``BatchedDose`` (with tune="auto" each sub-batch takes the autotuner's
+    schedule for its own shape).  ``doses``, ``grad_doses`` and ``beam_grads``
+    list every plan and beam in the order of ``engines``.
     """
 
-    def __init__(self, engines, n_streams=2, **kw):
+    def __init__(self, engines, n_streams=2, grad_fn=None, **kw):
         engines = list(engines)
         if not engines:
             raise ValueError("StreamedBatches needs at least one plan")
         cost = [float(np.prod(e.shape)) * sum(b.spots.n_spots for b in e.beams) for e in engines]
         self.groups = split_contiguous(cost, n_streams)
-        self.subs = [BatchedDose([engines[i] for i in g], **kw) for g in self.groups]
+        self.grad_fn = grad_fn
+        self.subs = []
+        for g in self.groups:
+            kwg = dict(kw)
+            if grad_fn is not None and kwg.get("tune_grad_fn") is None:
+                kwg["tune_grad_fn"] = lambda doses, first=g[0]: self._grads_of(doses, first)
+            self.subs.append(BatchedDose([engines[i] for i in g], **kwg))
         self.engines = engines
         self.device = self.subs[0].device
         self.streams = [None] + [torch.cuda.Stream(self.device) for _ in self.subs[1:]]
@@ -305,6 +314,12 @@ class StreamedBatches:
         self.n_beams = sum(s.n_beams for s in self.subs)
         self._beam_off = np.cumsum([0] + [s.n_beams for s in self.subs]).tolist()
         self._plan_off = np.cumsum([0] + [len(g) for g in self.groups]).tolist()
+
+    def _grads_of(self, doses, first):
+        out = [torch.empty_like(d) for d in doses]
+        for i, (d, o) in enumerate(zip(doses, out)):
+            self.grad_fn(first + i, d, o)
+        return out
 
     def _run(self, fn):
         """fn(j, sub-batch) for every sub-batch, each on its stream; joined on the current stream."""
@@ -346,9 +361,12 @@ class StreamedBatches:
 
     def step(self, mus=None, xys=None, grad_fn=None, input_order=True):
         """One optimisation step, each sub-batch on its stream from start to end:
-        forward, dL/dD of every plan p by grad_fn(p, dose, out) (element-wise
-        device work that writes out = grad_doses[p]; None: grad_doses as they
-        are), adjoint.  Returns (doses, beam gradients) as forward / adjoint."""
+        forward, dL/dD of every plan p by grad_fn(p, dose, out) (default: the
+        constructor's; element-wise device work that writes out =
+        grad_doses[p]; neither given: grad_doses as they are), adjoint.
+        Returns (doses, beam gradients) as forward / adjoint."""
+        grad_fn = self.grad_fn if grad_fn is None else grad_fn
+
         def one(j, s):
             doses = s.forward(self._beams(mus, j), self._beams(xys, j))
             if grad_fn is not None:
