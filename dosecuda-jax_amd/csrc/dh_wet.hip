@@ -222,6 +222,28 @@ __device__ __forceinline__ float trilinear_buf(__amdgpu_buffer_rsrc_t rs, float 
     return blend_quads(q0, q1, ax, ay, az);
 }
 
+// The same sample with the cell's two quads kept in q0, q1 between calls: a
+// sample in the cell of the lane's previous one (byte offset off_prev) reads
+// no memory.  Same values, same arithmetic as trilinear_buf.
+__device__ __forceinline__ float trilinear_buf_cached(__amdgpu_buffer_rsrc_t rs, float cx, float cy, float cz, int nx,
+                                                      int nxy, int so_z, unsigned &off_prev, float4 &q0,
+                                                      float4 &q1) {
+    const int ix = (int)cx, iy = (int)cy, iz = (int)cz;  // c >= 0: truncation is floor
+    const float ax = __builtin_amdgcn_fractf(cx), ay = __builtin_amdgcn_fractf(cy), az = __builtin_amdgcn_fractf(cz);
+    const unsigned off = ((unsigned)ix + __umul24((unsigned)iy, (unsigned)nx) + __umul24((unsigned)iz, (unsigned)nxy))
+                         << 4;
+    if (off != off_prev) {
+        const auto r0 = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+        const auto r1 = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, so_z, 0);
+        q0 = make_float4(__uint_as_float(r0[0]), __uint_as_float(r0[1]), __uint_as_float(r0[2]),
+                         __uint_as_float(r0[3]));
+        q1 = make_float4(__uint_as_float(r1[0]), __uint_as_float(r1[1]), __uint_as_float(r1[2]),
+                         __uint_as_float(r1[3]));
+        off_prev = off;
+    }
+    return blend_quads(q0, q1, ax, ay, az);
+}
+
 // Samples from sample n on that stay in the 4x4x4 cell block of position c,
 // minus one step of margin against the rounding of the positions (negative or
 // zero: no run).  Per axis the ray meets the block's far face (the one it moves
@@ -292,10 +314,14 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
     const BlockExit ex = block_exit_setup(dux, duy, duz);
     float acc0 = -0.05f, acc1 = 0.0f;
     int n = 0;
+    // quads of the cell of the lane's last sample (none yet: an offset no cell has)
+    unsigned off_prev = 0xffffffffu;
+    float4 qc0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), qc1 = qc0;
     // Per step: the table entry of sample n's block; in a uniform block the run
     // of samples n .. n + T that stay in its cube of same-valued blocks adds
     // T + 1 times its value at once, otherwise samples n and n + 1 are read (no
-    // quad loads for skipped runs).
+    // quad loads for skipped runs; a sample in its lane's previous cell reuses
+    // the quads read for it: about half the samples at 1 mm steps in 2 mm cells).
     while (n < n_safe) {
         const float f0 = (float)n;
         const float cx = fmaf(dux, f0, c0x), cy = fmaf(duy, f0, c0y), cz = fmaf(duz, f0, c0z);
@@ -314,12 +340,12 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
                 continue;
             }
         }
-        const float v0 = trilinear_buf(rs, cx, cy, cz, g.nx, nxy, so_z);
+        const float v0 = trilinear_buf_cached(rs, cx, cy, cz, g.nx, nxy, so_z, off_prev, qc0, qc1);
         acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
         if (n + 1 < n_safe) {
             const float f1 = (float)(n + 1);
-            const float v1 =
-                trilinear_buf(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z), g.nx, nxy, so_z);
+            const float v1 = trilinear_buf_cached(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z),
+                                                  g.nx, nxy, so_z, off_prev, qc0, qc1);
             acc1 = fmaf(fmaxf(v1, 0.0f), 0.1f, acc1);
             n += 2;
         } else {
