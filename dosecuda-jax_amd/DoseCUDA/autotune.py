@@ -120,7 +120,7 @@ def _phase_ms(fn, device, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=3, grad_doses=None, log=None, grad_fn=None):
+def tune(engines, cutoff_sigma=None, wet_mode="reach", reps=5, grad_doses=None, log=None, grad_fn=None):
     """Search the schedules for this batch; returns (tune dict, timings).
     The adjoint candidates run on dL/dD = grad_doses, else grad_fn(plan doses)
     (the caller's loss gradient), else a fixed pseudo-random field over the
