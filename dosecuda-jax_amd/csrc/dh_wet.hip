@@ -342,6 +342,10 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
         }
         const float v0 = trilinear_buf_cached(rs, cx, cy, cz, g.nx, nxy, so_z, off_prev, qc0, qc1);
         acc0 = fmaf(fmaxf(v0, 0.0f), 0.1f, acc0);
+        // an empty statement that orders the first sample's sum (and memory) before the second sample: the
+        // compiler otherwise sinks that sum below the second sample's conditional loads and keeps two copies
+        // of the cached quads (16 register moves per pair); it emits no instruction
+        asm volatile("" : "+v"(acc0)::"memory");
         if (n + 1 < n_safe) {
             const float f1 = (float)(n + 1);
             const float v1 = trilinear_buf_cached(rs, fmaf(dux, f1, c0x), fmaf(duy, f1, c0y), fmaf(duz, f1, c0z),
