@@ -66,7 +66,8 @@ class dh_batch_beam(ctypes.Structure):
 class dh_batch_tune(ctypes.Structure):
     _fields_ = [("adj_cs", ctypes.c_int32), ("adj_wg_target", ctypes.c_int32),
                 ("adj_min_bricks", ctypes.c_int32), ("adj_max_slabs", ctypes.c_int32),
-                ("fwd_blocks", ctypes.c_int32), ("ray_blocks", ctypes.c_int32)]
+                ("fwd_blocks", ctypes.c_int32), ("ray_blocks", ctypes.c_int32),
+                ("ray_shape", ctypes.c_int32)]
 
 
 class dh_batch(ctypes.Structure):
@@ -79,6 +80,7 @@ class dh_batch(ctypes.Structure):
                 ("n_adj32", ctypes.c_int64), ("n_adj64", ctypes.c_int64), ("n_adj128", ctypes.c_int64),
                 ("n_reduce_blocks", ctypes.c_int64), ("tab", ctypes.c_int64 * 16),
                 ("fwd_blocks", ctypes.c_int32), ("ray_blocks", ctypes.c_int32),
+                ("ray_shape", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("prepared", ctypes.c_int32), ("n_summary_rows", ctypes.c_int32),
                 ("events", ctypes.c_void_p)]
 

@@ -2,6 +2,10 @@
 
 What is tuned, per batch shape:
 
+* ``ray_shape``: the ray-march's voxel patch per wave (tile shape) -- 16x4x1
+  (four bricks of a row, one z-layer), 8x4x2 or 4x4x4.  The patch decides how
+  many cache lines each of the march's gathers touches.  Results do not
+  depend on it.
 * ``ray_blocks`` / ``fwd_blocks``: how bricks reach the ray-march and forward
   kernels -- one wave per brick through the hardware dispatcher (0) or a
   persistent grid of n 256-thread blocks striding over the bricks of all
@@ -39,6 +43,7 @@ from . import _native as N
 
 _LOCK = threading.Lock()
 
+RAY_SHAPES = (0, 1, 2)
 RAY_BLOCKS = (0, 2048, 8192)
 FWD_BLOCKS = (0, 2048, 8192)
 ADJ_CS = (32, 64, 128)
@@ -144,7 +149,7 @@ def _search(engines, cutoff_sigma, wet_mode, reps, grad_doses, log, grad_fn):
     cutoff = float(engines[0].cutoff_sigma if cutoff_sigma is None else cutoff_sigma)
     say = log or (lambda m: None)
     best = {"adj_cs": 0, "adj_wg_target": 0, "adj_min_bricks": 0, "adj_max_slabs": 0, "fwd_blocks": 0,
-            "ray_blocks": 0}
+            "ray_blocks": 0, "ray_shape": 0}
     timings = {}
     dev = engines[0].device
     bd = BatchedDose(engines, cutoff_sigma=cutoff, tune=make_tune(**best), wet_mode=wet_mode)
@@ -156,7 +161,8 @@ def _search(engines, cutoff_sigma, wet_mode, reps, grad_doses, log, grad_fn):
     def fwd():
         N.check(bd.lib.dh_batch_forward(ctypes.byref(bd.batch), 0, bd._stream()), "dh_batch_forward")
 
-    for field, cands, fn in (("ray_blocks", RAY_BLOCKS, ray), ("fwd_blocks", FWD_BLOCKS, fwd)):
+    for field, cands, fn in (("ray_shape", RAY_SHAPES, ray), ("ray_blocks", RAY_BLOCKS, ray),
+                             ("fwd_blocks", FWD_BLOCKS, fwd)):
         res = {}
         for c in cands:
             setattr(bd.batch, field, c)

@@ -38,13 +38,15 @@ from . import _native as N
 from . import device as D
 
 
-TUNE_FIELDS = ("adj_cs", "adj_wg_target", "adj_min_bricks", "adj_max_slabs", "fwd_blocks", "ray_blocks")
+TUNE_FIELDS = ("adj_cs", "adj_wg_target", "adj_min_bricks", "adj_max_slabs", "fwd_blocks", "ray_blocks",
+               "ray_shape")
 
 
-def make_tune(adj_cs=0, adj_wg_target=0, adj_min_bricks=0, adj_max_slabs=0, fwd_blocks=0, ray_blocks=0):
-    """dh_batch_tune; zero fields take the library defaults (0, 32768, 64, 1024, 0, 0)."""
+def make_tune(adj_cs=0, adj_wg_target=0, adj_min_bricks=0, adj_max_slabs=0, fwd_blocks=0, ray_blocks=0,
+              ray_shape=0):
+    """dh_batch_tune; zero fields take the library defaults (0, 32768, 64, 1024, 0, 0, 0)."""
     return N.dh_batch_tune(int(adj_cs), int(adj_wg_target), int(adj_min_bricks), int(adj_max_slabs),
-                           int(fwd_blocks), int(ray_blocks))
+                           int(fwd_blocks), int(ray_blocks), int(ray_shape))
 
 
 def tune_dict(t):

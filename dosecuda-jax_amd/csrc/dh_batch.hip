@@ -293,9 +293,9 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     }
     memset(batch, 0, sizeof *batch);
     if (tune && (tune->fwd_blocks < 0 || tune->ray_blocks < 0 || tune->fwd_blocks > (1 << 20) ||
-                 tune->ray_blocks > (1 << 20))) {
-        set_error("dh_batch_setup: invalid brick schedule (fwd_blocks=%d, ray_blocks=%d)", tune->fwd_blocks,
-                  tune->ray_blocks);
+                 tune->ray_blocks > (1 << 20) || tune->ray_shape < 0 || tune->ray_shape > 2)) {
+        set_error("dh_batch_setup: invalid brick schedule (fwd_blocks=%d, ray_blocks=%d, ray_shape=%d)",
+                  tune->fwd_blocks, tune->ray_blocks, tune->ray_shape);
         return DH_ERR_ARG;
     }
     const AdjTune t = tune_from(tune);
@@ -357,6 +357,7 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     batch->n_summary_rows = (int32_t)D.n_rows;
     batch->fwd_blocks = tune && tune->fwd_blocks > 0 ? tune->fwd_blocks : 0;
     batch->ray_blocks = tune && tune->ray_blocks > 0 ? tune->ray_blocks : 0;
+    batch->ray_shape = tune ? tune->ray_shape : 0;
     batch->magic = kMagic;
     return DH_OK;
 }
@@ -386,6 +387,10 @@ extern "C" int dh_batch_wet(dh_batch *b, int32_t flags, void *stream) {
         set_error("dh_batch_wet: a plan of the batch has no quad RLSP volume");
         return DH_ERR_ARG;
     }
+    if (b->ray_shape < 0 || b->ray_shape > 2) {
+        set_error("dh_batch_wet: ray_shape %d is not 0, 1 or 2", b->ray_shape);
+        return DH_ERR_ARG;
+    }
     if (b->n_beams == 0) return DH_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool reach = (flags & DH_BATCH_WET_REACH) != 0;
@@ -398,7 +403,7 @@ extern "C" int dh_batch_wet(dh_batch *b, int32_t flags, void *stream) {
     const int box = (b->tab[T_FLAGS] & F_SMOOTH_TILE) ? (int)(b->tab[T_FLAGS] >> 32) : 0;
     return launch_wet_batch(table<BeamCtx>(b, T_CTX), table<long long>(b, T_BRICK_OFF),
                             table<long long>(b, T_SMOOTH_OFF), b->n_beams, b->n_bricks, b->n_smooth_blocks, reach,
-                            b->ray_blocks, box, st, b->events);
+                            b->ray_blocks, b->ray_shape, box, st, b->events);
 }
 
 extern "C" int dh_batch_forward(dh_batch *b, int32_t flags, void *stream) {

@@ -283,7 +283,7 @@ bool grid_ok(const dh_grid &g);
 // smooth_box_floats > 0 (the largest box of the batch), else 256-voxel blocks.
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
                      long long total_bricks, long long total_blocks, bool use_need, int persist_blocks,
-                     int smooth_box_floats, hipStream_t st, void *const *ev = nullptr);
+                     int ray_shape, int smooth_box_floats, hipStream_t st, void *const *ev = nullptr);
 // dh_pencil.hip
 int adjoint_chunk(int n_spots, int n_layers, bool cull);
 int adjoint_slabs(long long nbricks, int n_spots, int n_layers, int cs, const AdjTune &t);

@@ -373,7 +373,8 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
 // Work mapping.  A workgroup takes four consecutive bricks; when they lie in
 // one row of the beam's brick grid (bx0 .. bx0 + 3), wave w marches layer
 // z = 4 bk + w of all four: lane (x, y) = (lane & 15, lane >> 4), a 16 x 4
-// voxel patch.  The rays of a patch run side by side, so each of its gathers
+// voxel patch (the default shape; the batched kernels also take 8 x 4 x 2
+// slabs or whole bricks, dh_batch_tune.ray_shape).  The rays of a patch run side by side, so each of its gathers
 // touches four 256-byte runs of quads (one per y) instead of the sixteen
 // 64-byte runs of a 4x4x4 brick: the march is bound by the cache tag lookups
 // of those gathers (one 64-byte segment per clock per CU), and this cuts them
@@ -391,14 +392,26 @@ __device__ __forceinline__ void raymarch_brick(const BeamCtx &c, long long brick
     raymarch_voxel_at(c, bi * BRICK + (lane & 3), bj * BRICK + ((lane >> 2) & 3), bk * BRICK + (lane >> 4));
 }
 
-// Bricks b0 .. b0 + 3 of one row: wave w, lane -> voxel of the 16 x 4 patch at
-// layer w.  Lanes of bricks the reach pass does not need stay idle.
-__device__ __forceinline__ void raymarch_row(const BeamCtx &c, long long b0, int w, int lane, int use_need) {
+// Bricks b0 .. b0 + 3 of one row: wave w, lane -> voxel of its patch, shape 0:
+// the 16 x 4 patch at layer w; shape 1: the 8 x 4 x 2 slab of bricks 2 (w & 1)
+// and 2 (w & 1) + 1 at layers 2 (w >> 1) .. + 1.  Lanes of bricks the reach
+// pass does not need stay idle.
+__device__ __forceinline__ void raymarch_row(const BeamCtx &c, long long b0, int w, int lane, int use_need,
+                                             int shape) {
     int bi, bj, bk;
     brick_coords(c.bg, b0, bi, bj, bk);
-    const int x = lane & 15;
+    int x, y, z;
+    if (shape == 0) {
+        x = lane & 15;
+        y = lane >> 4;
+        z = w;
+    } else {
+        x = 8 * (w & 1) + (lane & 7);
+        y = (lane >> 3) & 3;
+        z = 2 * (w >> 1) + (lane >> 5);
+    }
     if (use_need && !(gld(c.need, b0 + (x >> 2)) & 2u)) return;
-    raymarch_voxel_at(c, bi * BRICK + x, bj * BRICK + (lane >> 4), bk * BRICK + w);
+    raymarch_voxel_at(c, bi * BRICK + x, bj * BRICK + y, bk * BRICK + z);
 }
 
 __device__ __forceinline__ bool in_one_row(const BrickGrid &bg, long long b0, long long end) {
@@ -411,7 +424,7 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(BeamCtx c, unsigned nblock
     const long long b0 = (long long)blk * 4, nbr = c.bg.count();
     const int w = wave_in_block(), lane = threadIdx.x & 63;
     if (in_one_row(c.bg, b0, nbr)) {
-        raymarch_row(c, b0, w, lane, 0);
+        raymarch_row(c, b0, w, lane, 0, 0);
     } else if (b0 + w < nbr) {
         raymarch_brick(c, b0 + w, lane);
     }
@@ -433,14 +446,15 @@ __device__ __forceinline__ void raymarch_item(const BeamCtx *__restrict__ ctx, c
 }
 
 __device__ __forceinline__ void raymarch_quad(const BeamCtx *__restrict__ ctx, const long long *__restrict__ off,
-                                              int n_beams, long long q, long long total, int use_need) {
+                                              int n_beams, long long q, long long total, int use_need,
+                                              int shape) {
     const int w = wave_in_block();
     const int bm = find_owner(off, n_beams, q);
     // off holds the first item of each beam; the last beam ends at total
     const long long end = bm + 1 < n_beams ? off[bm + 1] : total;
     const BeamCtx &c = load_ctx(ctx, bm);
-    if (in_one_row(c.bg, q - off[bm], end - off[bm])) {
-        raymarch_row(c, q - off[bm], w, threadIdx.x & 63, use_need);
+    if (shape != 2 && in_one_row(c.bg, q - off[bm], end - off[bm])) {
+        raymarch_row(c, q - off[bm], w, threadIdx.x & 63, use_need, shape);
     } else if (q + w < total) {
         raymarch_item(ctx, off, n_beams, q + w, use_need);
     }
@@ -452,14 +466,14 @@ template <bool PERSIST>
 __global__ __launch_bounds__(256) void k_wet_raymarch_batch(const BeamCtx *__restrict__ ctx,
                                                            const long long *__restrict__ off, int n_beams,
                                                            long long base, long long total, int use_need,
-                                                           long long stride) {
+                                                           long long stride, int shape) {
     // one-shot grid: chunks of 256 workgroups per XCD (measured best of 4 .. 4096
     // on the batched head workload, 5-8% over the dispatcher's round-robin)
     long long q = base + (long long)(PERSIST ? blockIdx.x : xcd_chunk<256>(blockIdx.x, gridDim.x)) * 4;
     if (PERSIST) {
-        for (; q < total; q += stride) raymarch_quad(ctx, off, n_beams, q, total, use_need);
+        for (; q < total; q += stride) raymarch_quad(ctx, off, n_beams, q, total, use_need, shape);
     } else if (q < total) {
-        raymarch_quad(ctx, off, n_beams, q, total, use_need);
+        raymarch_quad(ctx, off, n_beams, q, total, use_need, shape);
     }
 }
 
@@ -954,21 +968,21 @@ extern "C" int dh_wet_raymarch(const float *rlsp_padded, float *wet_raw, dh_grid
 namespace dh {
 int launch_wet_batch(const BeamCtx *ctx, const long long *brick_off, const long long *block_off, int n_beams,
                      long long total_bricks, long long total_blocks, bool use_need, int persist_blocks,
-                     int smooth_box_floats, hipStream_t st, void *const *ev) {
+                     int ray_shape, int smooth_box_floats, hipStream_t st, void *const *ev) {
     // item spaces above 2^24 bricks go out in several launches
     constexpr long long kMaxItems = 1LL << 24;
     mark(ev, 2 * DH_TIME_RAYMARCH, st);
     if (persist_blocks > 0 && total_bricks > 0) {
         const long long nb = (total_bricks + 3) / 4 < persist_blocks ? (total_bricks + 3) / 4 : persist_blocks;
         hipLaunchKernelGGL(k_wet_raymarch_batch<true>, dim3((unsigned)nb), dim3(256), 0, st, ctx, brick_off,
-                           n_beams, 0LL, total_bricks, use_need ? 1 : 0, 4 * nb);
+                           n_beams, 0LL, total_bricks, use_need ? 1 : 0, 4 * nb, ray_shape);
         const int rc = check_launch("k_wet_raymarch_batch");
         if (rc) return rc;
     }
     for (long long base = 0; persist_blocks <= 0 && base < total_bricks; base += kMaxItems) {
         const long long n = total_bricks - base < kMaxItems ? total_bricks - base : kMaxItems;
         hipLaunchKernelGGL(k_wet_raymarch_batch<false>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, ctx,
-                           brick_off, n_beams, base, total_bricks, use_need ? 1 : 0, total_bricks);
+                           brick_off, n_beams, base, total_bricks, use_need ? 1 : 0, total_bricks, ray_shape);
         const int rc = check_launch("k_wet_raymarch_batch");
         if (rc) return rc;
     }

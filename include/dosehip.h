@@ -276,8 +276,10 @@ typedef struct dh_batch_beam {
  *   fwd_blocks      forward bricks: 0 = one wave per brick, n > 0 = a persistent grid of n 256-thread
  *                   blocks striding over the bricks of all beams
  *   ray_blocks      the same for the ray-march
- * Zero fields take the defaults (0, 32768, 64, 1024, 0, 0).  Results do not depend on fwd_blocks and
- * ray_blocks; the adjoint fields change only the order of partial sums. */
+ *   ray_shape       the ray-march's voxel patch per wave (a workgroup takes four consecutive bricks of a
+ *                   row): 0 = 16x4x1, 1 = 8x4x2, 2 = 4x4x4; bricks not four in one row take 2
+ * Zero fields take the defaults (0, 32768, 64, 1024, 0, 0, 0).  Results do not depend on fwd_blocks,
+ * ray_blocks and ray_shape; the adjoint fields change only the order of partial sums. */
 typedef struct dh_batch_tune {
     int32_t adj_cs;
     int32_t adj_wg_target;
@@ -285,6 +287,8 @@ typedef struct dh_batch_tune {
     int32_t adj_max_slabs;
     int32_t fwd_blocks;
     int32_t ray_blocks;
+    int32_t ray_shape;         /* ray-march voxel patch of a wave: 0 = 16x4x1 (four bricks of a row, one
+                                  z-layer), 1 = 8x4x2 (two bricks, two layers), 2 = 4x4x4 (one brick) */
 } dh_batch_tune;
 
 /* Filled by dh_batch_setup; plain data owned by the caller, read-only to it.
@@ -309,6 +313,7 @@ typedef struct dh_batch {
                                   tab[14] = flags, not an offset: bit 3 = tiled smoothing, whose LDS
                                   box (floats) is in bits 32 and up */
     int32_t fwd_blocks, ray_blocks;   /* brick-kernel schedules from dh_batch_tune */
+    int32_t ray_shape, reserved;      /* ray-march patch shape from dh_batch_tune; reserved = 0 */
     int32_t prepared;          /* spot / layer records prepared by a phase call (DH_BATCH_REUSE_PREP needs it) */
     int32_t n_summary_rows;    /* brick-summary items: rows of bricks along x of the beams with spots */
     void *const *events;       /* kernel timing events (dh_batch_timing), NULL = off */

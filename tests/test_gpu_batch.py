@@ -285,13 +285,15 @@ def test_batch_reach_limited_wet(dev, cutoff):
 
 
 def test_batch_persistent_brick_schedules(dev):
-    """Persistent forward / ray-march grids compute every brick once: WET, dose
-    and gradients equal the one-wave-per-brick schedule bit for bit."""
+    """Persistent forward / ray-march grids and the ray-march patch shapes
+    compute every brick once: WET, dose and gradients equal the default
+    schedule (one wave per brick, 16x4 patches) bit for bit."""
     from DoseCUDA.batch import BatchedDose, make_tune
     _say("persistent brick schedules")
     grads = _grads(_engines(dev)[:3], seed=8)
     out = []
-    for tune in (make_tune(), make_tune(fwd_blocks=5, ray_blocks=3), make_tune(fwd_blocks=4096, ray_blocks=4096)):
+    for tune in (make_tune(), make_tune(fwd_blocks=5, ray_blocks=3), make_tune(fwd_blocks=4096, ray_blocks=4096),
+                 make_tune(ray_shape=1), make_tune(ray_shape=2), make_tune(ray_blocks=3, ray_shape=1)):
         engines = _engines(dev)[:3]
         bd = BatchedDose(engines, tune=tune, wet_mode="reach")
         d = [x.clone() for x in bd.forward()]
@@ -322,6 +324,7 @@ def test_autotuner_picks_and_caches(dev, tmp_path, monkeypatch):
     t = tune_dict(bd.tune)
     _say("tuned %s" % t)
     assert t["adj_cs"] in (32, 64, 128) and t["fwd_blocks"] in A.FWD_BLOCKS and t["ray_blocks"] in A.RAY_BLOCKS
+    assert t["ray_shape"] in A.RAY_SHAPES
     assert A.lookup(A.signature(engines, 6.0, "reach")) == t
 
     def boom(*a, **k):
