@@ -2,9 +2,11 @@
 
 What is tuned, per batch shape:
 
-* ``ray_shape``: the ray-march's voxel patch per wave (tile shape) -- 16x4x1
-  (four bricks of a row, one z-layer), 8x4x2 or 4x4x4.  The patch decides how
-  many cache lines each of the march's gathers touches.  Results do not
+* ``ray_shape``: the ray-march's voxel patch per wave (tile shape) -- 4x4x4
+  (one brick), 8x4x2, 16x4x1 (four bricks of a row, one z-layer), 16x2x2,
+  16x1x4 or 8x2x4.
+  The patch decides how many cache lines each of the march's gathers touches
+  and how often its lanes reuse their last cell together.  Results do not
   depend on it.
 * ``ray_blocks`` / ``fwd_blocks``: how bricks reach the ray-march and forward
   kernels -- one wave per brick through the hardware dispatcher (0) or a
@@ -43,7 +45,7 @@ from . import _native as N
 
 _LOCK = threading.Lock()
 
-RAY_SHAPES = (0, 1, 2)
+RAY_SHAPES = (0, 1, 2, 3, 4, 5)
 RAY_BLOCKS = (0, 2048, 8192)
 FWD_BLOCKS = (0, 2048, 8192)
 ADJ_CS = (32, 64, 128)

@@ -293,7 +293,7 @@ extern "C" int dh_batch_setup(dh_batch *batch, const dh_batch_plan *plans, int32
     }
     memset(batch, 0, sizeof *batch);
     if (tune && (tune->fwd_blocks < 0 || tune->ray_blocks < 0 || tune->fwd_blocks > (1 << 20) ||
-                 tune->ray_blocks > (1 << 20) || tune->ray_shape < 0 || tune->ray_shape > 2)) {
+                 tune->ray_blocks > (1 << 20) || tune->ray_shape < 0 || tune->ray_shape > 5)) {
         set_error("dh_batch_setup: invalid brick schedule (fwd_blocks=%d, ray_blocks=%d, ray_shape=%d)",
                   tune->fwd_blocks, tune->ray_blocks, tune->ray_shape);
         return DH_ERR_ARG;
@@ -387,8 +387,8 @@ extern "C" int dh_batch_wet(dh_batch *b, int32_t flags, void *stream) {
         set_error("dh_batch_wet: a plan of the batch has no quad RLSP volume");
         return DH_ERR_ARG;
     }
-    if (b->ray_shape < 0 || b->ray_shape > 2) {
-        set_error("dh_batch_wet: ray_shape %d is not 0, 1 or 2", b->ray_shape);
+    if (b->ray_shape < 0 || b->ray_shape > 5) {
+        set_error("dh_batch_wet: ray_shape %d is not in 0 .. 5", b->ray_shape);
         return DH_ERR_ARG;
     }
     if (b->n_beams == 0) return DH_OK;

@@ -371,16 +371,17 @@ __device__ __forceinline__ float raymarch_voxel(const float4 *__restrict__ qv, c
 }
 
 // Work mapping.  A workgroup takes four consecutive bricks; when they lie in
-// one row of the beam's brick grid (bx0 .. bx0 + 3), wave w marches layer
-// z = 4 bk + w of all four: lane (x, y) = (lane & 15, lane >> 4), a 16 x 4
-// voxel patch (the default shape; the batched kernels also take 8 x 4 x 2
-// slabs or whole bricks, dh_batch_tune.ray_shape).  The rays of a patch run side by side, so each of its gathers
-// touches four 256-byte runs of quads (one per y) instead of the sixteen
-// 64-byte runs of a 4x4x4 brick: the march is bound by the cache tag lookups
-// of those gathers (one 64-byte segment per clock per CU), and this cuts them
-// by about a third.  Other quadruples (a row end, a beam end) fall back to one
-// wave per brick, lane = voxel of the 4x4x4 brick.  Every voxel's value is the
-// same either way.
+// one row of the beam's brick grid (bx0 .. bx0 + 3), its four waves tile their
+// 16 x 4 x 4 voxels with patches of one shape (kRayShape,
+// dh_batch_tune.ray_shape), e.g. 16 x 4 x 1: wave w marches layer z = 4 bk + w
+// of all four bricks, lane (x, y) = (lane & 15, lane >> 4).  The shape decides
+// how many cache lines each gather of the march touches (wide x-runs touch
+// fewer) and how often a wave's lanes reuse their last cell together (the
+// rays of a compact patch enter new cells at nearly the same steps).  On the
+// head batch with the cell cache the 4 x 4 x 4 brick is the fastest (the
+// default) and 16 x 4 x 1 the slowest, 6% apart.  Other quadruples (a row end,
+// a beam end) go one wave per brick.  Every voxel's value is the same
+// whatever the shape.
 __device__ __forceinline__ void raymarch_voxel_at(const BeamCtx &c, int i, int j, int k) {
     if (i >= c.g.nx || j >= c.g.ny || k >= c.g.nz) return;
     gst(c.wet_raw, i + (long long)c.g.nx * (j + (long long)c.g.ny * k), raymarch_voxel(c.rlsp_q, c.g, c.b, i, j, k));
@@ -392,24 +393,32 @@ __device__ __forceinline__ void raymarch_brick(const BeamCtx &c, long long brick
     raymarch_voxel_at(c, bi * BRICK + (lane & 3), bj * BRICK + ((lane >> 2) & 3), bk * BRICK + (lane >> 4));
 }
 
-// Bricks b0 .. b0 + 3 of one row: wave w, lane -> voxel of its patch, shape 0:
-// the 16 x 4 patch at layer w; shape 1: the 8 x 4 x 2 slab of bricks 2 (w & 1)
-// and 2 (w & 1) + 1 at layers 2 (w >> 1) .. + 1.  Lanes of bricks the reach
-// pass does not need stay idle.
+// Ray-march patch shapes (log2 of the x, y, z extents of a wave's 64 voxels):
+// the four waves of a workgroup tile the 16 x 4 x 4 voxels of bricks b0 .. b0 + 3
+// of one row with patches of this shape.
+constexpr int RAY_SHAPES = 6;
+__constant__ const unsigned char kRayShape[RAY_SHAPES][3] = {
+    {2, 2, 2},  // 4 x 4 x 4 (one brick; the default)
+    {3, 2, 1},  // 8 x 4 x 2
+    {4, 2, 0},  // 16 x 4 x 1
+    {4, 1, 1},  // 16 x 2 x 2
+    {4, 0, 2},  // 16 x 1 x 4
+    {3, 1, 2},  // 8 x 2 x 4
+};
+
+// Bricks b0 .. b0 + 3 of one row: wave w, lane -> voxel of its patch of the
+// given shape.  Lanes of bricks the reach pass does not need stay idle.
 __device__ __forceinline__ void raymarch_row(const BeamCtx &c, long long b0, int w, int lane, int use_need,
                                              int shape) {
     int bi, bj, bk;
     brick_coords(c.bg, b0, bi, bj, bk);
-    int x, y, z;
-    if (shape == 0) {
-        x = lane & 15;
-        y = lane >> 4;
-        z = w;
-    } else {
-        x = 8 * (w & 1) + (lane & 7);
-        y = (lane >> 3) & 3;
-        z = 2 * (w >> 1) + (lane >> 5);
-    }
+    const int lx = kRayShape[shape][0], ly = kRayShape[shape][1], lz = kRayShape[shape][2];
+    // waves per axis: 16 >> lx along x, 4 >> ly along y (the rest along z)
+    const int wlx = 4 - lx, wly = 2 - ly;
+    const int wx = w & ((1 << wlx) - 1), wy = (w >> wlx) & ((1 << wly) - 1), wz = w >> (wlx + wly);
+    const int x = (wx << lx) + (lane & ((1 << lx) - 1));
+    const int y = (wy << ly) + ((lane >> lx) & ((1 << ly) - 1));
+    const int z = (wz << lz) + (lane >> (lx + ly));
     if (use_need && !(gld(c.need, b0 + (x >> 2)) & 2u)) return;
     raymarch_voxel_at(c, bi * BRICK + x, bj * BRICK + y, bk * BRICK + z);
 }
@@ -424,7 +433,7 @@ __global__ __launch_bounds__(256) void k_wet_raymarch(BeamCtx c, unsigned nblock
     const long long b0 = (long long)blk * 4, nbr = c.bg.count();
     const int w = wave_in_block(), lane = threadIdx.x & 63;
     if (in_one_row(c.bg, b0, nbr)) {
-        raymarch_row(c, b0, w, lane, 0, 0);
+        raymarch_row(c, b0, w, lane, 0, 0);  // the default shape
     } else if (b0 + w < nbr) {
         raymarch_brick(c, b0 + w, lane);
     }
@@ -453,7 +462,7 @@ __device__ __forceinline__ void raymarch_quad(const BeamCtx *__restrict__ ctx, c
     // off holds the first item of each beam; the last beam ends at total
     const long long end = bm + 1 < n_beams ? off[bm + 1] : total;
     const BeamCtx &c = load_ctx(ctx, bm);
-    if (shape != 2 && in_one_row(c.bg, q - off[bm], end - off[bm])) {
+    if (in_one_row(c.bg, q - off[bm], end - off[bm])) {
         raymarch_row(c, q - off[bm], w, threadIdx.x & 63, use_need, shape);
     } else if (q + w < total) {
         raymarch_item(ctx, off, n_beams, q + w, use_need);

@@ -276,8 +276,9 @@ typedef struct dh_batch_beam {
  *   fwd_blocks      forward bricks: 0 = one wave per brick, n > 0 = a persistent grid of n 256-thread
  *                   blocks striding over the bricks of all beams
  *   ray_blocks      the same for the ray-march
- *   ray_shape       the ray-march's voxel patch per wave (a workgroup takes four consecutive bricks of a
- *                   row): 0 = 16x4x1, 1 = 8x4x2, 2 = 4x4x4; bricks not four in one row take 2
+ *   ray_shape       the ray-march's voxel patch per wave (a workgroup takes the 16x4x4 voxels of four
+ *                   consecutive bricks of a row): 0 = 4x4x4, 1 = 8x4x2, 2 = 16x4x1, 3 = 16x2x2,
+ *                   4 = 16x1x4, 5 = 8x2x4; bricks not four in one row go one brick per wave
  * Zero fields take the defaults (0, 32768, 64, 1024, 0, 0, 0).  Results do not depend on fwd_blocks,
  * ray_blocks and ray_shape; the adjoint fields change only the order of partial sums. */
 typedef struct dh_batch_tune {
@@ -287,8 +288,9 @@ typedef struct dh_batch_tune {
     int32_t adj_max_slabs;
     int32_t fwd_blocks;
     int32_t ray_blocks;
-    int32_t ray_shape;         /* ray-march voxel patch of a wave: 0 = 16x4x1 (four bricks of a row, one
-                                  z-layer), 1 = 8x4x2 (two bricks, two layers), 2 = 4x4x4 (one brick) */
+    int32_t ray_shape;         /* ray-march voxel patch of a wave: 0 = 4x4x4 (one brick), 1 = 8x4x2,
+                                  2 = 16x4x1 (four bricks of a row, one z-layer), 3 = 16x2x2, 4 = 16x1x4,
+                                  5 = 8x2x4 */
 } dh_batch_tune;
 
 /* Filled by dh_batch_setup; plain data owned by the caller, read-only to it.

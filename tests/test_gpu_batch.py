@@ -293,7 +293,8 @@ def test_batch_persistent_brick_schedules(dev):
     grads = _grads(_engines(dev)[:3], seed=8)
     out = []
     for tune in (make_tune(), make_tune(fwd_blocks=5, ray_blocks=3), make_tune(fwd_blocks=4096, ray_blocks=4096),
-                 make_tune(ray_shape=1), make_tune(ray_shape=2), make_tune(ray_blocks=3, ray_shape=1)):
+                 make_tune(ray_shape=1), make_tune(ray_shape=2), make_tune(ray_shape=3), make_tune(ray_shape=4),
+                 make_tune(ray_shape=5), make_tune(ray_blocks=3, ray_shape=1)):
         engines = _engines(dev)[:3]
         bd = BatchedDose(engines, tune=tune, wet_mode="reach")
         d = [x.clone() for x in bd.forward()]

@@ -180,7 +180,7 @@ def test_batch_sizes_and_errors_without_gpu():
     rc = lib.dh_batch_setup(ctypes.byref(b), plans, 2, beams, 3, ctypes.byref(neg), 6.0, ctypes.c_void_p(4096),
                             1 << 40, None)
     assert rc == N.DH_ERR_ARG and "brick schedule" in N.last_error()
-    for shape in (-1, 3):
+    for shape in (-1, 6):
         bad_shape = N.dh_batch_tune(0, 0, 0, 0, 0, 0, shape)
         rc = lib.dh_batch_setup(ctypes.byref(b), plans, 2, beams, 3, ctypes.byref(bad_shape), 6.0,
                                 ctypes.c_void_p(4096), 1 << 40, None)
