@@ -626,8 +626,12 @@ def main():
         log("plain path, dense, %d steps" % args.plain_steps)
         plain_ms, (_, gd_dense) = plain_time(0.0)
     else:
+        # the plain path's dL/dD: dose of every pair (cutoff 0), as the plain steps would compute it
         e0.compute_wet(force=True)
+        cut0 = e0.cutoff_sigma
+        e0.cutoff_sigma = 0.0
         dose_d = e0.dose()
+        e0.cutoff_sigma = cut0
         gd_dense = lgrad(dose_d)
     counts_ref = torch.zeros(4, dtype=torch.int64, device=dev)
     for b in e0.beams:

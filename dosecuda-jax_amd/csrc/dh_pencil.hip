@@ -368,6 +368,19 @@ __device__ __forceinline__ VoxTerms vox_terms(const Model &m, const LayerC &c, f
     return vox_terms(m, c, wet, dsrc, idd, sms);
 }
 
+// Distance from a spot axis beyond which the voxel's halo term is below
+// exp(-cut^2 / 2) of the spot's peak dose at the voxel's depth (primary plus
+// halo on the axis, Af + Bf): cut_h sigma_h with cut_h^2 = cut^2 + 2 ln(Bf /
+// (Af + Bf)) <= cut^2 (a margin of 1e-3 in cut_h^2 covers the rounding of the
+// logarithm); 0 without a halo term.  A primary term is dropped beyond cut
+// sigma_t, below exp(-cut^2 / 2) of the same peak, so every skipped term keeps
+// that bound.
+__device__ __forceinline__ float halo_reach(const VoxTerms &T, float cut) {
+    if (!(T.Bf > 0.0f)) return 0.0f;
+    const float ch2 = fmaf(2.0f, fast_log(T.Bf * fast_rcp(T.Af + T.Bf)), fmaf(cut, cut, 1e-3f));
+    return T.sh * fast_sqrt(fmaxf(ch2, 0.0f));
+}
+
 // Squared distance from head-frame point (hx, hy, hz) to a spot axis.
 __device__ __forceinline__ float axis_d2(float hx, float hy, float hz, const SpotExt &q) {
     const float u = fmaf(hz, q.tx, hx - q.x);
@@ -661,7 +674,7 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
             float rcH2 = 0.0f, rcP2 = 0.0f;
             if (CULL) {
                 // sigmas are non-negative (sh a square root, st 0 unless positive)
-                const float rh = fmaf(c.cut, wave_max_nonneg(act ? T.sh : 0.0f), c.rbrick);
+                const float rh = wave_max_nonneg(act ? halo_reach(T, c.cut) : 0.0f) + c.rbrick;
                 const float rp = fmaf(c.cut, wave_max_nonneg(act ? T.st : 0.0f), c.rbrick);
                 rcH2 = rh * rh;
                 rcP2 = rp * rp;
@@ -676,7 +689,7 @@ __device__ __forceinline__ float forward_brick(const BeamCtx &c, long long brick
                     if (lane < nin) {
                         const float d2c = c0 == 0 ? d2c0 : axis_d2(B.hc.x, B.hc.y, B.hc.z, lsp[c0 + lane]);
                         relH = d2c < rcH2;
-                        relP = d2c < rcP2;  // implies relH: the halo sigma is the wider
+                        relP = d2c < rcP2;  // both terms; relH alone: the halo term only
                     }
                     mask = __ballot(relP);
                     hmask = __ballot(relH) & ~mask;
@@ -1092,7 +1105,7 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
 
             bool rel = sval;
             if (CULL) {
-                const float rc = fmaf(c.cut, wave_max_nonneg(act ? T.sh : 0.0f), c.rbrick);
+                const float rc = wave_max_nonneg(act ? fmaxf(c.cut * T.st, halo_reach(T, c.cut)) : 0.0f) + c.rbrick;
                 const float3 hc = image_to_head(c.b, brick_centre(c.g, c.b, bi, bj, bk));
                 rel = sval && axis_d2(hc.x, hc.y, hc.z, q) < rc * rc;
             }
@@ -1343,7 +1356,7 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
 
             bool r0 = v0, r1 = v1;
             if (CULL) {
-                const float rc = fmaf(c.cut, wave_max_nonneg(act ? T.sh : 0.0f), c.rbrick);
+                const float rc = wave_max_nonneg(act ? fmaxf(c.cut * T.st, halo_reach(T, c.cut)) : 0.0f) + c.rbrick;
                 r0 = v0 && d2s0 < rc * rc;
                 r1 = v1 && d2s1 < rc * rc;
             }
