@@ -1183,7 +1183,9 @@ __device__ __forceinline__ void adjoint_item(const BeamCtx &c, int cid, int slab
 // subsets per pass, so the lanes carry pairs of reachable spots only.  The
 // pass count minimises the loop trips (plus a fixed cost per pass).  Each
 // pass folds its G subsets through LDS into the spot's owner lane in a fixed
-// order (deterministic).
+// order (deterministic).  With culling, spots beyond the primary radius of the
+// brick come last in the list, and a pass holding only such spots evaluates
+// the halo term alone (the forward drops the same primary terms).
 constexpr int PACK_CS = 128;
 constexpr int PACK_GMAX = 16;
 constexpr int PACK_VOX = WAVE + PACK_GMAX;  // voxel list incl. the zero tail a pass may read
@@ -1354,21 +1356,29 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
             const float dsrc = sqrtf(sx * sx + sy * sy + sz * sz);
             const VoxTerms T = vox_terms(c.m, L, wet, dsrc);
 
-            bool r0 = v0, r1 = v1;
+            // r: within reach of a term; h: of the halo term only (beyond the primary radius)
+            bool r0 = v0, r1 = v1, h0 = false, h1 = false;
             if (CULL) {
                 const float rc = wave_max_nonneg(act ? fmaxf(c.cut * T.st, halo_reach(T, c.cut)) : 0.0f) + c.rbrick;
+                const float rp = fmaf(c.cut, wave_max_nonneg(act ? T.st : 0.0f), c.rbrick);
                 r0 = v0 && d2s0 < rc * rc;
                 r1 = v1 && d2s1 < rc * rc;
+                h0 = r0 && !(d2s0 < rp * rp);
+                h1 = r1 && !(d2s1 < rp * rp);
             }
-            const unsigned long long m0 = __ballot(r0), m1 = __ballot(r1);
-            const int n0 = __popcll(m0), nrel = n0 + __popcll(m1);
+            const unsigned long long mp0 = __ballot(r0 && !h0), mp1 = __ballot(r1 && !h1);
+            const unsigned long long mh0 = __ballot(h0), mh1 = __ballot(h1);
+            const int np0 = __popcll(mp0), nP = np0 + __popcll(mp1);
+            const int nh0 = __popcll(mh0), nrel = nP + nh0 + __popcll(mh1);
             if (nrel == 0) continue;
             if (COUNT) {
                 n_exec += (unsigned long long)__popcll(am) * (unsigned)nrel;
                 continue;
             }
-            // list positions: reachable first-set spots, then second-set spots
-            const int p0 = __popcll(m0 & lt_mask), p1 = n0 + __popcll(m1 & lt_mask);
+            // list positions: spots with both terms (first set, second set), then
+            // halo-only spots (first set, second set); passes past nP take the halo term alone
+            const int p0 = h0 ? nP + __popcll(mh0 & lt_mask) : __popcll(mp0 & lt_mask);
+            const int p1 = h1 ? nP + nh0 + __popcll(mh1 & lt_mask) : np0 + __popcll(mp1 & lt_mask);
             if (r0) S.list[p0] = lane;
             if (r1) S.list[p1] = lane + WAVE;
             const int na = __popcll(am);
@@ -1411,24 +1421,44 @@ __device__ __forceinline__ void adjoint_item_packed(const BeamCtx &c, int cid, i
                 const int nit = kPackPlan.trips[na][np];
                 float K = 0.0f, X0 = 0.0f, X1 = 0.0f, Y0 = 0.0f, Y1 = 0.0f;
                 if (on) {
-                    for (int it = 0; it < nit; ++it) {
-                        const LdsVox e = S.vox[it * G + group];
-                        // same terms as adjoint_item
-                        const float a = e.hx - qx, b = e.hy - qy;
-                        const float u = fmaf(e.hz, qtx, a), w = fmaf(e.hz, qty, b);
-                        const float qr = fmaf(u, ttx, fmaf(w, tty, -e.hz));
-                        const float px = fmaf(-qr, qtx, a), py = fmaf(-qr, qty, b);
-                        const float d2 = fmaf(u, px, w * py);
-                        const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
-                        const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
-                        K = fmaf(e.GA, e1, K);
-                        K = fmaf(e.GB, e2, K);
-                        const float F = fmaf(e.GAa, e1, e.GBb * e2);
-                        const float Fq = F * qr;
-                        X0 = fmaf(F, px, X0);
-                        X1 = fmaf(Fq, px, X1);
-                        Y0 = fmaf(F, py, Y0);
-                        Y1 = fmaf(Fq, py, Y1);
+                    if (s0 < nP) {
+                        for (int it = 0; it < nit; ++it) {
+                            const LdsVox e = S.vox[it * G + group];
+                            // same terms as adjoint_item
+                            const float a = e.hx - qx, b = e.hy - qy;
+                            const float u = fmaf(e.hz, qtx, a), w = fmaf(e.hz, qty, b);
+                            const float qr = fmaf(u, ttx, fmaf(w, tty, -e.hz));
+                            const float px = fmaf(-qr, qtx, a), py = fmaf(-qr, qty, b);
+                            const float d2 = fmaf(u, px, w * py);
+                            const float e1 = __builtin_amdgcn_exp2f(e.a1 * d2);
+                            const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
+                            K = fmaf(e.GA, e1, K);
+                            K = fmaf(e.GB, e2, K);
+                            const float F = fmaf(e.GAa, e1, e.GBb * e2);
+                            const float Fq = F * qr;
+                            X0 = fmaf(F, px, X0);
+                            X1 = fmaf(Fq, px, X1);
+                            Y0 = fmaf(F, py, Y0);
+                            Y1 = fmaf(Fq, py, Y1);
+                        }
+                    } else {
+                        // every spot of the pass is beyond the primary radius: the halo term alone
+                        for (int it = 0; it < nit; ++it) {
+                            const LdsVox e = S.vox[it * G + group];
+                            const float a = e.hx - qx, b = e.hy - qy;
+                            const float u = fmaf(e.hz, qtx, a), w = fmaf(e.hz, qty, b);
+                            const float qr = fmaf(u, ttx, fmaf(w, tty, -e.hz));
+                            const float px = fmaf(-qr, qtx, a), py = fmaf(-qr, qty, b);
+                            const float d2 = fmaf(u, px, w * py);
+                            const float e2 = __builtin_amdgcn_exp2f(e.a2 * d2);
+                            K = fmaf(e.GB, e2, K);
+                            const float F = e.GBb * e2;
+                            const float Fq = F * qr;
+                            X0 = fmaf(F, px, X0);
+                            X1 = fmaf(Fq, px, X1);
+                            Y0 = fmaf(F, py, Y0);
+                            Y1 = fmaf(Fq, py, Y1);
+                        }
                     }
                 }
                 S.part[lane] = make_float4(K, fmaf(X1, ivx, X0), fmaf(Y1, ivy, Y0), 0.0f);
