@@ -85,6 +85,27 @@ class dh_batch(ctypes.Structure):
                 ("events", ctypes.c_void_p)]
 
 
+class dh_mlc_pair(ctypes.Structure):
+    _fields_ = [("x1", ctypes.c_float), ("x2", ctypes.c_float), ("y_offset", ctypes.c_float),
+                ("y_width", ctypes.c_float)]
+
+
+class dh_photon_model(ctypes.Structure):
+    _fields_ = [("profile", ctypes.c_void_p), ("spectrum", ctypes.c_void_p),
+                ("n_profile", ctypes.c_int32), ("n_spectrum", ctypes.c_int32),
+                ("kernel", (ctypes.c_float * 6) * 6), ("mu_cal", ctypes.c_float),
+                ("primary_src_dist", ctypes.c_float), ("scatter_src_dist", ctypes.c_float),
+                ("primary_src_size", ctypes.c_float), ("scatter_src_size", ctypes.c_float),
+                ("mlc_distance", ctypes.c_float), ("scatter_src_weight", ctypes.c_float),
+                ("electron_attenuation", ctypes.c_float), ("electron_src_weight", ctypes.c_float),
+                ("electron_fitted_dmax", ctypes.c_float)]
+
+
+class dh_photon_beam(ctypes.Structure):
+    _fields_ = [("geom", dh_beam_geom), ("sin_col", ctypes.c_float), ("cos_col", ctypes.c_float),
+                ("mlc", ctypes.c_void_p), ("n_pairs", ctypes.c_int32), ("mu", ctypes.c_float)]
+
+
 DH_TIME_RAYMARCH, DH_TIME_SMOOTH, DH_TIME_FORWARD, DH_TIME_ADJOINT, DH_TIME_NEED = range(5)
 DH_TIME_N = 5
 TIMED_KERNELS = {DH_TIME_RAYMARCH: "k_wet_raymarch_batch", DH_TIME_SMOOTH: "k_wet_smooth_tile_batch",
@@ -133,6 +154,16 @@ SIGNATURES = {
     "dh_batch_forward": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
     "dh_batch_adjoint": (ctypes.c_int, [ctypes.POINTER(dh_batch), ctypes.c_int32, _P]),
     "dh_batch_timing": (ctypes.c_int, [ctypes.POINTER(dh_batch), _P]),
+    "dh_photon_record_len": (ctypes.c_int64, [dh_grid]),
+    "dh_photon_density": (ctypes.c_int, [_P, _P, dh_grid, _P]),
+    "dh_photon_density_from_hu": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double), ctypes.c_int32, _P]),
+    "dh_photon_raytrace": (ctypes.c_int, [_P, _P, dh_grid, ctypes.POINTER(dh_photon_beam), _P]),
+    "dh_photon_terma": (ctypes.c_int, [_P, _P, _P, dh_grid, ctypes.POINTER(dh_photon_model),
+                                       ctypes.POINTER(dh_photon_beam), _P]),
+    "dh_photon_terma_records": (ctypes.c_int, [_P, _P, dh_grid, _P]),
+    "dh_photon_ccc": (ctypes.c_int, [_P, _P, _P, _P, dh_grid, ctypes.POINTER(dh_photon_model),
+                                     ctypes.POINTER(dh_photon_beam), ctypes.c_float, ctypes.c_int32, _P]),
 }
 
 _lib = None

@@ -16,8 +16,10 @@ DoseCUDA/dose_kernels/dosemodule.cu:104-374:
 
 The work runs on the HIP device ``gpu_id`` through libdosehip.so, on torch's
 current stream of that device; the call returns once the result is on the host,
-as the reference's blocking copy does.  The photon entry point
-(``photon_dose_cuda``) is not provided.
+as the reference's blocking copy does.
+
+``photon_dose_cuda(model, vol, cp, gpu_id)`` does the same for the photon
+collapsed-cone dose of one control point (photon_dose, dosemodule.cu:377-537).
 """
 
 import numpy as np
@@ -25,6 +27,7 @@ import torch
 
 from . import _native as N
 from . import device as D
+from . import photon as PH
 
 _native = N.load()  # fail at import if the HIP library is missing
 
@@ -112,5 +115,56 @@ def proton_spot_cuda(model, vol, wet, beam, gpu_id, cutoff_sigma=0.0):
         dose = torch.empty_like(d_wet)
         ws, nbytes = D.alloc_workspace(dspots.n_spots, dspots.n_layers, grid, dev)
         D.pb_forward(d_wet, dose, grid, geo, dmodel, dspots, ws, nbytes, cutoff_sigma=cutoff_sigma)
+        out = dose.cpu().numpy()
+    return np.ascontiguousarray(out)
+
+
+def _getbool(obj, attr):
+    # PyObject_IsTrue semantics (dosemodule.cu:91-101)
+    return bool(getattr(obj, attr))
+
+
+def photon_dose_cuda(model, vol, cp, gpu_id):
+    """Collapsed-cone dose of one control point (photon_dose,
+    dosemodule.cu:377-537): radiological depth through the density volume,
+    TERMA and electron contamination through the leaf pairs, then the sum over
+    6 cones x 12 azimuths.  Not multiplied by the output factor.
+
+    ``model.kernel`` ([6, 6], one row per cone) and ``cp.mlc`` ([n, 4], one row
+    (x1, x2, y_offset, y_width) per leaf pair) are read by index.  The
+    reference reads their raw buffers column-wise, which gives the same values
+    for the Fortran-ordered arrays IMRTPlan builds (the pandas table and the
+    transposed leaf stack, dosemodule.cu:228-240, IMRTClasses.cu:292-297)."""
+    for attr in ("mu_calibration", "primary_source_distance", "scatter_source_distance", "primary_source_size",
+                 "scatter_source_size", "mlc_distance", "scatter_source_weight", "electron_attenuation",
+                 "electron_source_weight", "electron_fitted_dmax", "jaw_transmission", "mlc_transmission"):
+        _getfloat(model, attr)
+    _getbool(model, "has_xjaws")
+    _getbool(model, "has_yjaws")
+    for attr in ("profile_radius", "profile_intensities", "profile_softening", "spectrum_attenuation_coefficients",
+                 "spectrum_primary_weights", "spectrum_scatter_weights"):
+        _getarray(model, attr, 1)
+    kern = _getarray(model, "kernel", 2)
+    density = _getarray(vol, "voxel_data", 3)
+    spacing = _getarray(vol, "spacing", 1)
+    origin = _getarray(vol, "origin", 1)
+    _getarray(cp, "iso", 1)
+    mlc = _getarray(cp, "mlc", 2)
+    mu = _getfloat(cp, "mu")
+    for attr in ("ga", "ca", "ta"):
+        _getfloat(cp, attr)
+    if kern.shape != (6, 6):
+        raise ValueError("'kernel' must be 2-dimensional and of type float.")
+    if mlc.shape[1] != 4:
+        raise ValueError("'mlc' must be 2-dimensional and of type float.")
+
+    dev = _device(gpu_id)
+    with torch.cuda.device(dev):
+        dmodel = PH.DevicePhotonModel(model, dev)
+        beam = PH.control_point(cp, origin, dmodel, dev, mu=mu)
+        ws = PH.PhotonWorkspace(density.shape, float(spacing[0]), dev)
+        ws.set_density(torch.from_numpy(np.ascontiguousarray(density)).to(dev))
+        dose = torch.empty(density.shape, dtype=torch.float32, device=dev)
+        PH.cp_dose(ws, dmodel, beam, dose)
         out = dose.cpu().numpy()
     return np.ascontiguousarray(out)

@@ -382,4 +382,26 @@ __device__ __forceinline__ int find_owner(const long long *__restrict__ off, int
     return lo;
 }
 
+// ---------------------------------------------------------------------------
+// Piecewise-linear calibration curve (HU -> RLSP, plan_impt.py:158-168; HU ->
+// density, plan_imrt.py:114-124), evaluated as np.interp in float64.
+constexpr int MAX_KNOTS = 32;
+struct Knots {
+    double hu[MAX_KNOTS];
+    double val[MAX_KNOTS];   // RLSP or density at the knots
+    int n;
+};
+
+__device__ __forceinline__ double interp_knots(const Knots &kn, double x) {
+    // np.interp: clamp outside [hu[0], hu[n-1]]; otherwise linear on the
+    // bracketing interval, slope * (x - hu[j]) + val[j].
+    if (!(x == x)) return x;  // NaN stays NaN
+    if (x <= kn.hu[0]) return kn.val[0];
+    if (x >= kn.hu[kn.n - 1]) return kn.val[kn.n - 1];
+    int j = 0;
+    for (int t = 1; t < kn.n - 1; ++t) j = (x >= kn.hu[t]) ? t : j;
+    double slope = __ddiv_rn(kn.val[j + 1] - kn.val[j], kn.hu[j + 1] - kn.hu[j]);
+    return __dadd_rn(__dmul_rn(slope, x - kn.hu[j]), kn.val[j]);
+}
+
 }  // namespace dh

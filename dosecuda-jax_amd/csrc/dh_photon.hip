@@ -1,0 +1,577 @@
+// Photon collapsed-cone convolution (CCC) dose of one control point on gfx950:
+// the photon half of the reference extension (photon_dose_cuda,
+// IMRTClasses.cu:353-405; kernels CudaClasses.cu:180-228 and
+// IMRTClasses.cu:209-350).
+//
+// In place of the reference's two border-addressed float textures (density,
+// TERMA) the kernels read a record volume over the grid padded by one voxel on
+// every side: per padded cell 32 bytes, the bilinear face forms of TERMA and of
+// density (include/dosehip.h).  A trilinear sample is two records' blend, four
+// 16-byte buffer loads, with the texture's border values (TERMA 0, density
+// AIR_DENSITY) built into the padding and returned outright for samples whose
+// every corner lies outside the grid.  Work mapping as in the proton kernels:
+// one wave per 4x4x4 brick of voxels, lane = voxel, four bricks per workgroup.
+#include "dh_common.h"
+
+namespace dh {
+
+constexpr float kAirDensity = 0.0012f;  // AIR_DENSITY, IMRTClasses.cuh:6
+
+struct PGrid {
+    int nx, ny, nz;      // voxels
+    int px, pxy;         // padded row and slice lengths (records)
+    float s, inv_s;
+    float hx, hy, hz;    // nx + 1, ny + 1, nz + 1: a sample at padded index c reads the grid iff c < h (and c >= 0)
+};
+
+static PGrid make_pgrid(const dh_grid &g) {
+    PGrid p;
+    p.nx = g.nx; p.ny = g.ny; p.nz = g.nz;
+    p.px = g.nx + 2;
+    p.pxy = (g.nx + 2) * (g.ny + 2);
+    p.s = g.spacing;
+    p.inv_s = 1.0f / g.spacing;
+    p.hx = (float)(g.nx + 1); p.hy = (float)(g.ny + 1); p.hz = (float)(g.nz + 1);
+    return p;
+}
+
+static long long record_count(const dh_grid &g) {
+    return (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+}
+
+static bool photon_grid_ok(const dh_grid &g, const char *what) {
+    if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0.0f)) {
+        set_error("%s: invalid grid %d x %d x %d, spacing %g", what, g.nx, g.ny, g.nz, (double)g.spacing);
+        return false;
+    }
+    // 32-bit byte offsets of the record loads
+    if (32.0 * (double)record_count(g) >= 4294967296.0) {
+        set_error("%s: grid %d x %d x %d too large (32 * (nx+2)(ny+2)(nz+2) must stay below 2^32)", what, g.nx, g.ny,
+                  g.nz);
+        return false;
+    }
+    return true;
+}
+
+// Image -> photon head frame (IMRTBeam::pointXYZImageToHead, IMRTClasses.cu:41-79):
+// couch about y, gantry about z, collimator about y, then (-x_c, z_c, y_c).
+__device__ __forceinline__ float3 ph_image_to_head(const Geom &b, float sca, float cca, float3 p) {
+    const float xt = p.x * b.cc - p.z * b.sc;
+    const float zt = p.x * b.sc + p.z * b.cc;
+    const float xg = xt * b.cg + p.y * b.sg;
+    const float yg = p.y * b.cg - xt * b.sg;
+    const float xc = xg * cca - zt * sca;
+    const float zc = xg * sca + zt * cca;
+    return make_float3(-xc, zc, yg);
+}
+
+// ---------------------------------------------------------------------------
+// Record volume: padded cell t -> (i', j', k'); value f(i', j', k') of the
+// voxel (i'-1, j'-1, k'-1) inside the grid, the border value outside.
+template <typename F>
+__device__ __forceinline__ float4 padded_quad(const PGrid &g, int i, int j, int k, float border, F val) {
+    auto f = [&](int a, int b, int c) -> float {
+        const bool in = a >= 1 && a <= g.nx && b >= 1 && b <= g.ny && c >= 1 && c <= g.nz;
+        return in ? val(a - 1, b - 1, c - 1) : border;
+    };
+    const float a = f(i, j, k), b = f(i + 1, j, k), c = f(i, j + 1, k), d = f(i + 1, j + 1, k);
+    return make_float4(a, b - a, c - a, (d - c) - (b - a));
+}
+
+__device__ __forceinline__ void padded_coords(const PGrid &g, long long t, int &i, int &j, int &k) {
+    const unsigned u = (unsigned)t;
+    const unsigned r = u / (unsigned)g.px;
+    i = (int)(u - r * (unsigned)g.px);
+    const unsigned py = (unsigned)(g.pxy / g.px);
+    k = (int)(r / py);
+    j = (int)(r - (unsigned)k * py);
+}
+
+__device__ __forceinline__ long long vox_index(const PGrid &g, int i, int j, int k) {
+    return i + (long long)g.nx * (j + (long long)g.ny * k);
+}
+
+// field 0: TERMA, 1: density
+__global__ __launch_bounds__(256) void k_ph_records_from(const float *__restrict__ src, float4 *__restrict__ rec,
+                                                         PGrid g, long long n, int field, float border) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    padded_coords(g, t, i, j, k);
+    gst(rec, 2 * t + field, padded_quad(g, i, j, k, border, [&](int a, int b, int c) {
+        return gld(src, vox_index(g, a, b, c));
+    }));
+}
+
+__global__ __launch_bounds__(256) void k_ph_density_from_hu(const float *__restrict__ hu, float4 *__restrict__ rec,
+                                                            PGrid g, Knots kn, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    padded_coords(g, t, i, j, k);
+    gst(rec, 2 * t + 1, padded_quad(g, i, j, k, kAirDensity, [&](int a, int b, int c) {
+        return (float)interp_knots(kn, (double)gld(hu, vox_index(g, a, b, c)));
+    }));
+}
+
+// ---------------------------------------------------------------------------
+// Trilinear samples from the records (buffer loads; byte offsets < 2^32).
+__device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t rs, unsigned off, int soff) {
+    const auto r = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, soff, 0);
+    return make_float4(__uint_as_float(r[0]), __uint_as_float(r[1]), __uint_as_float(r[2]), __uint_as_float(r[3]));
+}
+
+__device__ __forceinline__ float bilerp(float4 q, float ax, float ay) {
+    return fmaf(ay, fmaf(ax, q.w, q.z), fmaf(ax, q.y, q.x));
+}
+
+__device__ __forceinline__ unsigned record_offset(const PGrid &g, int ix, int iy, int iz) {
+    return ((unsigned)ix + __umul24((unsigned)iy, (unsigned)g.px) + (unsigned)iz * (unsigned)g.pxy) << 5;
+}
+
+// Density at padded index c (every corner inside the padded grid).
+__device__ __forceinline__ float sample_rho(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy, float cz) {
+    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    const unsigned off = record_offset(g, (int)fx, (int)fy, (int)fz);
+    const int so_z = g.pxy << 5;
+    const float ax = cx - fx, ay = cy - fy, az = cz - fz;
+    const float r0 = bilerp(load4(rs, off + 16u, 0), ax, ay);
+    const float r1 = bilerp(load4(rs, off + 16u, so_z), ax, ay);
+    return fmaf(az, r1 - r0, r0);
+}
+
+// TERMA and density at padded index c, border values where the texture would
+// read only border texels (some axis outside [0, n + 1)).
+__device__ __forceinline__ void sample_tr(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy, float cz,
+                                          float &te, float &rho) {
+    const bool in = cx >= 0.0f && cx < g.hx && cy >= 0.0f && cy < g.hy && cz >= 0.0f && cz < g.hz;
+    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    const unsigned off = in ? record_offset(g, (int)fx, (int)fy, (int)fz) : 0u;
+    const int so_z = g.pxy << 5;
+    const float ax = cx - fx, ay = cy - fy, az = cz - fz;
+    const float4 t0 = load4(rs, off, 0), d0 = load4(rs, off + 16u, 0);
+    const float4 t1 = load4(rs, off, so_z), d1 = load4(rs, off + 16u, so_z);
+    const float tt0 = bilerp(t0, ax, ay), tt1 = bilerp(t1, ax, ay);
+    const float dd0 = bilerp(d0, ax, ay), dd1 = bilerp(d1, ax, ay);
+    te = in ? fmaf(az, tt1 - tt0, tt0) : 0.0f;
+    rho = in ? fmaf(az, dd1 - dd0, dd0) : kAirDensity;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t record_rsrc(const float4 *rec, const PGrid &g) {
+    const unsigned bytes = (unsigned)g.pxy * (unsigned)(g.nz + 2) * 32u;
+    return __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)0, (int)bytes, 0x00020000);
+}
+
+// Brick mapping: workgroup q takes bricks 4q .. 4q + 3 (remapped per XCD), wave
+// w brick 4q + w, lane = x + 4 y + 16 z of the brick.
+__device__ __forceinline__ bool brick_voxel(const BrickGrid &bg, const PGrid &g, unsigned nblocks, int &i, int &j,
+                                            int &k) {
+    const unsigned q = xcd_remap(blockIdx.x, nblocks);
+    const long long brick = 4LL * q + (threadIdx.x >> 6);
+    if (brick >= bg.count()) return false;
+    int bi, bj, bk;
+    brick_coords(bg, brick, bi, bj, bk);
+    const int lane = threadIdx.x & 63;
+    i = bi * BRICK + (lane & 3);
+    j = bj * BRICK + ((lane >> 2) & 3);
+    k = bk * BRICK + (lane >> 4);
+    return i < g.nx && j < g.ny && k < g.nz;
+}
+
+__device__ __forceinline__ float3 ph_voxel_xyz(const PGrid &g, const Geom &b, int i, int j, int k) {
+    return make_float3(fmaf((float)i, g.s, -b.iso[0]), fmaf((float)j, g.s, -b.iso[1]),
+                       fmaf((float)k, g.s, -b.iso[2]));
+}
+
+// ---------------------------------------------------------------------------
+// Radiological depth (rayTraceKernel, CudaClasses.cu:180-228) with the
+// reference's sample positions p_n = fma(u, n, v), texture coordinate
+// fma(1/s, p + iso, 1/2) and bounds test [0, n) per axis.
+__global__ __launch_bounds__(256) void k_ph_raytrace(const float4 *__restrict__ rec, float *__restrict__ depth,
+                                                     PGrid g, Geom b, BrickGrid bg, unsigned nblocks) {
+    int i, j, k;
+    if (!brick_voxel(bg, g, nblocks, i, j, k)) return;
+    const float3 v = ph_voxel_xyz(g, b, i, j, k);
+    const float dx = b.src[0] - v.x, dy = b.src[1] - v.y, dz = b.src[2] - v.z;
+    const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float ux = dx / len, uy = dy / len, uz = dz / len;
+    const __amdgpu_buffer_rsrc_t rs = record_rsrc(rec, g);
+    const float nxf = (float)g.nx, nyf = (float)g.ny, nzf = (float)g.nz;
+    float acc = -0.05f;
+    for (int n = 0;; ++n) {
+        const float fn = (float)n;
+        const float tx = fmaf(g.inv_s, fmaf(ux, fn, v.x) + b.iso[0], 0.5f);
+        const float ty = fmaf(g.inv_s, fmaf(uy, fn, v.y) + b.iso[1], 0.5f);
+        const float tz = fmaf(g.inv_s, fmaf(uz, fn, v.z) + b.iso[2], 0.5f);
+        if (!(tx >= 0.0f && tx < nxf && ty >= 0.0f && ty < nyf && tz >= 0.0f && tz < nzf)) break;
+        const float rho = sample_rho(rs, g, tx + 0.5f, ty + 0.5f, tz + 0.5f);
+        acc = fmaf(fmaxf(rho, 0.0f), 0.1f, acc);
+    }
+    gst(depth, vox_index(g, i, j, k), acc);
+}
+
+// ---------------------------------------------------------------------------
+// TERMA and electron contamination (termaKernel, IMRTClasses.cu:209-250).
+struct PhotonTermaArgs {
+    const float *profile;   // [3][n_profile]
+    const float *spectrum;  // [3][n_spectrum]
+    const dh_mlc_pair *mlc;
+    int n_profile, n_spectrum, n_pairs;
+    float sad_p, sad_s, inv_sig_p, inv_sig_s;  // 1 / (sigma sqrt 2)
+    float mlc_distance, mlc_scale;             // (SAD - mlc_distance) / SAD
+    float ssw, ea, esw, el_floor, el_inv;      // exp(-ea dmax), 1 / (1 - exp(-ea dmax))
+    float sca, cca;
+};
+
+// offAxisFactors (IMRTClasses.cu:152-171): lowerBound of r in the radius table,
+// clamped at both ends, linear in between.
+__device__ __forceinline__ void off_axis(const PhotonTermaArgs &a, float r, float &oaf, float &soft) {
+    const float *rad = a.profile, *inten = a.profile + a.n_profile, *sf = a.profile + 2 * a.n_profile;
+    int lo = 0;  // first index with r <= rad[lo]
+    for (int t = 0; t < a.n_profile; ++t) lo += (r > rad[t]) ? 1 : 0;
+    if (lo == 0 || lo == a.n_profile) {
+        const int e = lo == 0 ? 0 : a.n_profile - 1;
+        oaf = inten[e];
+        soft = sf[e];
+        return;
+    }
+    const float m = (r - rad[lo - 1]) / (rad[lo] - rad[lo - 1]);
+    oaf = fmaf(inten[lo] - inten[lo - 1], m, inten[lo - 1]);
+    soft = fmaf(sf[lo] - sf[lo - 1], m, sf[lo - 1]);
+}
+
+// One source's aperture term of a leaf pair, 0 without evaluating the error
+// functions when the pair is closed or every edge lies 5 sigma sqrt 2 beyond the
+// point on one side (erff is then +-1 exactly in float32, so the term is 0).
+__device__ __forceinline__ float pair_term(float xl, float xr, float yb, float yt, float3 h, float div, float inv) {
+    const float tip1 = fmaf(xl - h.x, div, h.x) * inv, tip2 = fmaf(xr - h.x, div, h.x) * inv;
+    const float e1 = fmaf(yb - h.y, div, h.y) * inv, e2 = fmaf(yt - h.y, div, h.y) * inv;
+    const bool far = (tip1 >= 5.0f && tip2 >= 5.0f) || (tip1 <= -5.0f && tip2 <= -5.0f) ||
+                     (e1 >= 5.0f && e2 >= 5.0f) || (e1 <= -5.0f && e2 <= -5.0f);
+    if (far) return 0.0f;
+    const float ex = 0.5f * (erff(tip2) - erff(tip1));
+    const float ey = 0.5f * (erff(e2) - erff(e1));
+    return ex * ey;
+}
+
+__global__ __launch_bounds__(256) void k_ph_terma(const float *__restrict__ depth, float *__restrict__ terma,
+                                                  float *__restrict__ electron, PGrid g, Geom b, PhotonTermaArgs a,
+                                                  BrickGrid bg, unsigned nblocks) {
+    int i, j, k;
+    if (!brick_voxel(bg, g, nblocks, i, j, k)) return;
+    const float3 h = ph_image_to_head(b, a.sca, a.cca, ph_voxel_xyz(g, b, i, j, k));
+    const float d_p = a.sad_p - h.z, d_s = a.sad_s - h.z;
+    float oaf, soft;
+    off_axis(a, hypotf(h.x, h.y) * d_p / a.sad_p, oaf, soft);
+    // headTransmission (IMRTClasses.cu:120-150) for the primary and the scatter source
+    const float div_p = (a.sad_p - h.z) / (a.mlc_distance - h.z);
+    const float div_s = (a.sad_s - h.z) / (a.mlc_distance - h.z);
+    float tp = 0.0f, ts = 0.0f;
+    for (int p = 0; p < a.n_pairs; ++p) {
+        const dh_mlc_pair m = a.mlc[p];
+        if (m.x1 == m.x2) continue;  // closed pair: both tips equal, the term is exactly 0
+        const float yb = (m.y_offset - 0.5f * m.y_width) * a.mlc_scale;
+        const float yt = (m.y_offset + 0.5f * m.y_width) * a.mlc_scale;
+        const float xl = m.x1 * a.mlc_scale, xr = m.x2 * a.mlc_scale;
+        tp += pair_term(xl, xr, yb, yt, h, div_p, a.inv_sig_p);
+        ts += pair_term(xl, xr, yb, yt, h, div_s, a.inv_sig_s);
+    }
+    const long long v = vox_index(g, i, j, k);
+    const float wet = gld(depth, v);
+    const float tr = fminf(1.0f, ts / tp);  // fminf: a NaN quotient gives 1
+    const float inv_sq = (a.sad_p / d_p) * (a.sad_p / d_p);
+    const float inv_s = a.sad_s / d_s;
+    float te = 0.0f;
+    for (int e = 0; e < a.n_spectrum; ++e) {
+        const float att = __expf(-a.spectrum[e] * wet * soft);
+        te += (1.0f - tr) * (a.spectrum[a.n_spectrum + e] * att * inv_sq) +
+              tr * a.spectrum[2 * a.n_spectrum + e] * att * inv_s;
+    }
+    const float el = fmaxf(0.0f, (__expf(-a.ea * wet) - a.el_floor) * a.el_inv);
+    gst(terma, v, oaf * ((1.0f - a.ssw) * tp * te + a.ssw * ts * te));
+    gst(electron, v, a.esw * (0.4f + 0.3f * tr) * el * tp);
+}
+
+// ---------------------------------------------------------------------------
+// Collapsed-cone dose (cccKernel, IMRTClasses.cu:252-350).
+constexpr int N_CONES = 6;
+constexpr int N_AZ = 12;
+struct PhotonCccArgs {
+    float t[N_CONES * N_AZ][3];  // image-space cone directions before the tilt (head -> image of the head-frame set)
+    float k1[N_CONES];           // sin(theta) Am / am^2
+    float k2[N_CONES];           // sin(theta) Bm / bm
+    float am_l2[N_CONES];        // am log2(e)
+    float am[N_CONES], bm[N_CONES];
+    float len0[N_CONES];         // initial ray length (cm, float32 counter)
+    float ax[3];                 // unit vector from the isocentre to the source
+    float sp;                    // spacing / 10 (cm)
+    float mu_cal, mu, scale;
+    int accumulate;
+};
+
+__global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, const float *__restrict__ terma,
+                                                const float *__restrict__ electron, float *__restrict__ dose, PGrid g,
+                                                Geom b, PhotonCccArgs a, BrickGrid bg, unsigned nblocks) {
+    int i, j, k;
+    if (!brick_voxel(bg, g, nblocks, i, j, k)) return;
+    const long long v = vox_index(g, i, j, k);
+    if (!(gld(terma, v) > 0.01f)) {
+        if (!a.accumulate) gst(dose, v, 0.0f);
+        return;
+    }
+    const float3 p = ph_voxel_xyz(g, b, i, j, k);
+    // kernelTilt (IMRTClasses.cu:173-203): rotation about a x u taking the
+    // isocentre's direction a to the voxel's direction u to the source, as the
+    // matrix R = c I + [k]x + k k^T / (1 + c), k = a x u, c = a . u
+    const float dx = b.src[0] - p.x, dy = b.src[1] - p.y, dz = b.src[2] - p.z;
+    const float rn = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    const float ux = dx * rn, uy = dy * rn, uz = dz * rn;
+    const float c = a.ax[0] * ux + a.ax[1] * uy + a.ax[2] * uz;
+    const float kx = a.ax[1] * uz - a.ax[2] * uy, ky = a.ax[2] * ux - a.ax[0] * uz, kz = a.ax[0] * uy - a.ax[1] * ux;
+    const float f = 1.0f / (1.0f + c);
+    const float r00 = fmaf(kx * kx, f, c), r01 = fmaf(kx * ky, f, -kz), r02 = fmaf(kx * kz, f, ky);
+    const float r10 = fmaf(ky * kx, f, kz), r11 = fmaf(ky * ky, f, c), r12 = fmaf(ky * kz, f, -kx);
+    const float r20 = fmaf(kz * kx, f, -ky), r21 = fmaf(kz * ky, f, kx), r22 = fmaf(kz * kz, f, c);
+    // padded continuous index of the voxel (texture coordinate + 1/2)
+    const float c0x = fmaf(p.x + b.iso[0], g.inv_s, 1.0f);
+    const float c0y = fmaf(p.y + b.iso[1], g.inv_s, 1.0f);
+    const float c0z = fmaf(p.z + b.iso[2], g.inv_s, 1.0f);
+    const __amdgpu_buffer_rsrc_t rs = record_rsrc(rec, g);
+    const float sp = a.sp;
+    float sum = 0.0f;
+    for (int cone = 0; cone < N_CONES; ++cone) {
+        const float k1 = a.k1[cone], k2 = a.k2[cone], am_l2 = a.am_l2[cone], bm = a.bm[cone];
+        float cone_sum_p = 0.0f, cone_sum_s = 0.0f;
+        for (int az = 0; az < N_AZ; ++az) {
+            const float *t0 = a.t[cone * N_AZ + az];
+            // tilted direction, in voxels per cm of ray length (x 10 mm / s)
+            const float sc10 = 10.0f * g.inv_s;
+            const float ddx = (r00 * t0[0] + r01 * t0[1] + r02 * t0[2]) * sc10;
+            const float ddy = (r10 * t0[0] + r11 * t0[1] + r12 * t0[2]) * sc10;
+            const float ddz = (r20 * t0[0] + r21 * t0[1] + r22 * t0[2]) * sc10;
+            float rp = 0.0f, rsum = 0.0f;
+            for (float rl = a.len0[cone]; rl >= 0.0f; rl = rl - sp) {
+                float te, rho;
+                sample_tr(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), te, rho);
+                const float di = fmaxf(rho, kAirDensity) * sp;
+                const float e = __builtin_amdgcn_exp2f(-am_l2 * di);
+                rp = fmaf(rp, e, te * k1 * (1.0f - e));
+                rsum = fmaf(rsum, fmaf(-bm, di, 1.0f), te * di * k2);
+            }
+            cone_sum_p += rp;
+            cone_sum_s += rsum;
+        }
+        sum = fmaf(a.am[cone], cone_sum_p, fmaf(bm, cone_sum_s, sum));
+    }
+    sum += gld(electron, v);
+    const float d = (sum == sum && sum >= 0.0f) ? a.mu_cal * sum * a.mu : 0.0f;
+    gst(dose, v, a.accumulate ? fmaf(a.scale, d, gld(dose, v)) : a.scale * d);
+}
+
+static unsigned brick_blocks(const BrickGrid &bg) { return (unsigned)((bg.count() + 3) / 4); }
+
+}  // namespace dh
+
+using namespace dh;
+
+extern "C" int64_t dh_photon_record_len(dh_grid g) {
+    if (!photon_grid_ok(g, "dh_photon_record_len")) return -1;
+    return 8 * record_count(g);
+}
+
+static int records_ok(const void *rec, const char *what) {
+    if (!rec || ((uintptr_t)rec & 31u)) {
+        set_error("%s: the record volume must be a non-null, 32-byte aligned device pointer", what);
+        return 0;
+    }
+    return 1;
+}
+
+extern "C" int dh_photon_density(const float *density, float *records, dh_grid g, void *stream) {
+    if (!photon_grid_ok(g, "dh_photon_density")) return DH_ERR_ARG;
+    if (!density || !records_ok(records, "dh_photon_density")) {
+        if (!density) set_error("dh_photon_density: null density");
+        return DH_ERR_ARG;
+    }
+    const long long n = record_count(g);
+    hipLaunchKernelGGL(k_ph_records_from, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       density, reinterpret_cast<float4 *>(records), make_pgrid(g), n, 1, kAirDensity);
+    return check_launch("k_ph_records_from");
+}
+
+extern "C" int dh_photon_density_from_hu(const float *hu, float *records, dh_grid g, const double *hu_knots,
+                                         const double *density_knots, int32_t n_knots, void *stream) {
+    if (!photon_grid_ok(g, "dh_photon_density_from_hu")) return DH_ERR_ARG;
+    if (!records_ok(records, "dh_photon_density_from_hu")) return DH_ERR_ARG;
+    if (!hu || !hu_knots || !density_knots || n_knots < 2 || n_knots > MAX_KNOTS) {
+        set_error("dh_photon_density_from_hu: bad arguments (n_knots=%d, max %d)", n_knots, MAX_KNOTS);
+        return DH_ERR_ARG;
+    }
+    Knots kn;
+    kn.n = n_knots;
+    for (int t = 0; t < MAX_KNOTS; ++t) kn.hu[t] = kn.val[t] = 0.0;
+    for (int t = 0; t < n_knots; ++t) {
+        kn.hu[t] = hu_knots[t];
+        kn.val[t] = density_knots[t];
+        if (t && !(hu_knots[t] > hu_knots[t - 1])) {
+            set_error("dh_photon_density_from_hu: HU knots must be strictly ascending");
+            return DH_ERR_ARG;
+        }
+    }
+    const long long n = record_count(g);
+    hipLaunchKernelGGL(k_ph_density_from_hu, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, hu,
+                       reinterpret_cast<float4 *>(records), make_pgrid(g), kn, n);
+    return check_launch("k_ph_density_from_hu");
+}
+
+extern "C" int dh_photon_terma_records(const float *terma, float *records, dh_grid g, void *stream) {
+    if (!photon_grid_ok(g, "dh_photon_terma_records")) return DH_ERR_ARG;
+    if (!terma || !records_ok(records, "dh_photon_terma_records")) {
+        if (!terma) set_error("dh_photon_terma_records: null TERMA");
+        return DH_ERR_ARG;
+    }
+    const long long n = record_count(g);
+    hipLaunchKernelGGL(k_ph_records_from, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, terma,
+                       reinterpret_cast<float4 *>(records), make_pgrid(g), n, 0, 0.0f);
+    return check_launch("k_ph_records_from");
+}
+
+static bool beam_ok(const dh_photon_beam *beam, const char *what) {
+    if (!beam) {
+        set_error("%s: null beam", what);
+        return false;
+    }
+    return true;
+}
+
+extern "C" int dh_photon_raytrace(const float *records, float *depth, dh_grid g, const dh_photon_beam *beam,
+                                  void *stream) {
+    if (!photon_grid_ok(g, "dh_photon_raytrace")) return DH_ERR_ARG;
+    if (!records_ok(records, "dh_photon_raytrace") || !beam_ok(beam, "dh_photon_raytrace")) return DH_ERR_ARG;
+    if (!depth) {
+        set_error("dh_photon_raytrace: null output");
+        return DH_ERR_ARG;
+    }
+    const PGrid pg = make_pgrid(g);
+    const BrickGrid bg = brick_grid(make_grid(g));
+    const unsigned nb = brick_blocks(bg);
+    hipLaunchKernelGGL(k_ph_raytrace, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float4 *>(records), depth, pg, make_geom(beam->geom), bg, nb);
+    return check_launch("k_ph_raytrace");
+}
+
+static bool model_ok(const dh_photon_model *m, const char *what) {
+    if (!m || !m->profile || !m->spectrum || m->n_profile < 1 || m->n_spectrum < 0) {
+        set_error("%s: bad model (null tables or empty profile)", what);
+        return false;
+    }
+    return true;
+}
+
+extern "C" int dh_photon_terma(const float *depth, float *terma, float *electron, dh_grid g,
+                               const dh_photon_model *model, const dh_photon_beam *beam, void *stream) {
+    if (!photon_grid_ok(g, "dh_photon_terma")) return DH_ERR_ARG;
+    if (!model_ok(model, "dh_photon_terma") || !beam_ok(beam, "dh_photon_terma")) return DH_ERR_ARG;
+    if (!depth || !terma || !electron || (beam->n_pairs > 0 && !beam->mlc) || beam->n_pairs < 0) {
+        set_error("dh_photon_terma: null volume or leaf-pair array");
+        return DH_ERR_ARG;
+    }
+    PhotonTermaArgs a;
+    a.profile = model->profile;
+    a.spectrum = model->spectrum;
+    a.mlc = beam->mlc;
+    a.n_profile = model->n_profile;
+    a.n_spectrum = model->n_spectrum;
+    a.n_pairs = beam->n_pairs;
+    a.sad_p = model->primary_src_dist;
+    a.sad_s = model->scatter_src_dist;
+    a.inv_sig_p = 1.0f / (model->primary_src_size * sqrtf(2.0f));
+    a.inv_sig_s = 1.0f / (model->scatter_src_size * sqrtf(2.0f));
+    a.mlc_distance = model->mlc_distance;
+    a.mlc_scale = (model->primary_src_dist - model->mlc_distance) / model->primary_src_dist;
+    a.ssw = model->scatter_src_weight;
+    a.ea = model->electron_attenuation;
+    a.esw = model->electron_src_weight;
+    a.el_floor = expf(-model->electron_attenuation * model->electron_fitted_dmax);
+    a.el_inv = 1.0f / (1.0f - a.el_floor);
+    a.sca = beam->sin_col;
+    a.cca = beam->cos_col;
+    const PGrid pg = make_pgrid(g);
+    const BrickGrid bg = brick_grid(make_grid(g));
+    const unsigned nb = brick_blocks(bg);
+    hipLaunchKernelGGL(k_ph_terma, dim3(nb), dim3(256), 0, (hipStream_t)stream, depth, terma, electron, pg,
+                       make_geom(beam->geom), a, bg, nb);
+    return check_launch("k_ph_terma");
+}
+
+// Image-space cone directions of a control point: the head-frame set
+// (sin th cos ph, sin th sin ph, cos th), ph = j pi / 6, taken to the image
+// frame by the inverse of the photon head transform (IMRTClasses.cu:81-118,
+// 286-309), in float32 as the reference evaluates them.
+static void cone_directions(const dh_photon_model &m, const dh_photon_beam &b, PhotonCccArgs &a) {
+    const float pi = 3.14159265358979f;
+    const dh_beam_geom &gm = b.geom;
+    for (int c = 0; c < N_CONES; ++c) {
+        const float th = m.kernel[c][0] * pi / 180.0f;
+        const float st = sinf(th), ct = cosf(th);
+        const float Am = m.kernel[c][1], am = m.kernel[c][2], Bm = m.kernel[c][3], bm = m.kernel[c][4];
+        a.k1[c] = st * (Am / (am * am));
+        a.k2[c] = st * (Bm / bm);
+        a.am[c] = am;
+        a.bm[c] = bm;
+        a.am_l2[c] = am * 1.44269504088896341f;
+        a.len0[c] = m.kernel[c][5];
+        for (int j = 0; j < N_AZ; ++j) {
+            const float phi = (float)j * pi / 6.0f;
+            const float hx = st * cosf(phi), hy = st * sinf(phi), hz = ct;
+            // head -> image: undo the collimator, gantry and couch rotations
+            const float xz = -hx, yz = hz, zz = hy;
+            const float xc = xz * b.cos_col + zz * b.sin_col;
+            const float yc = yz;
+            const float zc = -xz * b.sin_col + zz * b.cos_col;
+            const float xg = xc * gm.cos_g - yc * gm.sin_g;
+            const float yg = xc * gm.sin_g + yc * gm.cos_g;
+            const float zg = zc;
+            float *t = a.t[c * N_AZ + j];
+            t[0] = xg * gm.cos_c + zg * gm.sin_c;
+            t[1] = yg;
+            t[2] = -xg * gm.sin_c + zg * gm.cos_c;
+        }
+    }
+    const float n = sqrtf(gm.src[0] * gm.src[0] + gm.src[1] * gm.src[1] + gm.src[2] * gm.src[2]);
+    for (int d = 0; d < 3; ++d) a.ax[d] = gm.src[d] / n;
+}
+
+extern "C" int dh_photon_ccc(const float *records, const float *terma, const float *electron, float *dose, dh_grid g,
+                             const dh_photon_model *model, const dh_photon_beam *beam, float scale, int32_t accumulate,
+                             void *stream) {
+    if (!photon_grid_ok(g, "dh_photon_ccc")) return DH_ERR_ARG;
+    if (!records_ok(records, "dh_photon_ccc") || !beam_ok(beam, "dh_photon_ccc")) return DH_ERR_ARG;
+    if (!model || !terma || !electron || !dose) {
+        set_error("dh_photon_ccc: null model or volume");
+        return DH_ERR_ARG;
+    }
+    const float sp = g.spacing / 10.0f;
+    for (int c = 0; c < N_CONES; ++c) {
+        // the ray-length counter must reach below zero in a bounded number of steps
+        if (!(model->kernel[c][5] >= 0.0f) || !(model->kernel[c][5] / sp < 4096.0f)) {
+            set_error("dh_photon_ccc: cone %d ray length %g cm is negative or beyond 4096 steps of %g cm", c,
+                      (double)model->kernel[c][5], (double)sp);
+            return DH_ERR_ARG;
+        }
+    }
+    PhotonCccArgs a;
+    cone_directions(*model, *beam, a);
+    a.sp = sp;
+    a.mu_cal = model->mu_cal;
+    a.mu = beam->mu;
+    a.scale = scale;
+    a.accumulate = accumulate ? 1 : 0;
+    const PGrid pg = make_pgrid(g);
+    const BrickGrid bg = brick_grid(make_grid(g));
+    const unsigned nb = brick_blocks(bg);
+    hipLaunchKernelGGL(k_ph_ccc, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float4 *>(records), terma, electron, dose, pg, make_geom(beam->geom), a,
+                       bg, nb);
+    return check_launch("k_ph_ccc");
+}

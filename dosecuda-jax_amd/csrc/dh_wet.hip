@@ -111,24 +111,7 @@ __global__ __launch_bounds__(256) void k_block_radius_set(float2 *__restrict__ t
 
 // ---------------------------------------------------------------------------
 // HU -> RLSP (IMPTDoseGrid.RLSPFromHU, plan_impt.py:158-168)
-constexpr int MAX_KNOTS = 32;
-struct Knots {
-    double hu[MAX_KNOTS];
-    double rlsp[MAX_KNOTS];
-    int n;
-};
-
-__device__ __forceinline__ double interp_knots(const Knots &kn, double x) {
-    // np.interp: clamp outside [hu[0], hu[n-1]]; otherwise linear on the
-    // bracketing interval, slope * (x - hu[j]) + rlsp[j].
-    if (!(x == x)) return x;  // NaN stays NaN
-    if (x <= kn.hu[0]) return kn.rlsp[0];
-    if (x >= kn.hu[kn.n - 1]) return kn.rlsp[kn.n - 1];
-    int j = 0;
-    for (int t = 1; t < kn.n - 1; ++t) j = (x >= kn.hu[t]) ? t : j;
-    double slope = __ddiv_rn(kn.rlsp[j + 1] - kn.rlsp[j], kn.hu[j + 1] - kn.hu[j]);
-    return __dadd_rn(__dmul_rn(slope, x - kn.hu[j]), kn.rlsp[j]);
-}
+// (Knots, interp_knots: dh_common.h)
 
 __device__ __forceinline__ void quad_coords(const Grid &g, long long t, int &i, int &j, int &k) {
     i = (int)(t % g.nx);
@@ -922,13 +905,13 @@ extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, c
     kn.n = n_knots;
     for (int t = 0; t < n_knots; ++t) {
         kn.hu[t] = hu_knots[t];
-        kn.rlsp[t] = rlsp_knots[t];
+        kn.val[t] = rlsp_knots[t];
         if (t && !(hu_knots[t] > hu_knots[t - 1])) {
             set_error("dh_rlsp_from_hu: HU knots must be strictly ascending");
             return DH_ERR_ARG;
         }
     }
-    for (int t = n_knots; t < MAX_KNOTS; ++t) kn.hu[t] = kn.rlsp[t] = 0.0;
+    for (int t = n_knots; t < MAX_KNOTS; ++t) kn.hu[t] = kn.val[t] = 0.0;
     const long long nquad = quad_count(g.nx, g.ny, g.nz);
     const unsigned nb = (unsigned)((nquad + 255) / 256);
     hipLaunchKernelGGL(k_rlsp_from_hu, dim3(nb), dim3(256), 0, (hipStream_t)stream, hu,

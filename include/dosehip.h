@@ -1,5 +1,6 @@
 /*
- * dosehip.h - C ABI of the gfx950 proton pencil-beam dose library (libdosehip.so).
+ * dosehip.h - C ABI of the gfx950 proton pencil-beam and photon
+ * collapsed-cone dose library (libdosehip.so).
  *
  * Replaces the proton half of the reference extension `dose_kernels`
  * (DoseCUDA/dose_kernels/dosemodule.cu:104-374, IMPTClasses.cu:274-340):
@@ -374,6 +375,101 @@ int dh_batch_forward(dh_batch *batch, int32_t flags, void *stream);
 /* Spot gradients of every beam from its plan's grad_dose (dh_pb_adjoint per beam).
  * The three phase calls update batch->prepared (host side), hence the non-const batch. */
 int dh_batch_adjoint(dh_batch *batch, int32_t flags, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Photon collapsed-cone convolution (CCC) dose of one control point.
+ * Replaces the photon half of `dose_kernels`:
+ *   photon_dose_cuda (dosemodule.cu:377-557, photon_dose_cuda
+ *   IMRTClasses.cu:353-405)  ->  dh_photon_density[_from_hu], dh_photon_raytrace,
+ *                                dh_photon_terma, dh_photon_terma_records,
+ *                                dh_photon_ccc
+ *
+ * Density and TERMA are read through a "record" volume in place of the
+ * reference's two border-addressed 3-D textures (TextureClasses.cu:14-41):
+ * one record per cell of the grid padded by one voxel on every side,
+ * (nx+2)*(ny+2)*(nz+2) records, index i' + (nx+2)*(j' + (ny+2)*k'), each
+ * 8 floats: the bilinear form (a, b-a, c-a, (d-c)-(b-a)) of TERMA on the
+ * slice-k' face of the cell, then the same of density, where a = f(i',j',k'),
+ * b = f(i'+1,j',k'), c = f(i',j'+1,k'), d = f(i'+1,j'+1,k') and f(i',j',k') is
+ * the voxel value at (i'-1, j'-1, k'-1) inside the grid and the texture's
+ * border value outside it (TERMA 0, density AIR_DENSITY = 0.0012,
+ * IMRTClasses.cuh:6).  A sample at continuous voxel index c (texture
+ * coordinate c + 1/2) is then two records' blend at c + 1, with the border
+ * values wherever the texture would read only border texels.
+ * dh_photon_record_len gives the floats of a record volume (32-byte aligned).
+ * ---------------------------------------------------------------------- */
+
+/* MLC leaf pair (MLCPair, IMRTClasses.cuh:8-15), mm at the isocentre plane. */
+typedef struct dh_mlc_pair {
+    float x1, x2;        /* leaf tips along x */
+    float y_offset;      /* pair centre along y */
+    float y_width;
+} dh_mlc_pair;
+
+/* Photon beam model (IMRTBeam::Model, IMRTClasses.cuh:22-58).  Tables are
+ * device pointers; the six collapsed cones are passed by value. */
+typedef struct dh_photon_model {
+    const float *profile;    /* [3][n_profile]: radius (mm, ascending), intensity, softening */
+    const float *spectrum;   /* [3][n_spectrum]: attenuation (cm^2/g), primary weight, scatter weight */
+    int32_t n_profile;
+    int32_t n_spectrum;
+    float kernel[6][6];      /* per cone: theta (deg), Am, am, Bm, bm, ray length (cm) */
+    float mu_cal;
+    float primary_src_dist, scatter_src_dist;   /* mm */
+    float primary_src_size, scatter_src_size;   /* Gaussian sigma, mm */
+    float mlc_distance;                         /* mm from the isocentre toward the source */
+    float scatter_src_weight;
+    float electron_attenuation, electron_src_weight, electron_fitted_dmax;
+} dh_photon_model;
+
+/* One control point (IMRTBeam, IMRTClasses.cuh:20-83; dosemodule.cu:493-536).
+ * geom: as for protons with the source at primary_src_dist (vsadx / vsady
+ * unused); sin_col / cos_col of the collimator angle; mlc: device array. */
+typedef struct dh_photon_beam {
+    dh_beam_geom geom;
+    float sin_col, cos_col;
+    const dh_mlc_pair *mlc;
+    int32_t n_pairs;
+    float mu;
+} dh_photon_beam;
+
+/* Floats of a record volume for grid g. */
+int64_t dh_photon_record_len(dh_grid g);
+
+/* Density half of the records from a density volume (nz*ny*nx floats; the
+ * reference's DensityTexture, IMRTClasses.cu:394). */
+int dh_photon_density(const float *density, float *records, dh_grid g, void *stream);
+
+/* The same from HU, density = np.interp(HU, hu_knots, density_knots) rounded
+ * to float32 (IMRTDoseGrid.DensityFromHU, plan_imrt.py:114-124); n_knots <= 32. */
+int dh_photon_density_from_hu(const float *hu, float *records, dh_grid g, const double *hu_knots,
+                              const double *density_knots, int32_t n_knots, void *stream);
+
+/* Radiological depth (g/cm^2) of every voxel through the density
+ * (rayTraceKernel, CudaClasses.cu:180-228): samples 1 mm apart from the voxel
+ * toward the source while the texture coordinate is inside the grid,
+ * -0.05 + 0.1 * sum max(density, 0). */
+int dh_photon_raytrace(const float *records, float *depth, dh_grid g, const dh_photon_beam *beam, void *stream);
+
+/* TERMA and electron contamination of every voxel (termaKernel,
+ * IMRTClasses.cu:209-250): off-axis profile, Gaussian primary and scatter
+ * sources seen through the leaf pairs, spectrum attenuation at the depth. */
+int dh_photon_terma(const float *depth, float *terma, float *electron, dh_grid g, const dh_photon_model *model,
+                    const dh_photon_beam *beam, void *stream);
+
+/* TERMA half of the records from a TERMA volume (the reference's TERMATexture,
+ * IMRTClasses.cu:399). */
+int dh_photon_terma_records(const float *terma, float *records, dh_grid g, void *stream);
+
+/* Collapsed-cone dose (cccKernel, IMRTClasses.cu:252-350): per voxel with
+ * TERMA > 0.01, 6 cones x 12 azimuths tilted toward the source, each ray
+ * marched from its length down to the voxel in steps of one spacing;
+ * dose = mu_cal * mu * (sum + electron) where that is a non-negative number,
+ * else 0.  Writes dose[v] = scale * that (accumulate == 0) or adds it
+ * (accumulate != 0). */
+int dh_photon_ccc(const float *records, const float *terma, const float *electron, float *dose, dh_grid g,
+                  const dh_photon_model *model, const dh_photon_beam *beam, float scale, int32_t accumulate,
+                  void *stream);
 
 #ifdef __cplusplus
 }

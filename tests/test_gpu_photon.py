@@ -1,0 +1,294 @@
+"""Photon collapsed-cone path on the GPU (libdosehip.so dh_photon_*) against
+the float64 oracle (oracle/imrt_oracle.*), stage by stage and end to end.
+Reference: photon_dose_cuda (dosemodule.cu:377-537, IMRTClasses.cu:353-405),
+rayTraceKernel (CudaClasses.cu:180-228), termaKernel / cccKernel
+(IMRTClasses.cu:209-350), computeIMRTPlan (plan_imrt.py:126-158).
+
+Tolerances: depth 2e-4 g/cm^2 (the proton WET bar); TERMA / electron 1e-5 of
+their maximum on the same depth; dose 1e-4 of D_max on the same TERMA, and
+end to end (each side its own depth and TERMA) 1e-3 of D_max with the 2%/2 mm
+gamma pass rate >= 0.995 (the north-star bar is 5e-3).  A voxel whose TERMA
+lies within 1e-4 of the 0.01 threshold of cccKernel may fall on either side
+in float32 and float64; such voxels are counted and left out of end-to-end
+comparisons."""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import imrt_oracle as OR
+from oracle import impt_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PHOT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dosecuda-jax_amd", "DoseCUDA",
+                    "lookuptables", "photons")
+
+
+def _say(msg):
+    print("[gpu-photon] " + msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda", 0)
+
+
+@pytest.fixture(scope="module")
+def varian():
+    from DoseCUDA import IMRTPlan
+    return IMRTPlan("VarianTrueBeamHF")
+
+
+def _hetero_grid(shape, spacing, seed):
+    from DoseCUDA import IMRTDoseGrid
+    g = IMRTDoseGrid()
+    g.spacing = np.array([spacing] * 3, dtype=np.float64)
+    g.size = np.array(shape)
+    g.origin = np.array([-shape[2] * spacing / 2.0, -shape[1] * spacing / 2.0, -shape[0] * spacing / 2.0])
+    rng = np.random.default_rng(seed)
+    hu = np.full(shape, -1000.0, np.float32)
+    m = 2
+    hu[m:-m, m:-m, m:-m] = rng.normal(20.0, 30.0, size=tuple(n - 2 * m for n in shape))
+    nz, ny, nx = shape
+    hu[nz // 3:nz // 3 + 4, m:-m, m:-m] = rng.uniform(700, 1200, size=(4, ny - 2 * m, nx - 2 * m))  # bone slab
+    hu[nz // 2:nz // 2 + 5, ny // 2:ny // 2 + 6, nx // 3:nx // 3 + 6] = -800.0  # lung-like pocket
+    g.HU = hu
+    return g
+
+
+def _cp(plan, model_index, ga, ta, ca, half_x, half_y, mu=100.0, iso=(0.0, 0.0, 0.0), irregular=False):
+    from DoseCUDA.plan_imrt import IMRTControlPoint
+    bm = plan.beam_models[model_index]
+    leaves = np.zeros((2, bm.n_mlc_pairs), np.float32)
+    leaves[0], leaves[1] = -half_x, half_x
+    out = (bm.mlc_offsets < -half_y) | (bm.mlc_offsets > half_y)
+    leaves[:, out] = 0.0
+    if irregular:
+        rng = np.random.default_rng(int(ga) + 7)
+        open_ = ~out
+        leaves[0, open_] += rng.uniform(-10, 10, open_.sum()).astype(np.float32)
+        leaves[1, open_] += rng.uniform(-10, 10, open_.sum()).astype(np.float32)
+    mlc = plan._mlc_array(leaves)
+    return IMRTControlPoint(np.array(iso, np.float32), mu, mlc, ga, ca, ta, None, None)
+
+
+def _stages(g, model, cp, dev, density=None):
+    """GPU depth, TERMA, electron and dose of one control point (device pipeline)."""
+    from DoseCUDA import photon as PH
+    rho = OR.density_from_hu(g.HU, os.path.join(PHOT, "VarianTrueBeamHF")) if density is None else density
+    with torch.cuda.device(dev):
+        dm = PH.DevicePhotonModel(model, dev)
+        beam = PH.control_point(cp, np.asarray(g.origin, np.float32), dm, dev)
+        ws = PH.PhotonWorkspace(rho.shape, float(np.float32(g.spacing[0])), dev)
+        ws.set_density(torch.from_numpy(rho).to(dev))
+        dose = torch.empty(rho.shape, dtype=torch.float32, device=dev)
+        PH.cp_dose(ws, dm, beam, dose)
+        torch.cuda.synchronize()
+        return rho, ws.depth.cpu().numpy(), ws.terma.cpu().numpy(), ws.electron.cpu().numpy(), dose.cpu().numpy()
+
+
+def _geom(g, model, cp):
+    return OR.photon_geometry(np.asarray(cp.iso, np.float32), np.asarray(g.origin, np.float32), cp.ga, cp.ta,
+                              cp.ca, np.float32(model.primary_source_distance))
+
+
+def test_density_records_from_hu(dev):
+    """The density half of the record volume equals np.interp of HU (rounded
+    to float32), with AIR_DENSITY in the padding."""
+    from DoseCUDA import photon as PH
+    _say("density records from HU")
+    rng = np.random.default_rng(1)
+    hu = rng.uniform(-1100, 3200, size=(5, 6, 7)).astype(np.float32)
+    tab = np.loadtxt(os.path.join(PHOT, "VarianTrueBeamHF", "HU_Density.csv"), delimiter=",", skiprows=1)
+    want = np.interp(hu, tab[:, 0], tab[:, 1]).astype(np.float32)
+    with torch.cuda.device(dev):
+        ws = PH.PhotonWorkspace(hu.shape, 2.0, dev)
+        ws.set_density_from_hu(torch.from_numpy(hu).to(dev), tab[:, 0], tab[:, 1])
+        rec = ws.records.view(7, 8, 9, 8).cpu().numpy()   # (k', j', i', 8 floats)
+        torch.cuda.synchronize()
+    a = rec[..., 4]
+    inner = a[1:-1, 1:-1, 1:-1]
+    np.testing.assert_array_equal(inner, want)
+    border = np.ones_like(a, bool)
+    border[1:-1, 1:-1, 1:-1] = False
+    assert np.all(a[border] == np.float32(0.0012))
+    # the face form: a + (b - a) is the +x neighbour b up to the rounding of the float32 difference
+    np.testing.assert_allclose(rec[1:-1, 1:-1, 1:-2, 4] + rec[1:-1, 1:-1, 1:-2, 5], want[:, :, 1:], rtol=0,
+                               atol=4e-7)
+
+
+@pytest.mark.parametrize("ga,ta,ca,spacing", [(0.0, 0.0, 0.0, 2.0), (90.0, 0.0, 30.0, 2.5), (217.0, 35.0, 80.0, 3.0)])
+def test_stages_vs_oracle(dev, varian, ga, ta, ca, spacing):
+    """Depth vs the oracle; TERMA / electron vs the oracle on the GPU's depth;
+    dose vs the oracle on the GPU's TERMA, density and electron."""
+    _say("stages ga=%g ta=%g ca=%g s=%g" % (ga, ta, ca, spacing))
+    g = _hetero_grid((26, 30, 28), spacing, seed=int(ga))
+    model = varian.beam_models[0]
+    cp = _cp(varian, 0, ga, ta, ca, 20.0, 25.0, mu=80.0, irregular=True)
+    rho, depth, terma, el, dose = _stages(g, model, cp, dev)
+    geom = _geom(g, model, cp)
+    s = float(np.float32(g.spacing[0]))
+    d_ref = OR.raytrace(rho, s, geom)
+    assert np.max(np.abs(depth - d_ref)) < 2e-4, np.max(np.abs(depth - d_ref))
+    t_ref, e_ref = OR.terma(depth.astype(np.float64), rho.shape, s, geom, model, OR.mlc_rows(cp.mlc))
+    assert np.max(np.abs(terma - t_ref)) < 1e-5 * t_ref.max(), np.max(np.abs(terma - t_ref)) / t_ref.max()
+    assert np.max(np.abs(el - e_ref)) < 1e-5 * max(e_ref.max(), 1e-30)
+    dose_ref = OR.ccc(terma, rho, el.astype(np.float64), s, geom, model.kernel, np.float32(model.mu_calibration),
+                      np.float32(cp.mu))
+    err = np.max(np.abs(dose - dose_ref)) / dose_ref.max()
+    _say("  depth %.2e, TERMA %.2e, dose %.2e of D_max" % (np.max(np.abs(depth - d_ref)),
+                                                           np.max(np.abs(terma - t_ref)) / t_ref.max(), err))
+    assert dose_ref.max() > 0 and err < 1e-4, err
+
+
+def test_dropin_two_field_plan_vs_oracle(dev, varian):
+    """computeIMRTPlan (drop-in, host arrays) with two fields on a 60^3 @ 3 mm
+    cube vs the full-volume oracle: 1e-3 of D_max, gamma 2%/2 mm >= 0.995."""
+    from DoseCUDA import IMRTDoseGrid, IMRTPlan
+    _say("drop-in two-field plan, 60^3 @ 3 mm")
+    g = IMRTDoseGrid()
+    g.createCubePhantom(size=[60, 60, 60], spacing=3.0)
+    plan = IMRTPlan("VarianTrueBeamHF")
+    plan.n_fractions = 2
+    plan.addSquareField(dimx=6, dimy=5)
+    plan.addSquareField('10', 4.0, 7.0, mu=50, gantry_angle=90.0, collimator_angle=20.0)
+    g.computeIMRTPlan(plan)
+    ref, ref_beams = OR.compute_plan(g, plan, os.path.join(PHOT, "VarianTrueBeamHF"))
+    m = O.compare(g.dose, ref)
+    gam = O.gamma_pass_rate(g.dose, ref, 3.0)
+    _say("  max|dD|/Dmax %.2e, gamma %.4f" % (m["max_rel_to_dmax"], gam))
+    assert m["max_rel_to_dmax"] < 1e-3 and gam >= 0.995, (m, gam)
+    for b, rb in zip(g.beam_doses, ref_beams):
+        assert np.max(np.abs(b - rb)) < 1e-3 * rb.max()
+
+
+def test_reference_example_script_sampled(dev, varian, tmp_path):
+    """The reference's own IMRT example (tests/test_phantom_imrt.py): default
+    cube 138^3 @ 3 mm, a 10x10 field and a 15x15 field (50 MU, gantry 90,
+    collimator 20), PHYSICAL dose written as NRRD.  Oracle at 3000 voxels
+    (half above 10% of D_max): 1e-3 of D_max; the file equals the array."""
+    from DoseCUDA import IMRTDoseGrid, IMRTPlan
+    from DoseCUDA.imageio import read_nrrd
+    _say("reference IMRT example, 138^3 @ 3 mm")
+    dose = IMRTDoseGrid()
+    plan = IMRTPlan("VarianTrueBeamHF")
+    dose.createCubePhantom()
+    plan.addSquareField()
+    plan.addSquareField('6', 15.0, 15.0, mu=50, gantry_angle=90.0, collimator_angle=20.0)
+    dose.computeIMRTPlan(plan)
+    dose.writeDoseNRRD(str(tmp_path / "cube_imrt_dose.nrrd"), individual_beams=False, dose_type="PHYSICAL")
+    d, _, _ = read_nrrd(str(tmp_path / "cube_imrt_dose.nrrd"))
+    assert np.array_equal(d, dose.dose)
+
+    rng = np.random.default_rng(11)
+    flat = dose.dose.ravel()
+    hi = np.flatnonzero(flat > 0.1 * flat.max())
+    idx = np.unique(np.concatenate([rng.choice(hi, 1500, replace=False), rng.choice(flat.size, 1500, replace=False),
+                                    [int(flat.argmax())]]))
+    rho = OR.density_from_hu(dose.HU, os.path.join(PHOT, "VarianTrueBeamHF"))
+    s = float(np.float32(dose.spacing[0]))
+    ref = np.zeros(len(idx))
+    skipped = 0
+    for beam in plan.beam_list:
+        model = plan.beam_models[list(np.asarray(plan.dicom_energy_label).astype(str)).index(beam.dicom_energy_label)]
+        for cp in beam.cp_list:
+            geom = _geom(dose, model, cp)
+            depth = OR.raytrace(rho, s, geom)
+            te, el = OR.terma(depth.astype(np.float32).astype(np.float64), rho.shape, s, geom, model,
+                              OR.mlc_rows(cp.mlc))
+            t32 = te.astype(np.float32)
+            near = np.abs(te.ravel()[idx] - 0.01) < 1e-4
+            skipped += int(near.sum())
+            ref += OR.ccc(t32, rho, el.astype(np.float32).ravel()[idx].astype(np.float64), s, geom, model.kernel,
+                          np.float32(model.mu_calibration), np.float32(cp.mu), idx=idx) * OR.output_factor(model, cp)
+            idx_keep = ~near
+            idx, ref = idx[idx_keep], ref[idx_keep]
+    got = flat[idx]
+    err = np.max(np.abs(got - ref)) / ref.max()
+    _say("  %d voxels, %d near the TERMA threshold left out, max|dD|/Dmax %.2e" % (len(idx), skipped, err))
+    assert skipped < 20 and err < 1e-3, err
+
+
+def test_engine_matches_dropin_and_is_deterministic(dev, varian):
+    """PhotonDoseEngine (density from HU on the device, beams accumulated on
+    the device) equals the drop-in's plan dose to float rounding, and two runs
+    are bit-identical."""
+    from DoseCUDA import IMRTDoseGrid, IMRTPlan
+    from DoseCUDA.photon import PhotonDoseEngine
+    from DoseCUDA.plan_imrt import IMRTBeam
+    _say("engine vs drop-in")
+    g = _hetero_grid((30, 34, 32), 2.5, seed=5)
+    plan = IMRTPlan("VarianTrueBeamHF")
+    beam = IMRTBeam()
+    beam.dicom_energy_label = '6'
+    for k, ga in enumerate((10.0, 12.0, 14.0)):   # an arc-like beam: three segments
+        beam.addControlPoint(_cp(plan, 0, ga, 5.0, 15.0, 15.0 + 5 * k, 20.0, mu=30.0 + k, irregular=True))
+    plan.addBeam(beam)
+    plan.addSquareField('10', 5.0, 5.0, mu=40, gantry_angle=250.0, collimator_angle=45.0, table_angle=10.0)
+    g.computeIMRTPlan(plan)
+    eng = PhotonDoseEngine(g, plan, gpu_id=0)
+    d1, b1 = eng.compute()
+    d1 = d1.cpu().numpy().copy()
+    d2 = eng.compute()[0].cpu().numpy()
+    assert np.array_equal(d1, d2)
+    assert np.max(np.abs(d1 - g.dose)) < 1e-5 * g.dose.max()
+    for b, hb in zip(b1, g.beam_doses):
+        assert np.max(np.abs(b.cpu().numpy() - hb)) < 1e-5 * hb.max()
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 3, 4), (5, 1, 7)])
+def test_degenerate_grids(dev, varian, shape):
+    _say("degenerate grid %s" % (shape,))
+    g = _hetero_grid((8, 8, 8), 3.0, seed=2)
+    g.HU = np.random.default_rng(3).uniform(-200, 300, size=shape).astype(np.float32)
+    g.size = np.array(shape)
+    model = varian.beam_models[0]
+    cp = _cp(varian, 0, 30.0, 10.0, 5.0, 50.0, 50.0)
+    rho, depth, terma, el, dose = _stages(g, model, cp, dev)
+    geom = _geom(g, model, cp)
+    s = float(np.float32(g.spacing[0]))
+    assert np.max(np.abs(depth - OR.raytrace(rho, s, geom))) < 2e-4
+    dose_ref = OR.ccc(terma, rho, el.astype(np.float64), s, geom, model.kernel, np.float32(model.mu_calibration),
+                      np.float32(cp.mu))
+    assert np.max(np.abs(dose - dose_ref)) <= 1e-4 * max(dose_ref.max(), 1e-30)
+
+
+def test_closed_leaves_give_no_dose(dev, varian):
+    """Every leaf pair closed: zero transmission, zero TERMA and zero dose
+    (the 0/0 ratio of the two transmissions reads as 1, as fminf gives)."""
+    _say("closed leaves")
+    g = _hetero_grid((10, 12, 11), 3.0, seed=4)
+    model = varian.beam_models[0]
+    cp = _cp(varian, 0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    rho, depth, terma, el, dose = _stages(g, model, cp, dev)
+    assert np.all(terma == 0) and np.all(el == 0) and np.all(dose == 0)
+
+
+def test_dropin_argument_errors(dev, varian):
+    """photon_dose_cuda raises the reference's exceptions (dosemodule.cu:50-66)."""
+    from DoseCUDA import dose_kernels
+    from DoseCUDA.plan import VolumeObject
+    _say("drop-in argument errors")
+    vol = VolumeObject()
+    vol.voxel_data = np.zeros((4, 4, 4), np.float64)
+    vol.spacing = np.array([2.0] * 3, np.float32)
+    vol.origin = np.zeros(3, np.float32)
+    cp = _cp(varian, 0, 0.0, 0.0, 0.0, 10.0, 10.0)
+    with pytest.raises(ValueError, match="'voxel_data' must be 3-dimensional and of type float."):
+        dose_kernels.photon_dose_cuda(varian.beam_models[0], vol, cp, 0)
+    vol.voxel_data = np.ones((4, 4, 4), np.float32)
+
+    class NoMu:
+        pass
+    bad = NoMu()
+    for k, v in vars(cp).items():
+        if k != "mu":
+            setattr(bad, k, v)
+    with pytest.raises(AttributeError):
+        dose_kernels.photon_dose_cuda(varian.beam_models[0], vol, bad, 0)
+    out = dose_kernels.photon_dose_cuda(varian.beam_models[0], vol, cp, 0)
+    assert out.shape == (4, 4, 4) and out.dtype == np.float32

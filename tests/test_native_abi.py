@@ -222,3 +222,49 @@ def test_layer_table_matches_oracle(plan_models):
     sl[5, 3] = -2
     order, ids = sort_spots(sl)
     assert layer_table(ids, dvp)[0].shape == (0, 3)
+
+
+def test_photon_struct_layout_and_errors_without_gpu(tmp_path):
+    """dh_mlc_pair / dh_photon_model / dh_photon_beam as C lays them out, and the
+    photon entry points' argument checks (no kernel launched)."""
+    from DoseCUDA import _native as N
+    src = tmp_path / "layout_ph.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "dosehip.h"
+int main(void) {
+  printf("%zu %zu %zu\n", sizeof(dh_mlc_pair), sizeof(dh_photon_model), sizeof(dh_photon_beam));
+  printf("%zu %zu %zu %zu %zu %zu\n", offsetof(dh_photon_model, kernel), offsetof(dh_photon_model, mu_cal),
+         offsetof(dh_photon_model, electron_fitted_dmax), offsetof(dh_photon_beam, sin_col),
+         offsetof(dh_photon_beam, mlc), offsetof(dh_photon_beam, mu));
+  return 0;
+}
+''')
+    exe = tmp_path / "layout_ph"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    assert [int(v) for v in out[0].split()] == [ctypes.sizeof(c) for c in (N.dh_mlc_pair, N.dh_photon_model,
+                                                                            N.dh_photon_beam)]
+    assert [int(v) for v in out[1].split()] == [N.dh_photon_model.kernel.offset, N.dh_photon_model.mu_cal.offset,
+                                                N.dh_photon_model.electron_fitted_dmax.offset,
+                                                N.dh_photon_beam.sin_col.offset, N.dh_photon_beam.mlc.offset,
+                                                N.dh_photon_beam.mu.offset]
+    lib = N.load()
+    g = N.dh_grid(10, 12, 14, 2.0)
+    assert lib.dh_photon_record_len(g) == 8 * 12 * 14 * 16
+    assert lib.dh_photon_record_len(N.dh_grid(0, 12, 14, 2.0)) < 0
+    assert lib.dh_photon_record_len(N.dh_grid(2000, 2000, 2000, 1.0)) < 0   # 32-bit record offsets
+    assert "too large" in N.last_error()
+    beam = N.dh_photon_beam()
+    model = N.dh_photon_model()
+    assert lib.dh_photon_density(None, None, g, None) == N.DH_ERR_ARG
+    assert lib.dh_photon_raytrace(None, None, g, ctypes.byref(beam), None) == N.DH_ERR_ARG
+    assert lib.dh_photon_terma(None, None, None, g, ctypes.byref(model), ctypes.byref(beam), None) == N.DH_ERR_ARG
+    assert lib.dh_photon_terma_records(None, None, g, None) == N.DH_ERR_ARG
+    assert lib.dh_photon_ccc(None, None, None, None, g, ctypes.byref(model), ctypes.byref(beam), 1.0, 0,
+                             None) == N.DH_ERR_ARG
+    # a misaligned record volume is refused before any launch
+    assert lib.dh_photon_raytrace(ctypes.c_void_p(16), ctypes.c_void_p(64), g, ctypes.byref(beam), None) == \
+        N.DH_ERR_ARG
+    assert "32-byte aligned" in N.last_error()
