@@ -418,6 +418,128 @@ def run_body(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_photon(args):
+    """Photon collapsed-cone plan on the head phantom: 7 beams x 5 MLC segments (35 control points); step =
+    the whole plan's dose on the device (per control point: radiological depth, TERMA, TERMA records,
+    collapsed-cone sum accumulated with its output factor)."""
+    import numpy as np
+    import torch
+    from DoseCUDA import photon as PH
+    from DoseCUDA.workloads import imrt_head_plan
+    log("building the photon workload")
+    g, plan = imrt_head_plan()
+    t0 = time.perf_counter()
+    eng = PH.PhotonDoseEngine(g, plan, gpu_id=0)
+    dev = eng.device
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    n_cp = eng.control_points()
+    s = float(np.float32(g.spacing[0]))
+    per_voxel = PH.samples_per_voxel(plan.beam_models[0], s)
+    # work accounting (outside the timed region): voxels with TERMA > 0.01 per control point
+    in_field = []
+    for dmodel, cps in eng.beams:
+        for cpc, _, _ in cps:
+            PH.raytrace(eng.ws, cpc)
+            PH.terma(eng.ws, dmodel, cpc)
+            in_field.append(int((eng.ws.terma > 0.01).sum().item()))
+    samples = per_voxel * sum(in_field)
+    for _ in range(max(1, args.warmup)):
+        eng.compute()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.compute()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    log("photon plan: %.2f ms per plan (%d control points)" % (ms, n_cp))
+    # per-kernel HIP events on the launch stream (torch's current stream), one extra plan
+    st = torch.cuda.current_stream(dev)
+    names = ("k_ph_raytrace", "k_ph_terma+k_ph_records_from", "k_ph_ccc")
+    acc = {k: 0.0 for k in names}
+    ev = []
+    last = None
+    for (dmodel, cps), bd in zip(eng.beams, eng.beam_doses):
+        for cpc, of, key in cps:
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e[0].record(st)
+            if key != last:   # as PhotonDoseEngine.compute: one ray-trace per beam geometry
+                PH.raytrace(eng.ws, cpc)
+                last = key
+            e[1].record(st)
+            PH.terma(eng.ws, dmodel, cpc)
+            e[2].record(st)
+            PH.ccc(eng.ws, dmodel, cpc, bd, of, True)
+            e[3].record(st)
+            ev.append(e)
+    torch.cuda.synchronize()
+    for e in ev:
+        for k, name in enumerate(names):
+            acc[name] += e[k].elapsed_time(e[k + 1])
+    n_trace = eng.depth_traces()
+    ccc_ms = acc["k_ph_ccc"] / n_cp
+    flop = 64.0 * samples / n_cp   # per launch: 64 FLOP per cone-ray sample (two trilinear fields, recursion)
+    nvox = int(np.prod(g.HU.shape))
+    # CPU baseline: float32 oracle on a voxel sample of control point 0 (the GPU's TERMA volume as input)
+    from oracle import imrt_oracle as OR
+    dmodel, cps = eng.beams[0]
+    cpc, _, _ = cps[0]
+    cp0 = plan.beam_list[0].cp_list[0]
+    PH.raytrace(eng.ws, cpc)
+    PH.terma(eng.ws, dmodel, cpc)
+    t32 = eng.ws.terma.cpu().numpy()
+    el = eng.ws.electron.cpu().numpy().astype(np.float64)
+    rho = OR.density_from_hu(g.HU, os.path.join(PKG, "DoseCUDA", "lookuptables", "photons", plan.machine_name))
+    geom = OR.photon_geometry(np.asarray(cp0.iso, np.float32), np.asarray(g.origin, np.float32), cp0.ga, cp0.ta,
+                              cp0.ca, np.float32(plan.beam_models[0].primary_source_distance))
+    rng = np.random.default_rng(0)
+    hot = np.flatnonzero(t32.reshape(-1) > 0.01)
+    idx = np.sort(rng.choice(hot, size=min(args.photon_cpu_sample, hot.size), replace=False)).astype(np.int64)
+    tc0 = time.perf_counter()
+    dep = OR.raytrace(rho, s, geom, idx=idx, precision="f32")
+    OR.terma(dep, rho.shape, s, geom, plan.beam_models[0], OR.mlc_rows(cp0.mlc), idx=idx, precision="f32")
+    OR.ccc(t32, rho, el.reshape(-1)[idx], s, geom, plan.beam_models[0].kernel,
+           np.float32(plan.beam_models[0].mu_calibration), np.float32(cp0.mu), idx=idx, precision="f32")
+    tc = time.perf_counter() - tc0
+    cpu_samples_per_s = idx.size * per_voxel / tc
+    cpu_cp_per_s = cpu_samples_per_s / (samples / n_cp)
+    rec = {
+        "metric": "photon collapsed-cone plan dose: ms per plan and control points/s",
+        "value": n_cp / (ms * 1e-3),
+        "unit": "control points/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "head phantom 160x200x200 @ 2 mm on an IMRTDoseGrid, VarianTrueBeamHF 6 MV, "
+                               "7 beams (gantry 0..308.6 deg) x 5 step-and-shoot MLC segments = %d control points; "
+                               "step = the plan's dose on the device (density from HU once, at set-up)" % n_cp,
+                   "voxels": nvox, "setup_s": t_setup},
+        "in_field_voxels_per_cp": float(np.mean(in_field)),
+        "samples_per_in_field_voxel": per_voxel,
+        "cone_samples_per_s": samples / (ms * 1e-3),
+        "kernels": {"k_ph_raytrace": {"avg_ms_per_launch": acc["k_ph_raytrace"] / n_trace,
+                                      "launches_per_plan": n_trace},
+                    "k_ph_terma+k_ph_records_from": {"avg_ms_per_cp": acc["k_ph_terma+k_ph_records_from"] / n_cp},
+                    "k_ph_ccc": {"avg_ms_per_cp": acc["k_ph_ccc"] / n_cp}},
+        "roofline": {"kernel": "k_ph_ccc", "bound": "valu", "achieved": flop / (ccc_ms * 1e-3) / 1e12,
+                     "peak": 157.3, "unit": "TFLOP/s", "frac": flop / (ccc_ms * 1e-3) / 1e12 / 157.3,
+                     "traffic": load_traffic("k_ph_ccc"), "avg_launch_ms": ccc_ms, "flop_per_sample": 64.0,
+                     "samples_per_launch": samples / n_cp},
+        "cpu_baseline": {"value": cpu_cp_per_s, "unit": "control points/s", "cores": os.cpu_count(),
+                         "kind": "port", "cpu_model": cpu_model(),
+                         "sample": "%d in-field voxels of control point 0: depth, TERMA and the cone sum at "
+                                   "them (float32 oracle build, OpenMP on %d threads), %.3g cone samples in "
+                                   "%.1f s, scaled to the plan's %.3g samples per control point" %
+                                   (idx.size, os.cpu_count(), idx.size * per_voxel, tc, samples / n_cp)},
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,11 +554,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=800000,
                     help="voxels in the CPU-baseline sample (about 10-15 s of CPU work on 16 cores); 0 = skip")
     ap.add_argument("--no-tune", action="store_true", help="library default schedules instead of the autotuner's")
-    ap.add_argument("--workload", choices=("head", "uneven", "body"), default="head",
+    ap.add_argument("--workload", choices=("head", "uneven", "body", "photon"), default="head",
                     help="head: the headline metric (default); uneven: 64 uneven plans batched; body: 512x512x300 "
                          "optimisation loop (each prints its own JSON line)")
     ap.add_argument("--iters", type=int, default=50, help="optimisation iterations of the body workload")
     ap.add_argument("--uneven-plans", type=int, default=64, help="plans of the uneven workload")
+    ap.add_argument("--photon-cpu-sample", type=int, default=2000000,
+                    help="photon workload: in-field voxels of the CPU baseline's sample")
     ap.add_argument("--eager-only", action="store_true",
                     help="time eager batched steps only (no graph), e.g. under a profiler")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -468,6 +592,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP GPU"
     if args.workload == "uneven":
         return run_uneven(args)
+    if args.workload == "photon":
+        return run_photon(args)
     if args.workload == "body":
         return run_body(args)
     K = args.plans

@@ -125,19 +125,34 @@ class PhotonWorkspace:
                                             N.stream_handle(self.device)), "dh_photon_density_from_hu")
 
 
+def raytrace(ws, beam):
+    """Radiological depth of every voxel (rayTraceKernel on the density)."""
+    N.check(N.load().dh_photon_raytrace(N.ptr(ws.records), N.ptr(ws.depth), ws.grid, N.ctypes.byref(beam.c),
+                                        N.stream_handle(ws.device)), "dh_photon_raytrace")
+
+
+def terma(ws, dmodel, beam):
+    """TERMA and electron contamination of every voxel, then the TERMA half of the records."""
+    L = N.load()
+    st = N.stream_handle(ws.device)
+    N.check(L.dh_photon_terma(N.ptr(ws.depth), N.ptr(ws.terma), N.ptr(ws.electron), ws.grid,
+                              N.ctypes.byref(dmodel.c), N.ctypes.byref(beam.c), st), "dh_photon_terma")
+    N.check(L.dh_photon_terma_records(N.ptr(ws.terma), N.ptr(ws.records), ws.grid, st), "dh_photon_terma_records")
+
+
+def ccc(ws, dmodel, beam, dose, scale=1.0, accumulate=False):
+    """Collapsed-cone sum into ``dose`` (dose = scale * D, or += with accumulate)."""
+    N.check(N.load().dh_photon_ccc(N.ptr(ws.records), N.ptr(ws.terma), N.ptr(ws.electron), N.ptr(dose), ws.grid,
+                                   N.ctypes.byref(dmodel.c), N.ctypes.byref(beam.c), float(scale),
+                                   1 if accumulate else 0, N.stream_handle(ws.device)), "dh_photon_ccc")
+
+
 def cp_dose(ws, dmodel, beam, dose, scale=1.0, accumulate=False):
     """Depth, TERMA / electron, TERMA records, then the collapsed-cone sum of
     one control point into ``dose`` (dose = scale * D, or += with accumulate)."""
-    L = N.load()
-    st = N.stream_handle(ws.device)
-    bp = N.ctypes.byref(beam.c)
-    N.check(L.dh_photon_raytrace(N.ptr(ws.records), N.ptr(ws.depth), ws.grid, bp, st), "dh_photon_raytrace")
-    N.check(L.dh_photon_terma(N.ptr(ws.depth), N.ptr(ws.terma), N.ptr(ws.electron), ws.grid,
-                              N.ctypes.byref(dmodel.c), bp, st), "dh_photon_terma")
-    N.check(L.dh_photon_terma_records(N.ptr(ws.terma), N.ptr(ws.records), ws.grid, st), "dh_photon_terma_records")
-    N.check(L.dh_photon_ccc(N.ptr(ws.records), N.ptr(ws.terma), N.ptr(ws.electron), N.ptr(dose), ws.grid,
-                            N.ctypes.byref(dmodel.c), bp, float(scale), 1 if accumulate else 0, st),
-            "dh_photon_ccc")
+    raytrace(ws, beam)
+    terma(ws, dmodel, beam)
+    ccc(ws, dmodel, beam, dose, scale, accumulate)
 
 
 class PhotonDoseEngine:
@@ -169,8 +184,8 @@ class PhotonDoseEngine:
                 if li not in self.models:
                     self.models[li] = DevicePhotonModel(plan.beam_models[li], self.device)
                 model = plan.beam_models[li]
-                cps = [(control_point(cp, self.origin, self.models[li], self.device), float(model.outputFactor(cp)))
-                       for cp in beam.cp_list]
+                cps = [(control_point(cp, self.origin, self.models[li], self.device), float(model.outputFactor(cp)),
+                        _depth_key(cp)) for cp in beam.cp_list]
                 self.beams.append((self.models[li], cps))
             self.beam_doses = [torch.zeros(self.shape, dtype=torch.float32, device=self.device)
                                for _ in self.beams]
@@ -178,19 +193,38 @@ class PhotonDoseEngine:
 
     def compute(self):
         """Beam doses and the plan dose (device tensors), enqueued on the
-        current stream."""
+        current stream.  The radiological depth depends on the isocentre,
+        gantry and couch angles only (not on the leaves, collimator or
+        meterset), so consecutive control points with the same three reuse it:
+        a step-and-shoot beam traces its rays once."""
         with torch.cuda.device(self.device):
             self.dose.zero_()
+            last = None
             for (dmodel, cps), bd in zip(self.beams, self.beam_doses):
                 bd.zero_()
-                for cpc, of in cps:
-                    cp_dose(self.ws, dmodel, cpc, bd, scale=of, accumulate=True)
+                for cpc, of, key in cps:
+                    if key != last:
+                        raytrace(self.ws, cpc)
+                        last = key
+                    terma(self.ws, dmodel, cpc)
+                    ccc(self.ws, dmodel, cpc, bd, scale=of, accumulate=True)
                 self.dose.add_(bd)
             self.dose.mul_(float(self.plan.n_fractions))
         return self.dose, self.beam_doses
 
+    def depth_traces(self):
+        """Ray-traces one compute() runs (changes of the depth key)."""
+        keys = [key for _, cps in self.beams for _, _, key in cps]
+        return sum(1 for i, k in enumerate(keys) if i == 0 or k != keys[i - 1])
+
     def control_points(self):
         return sum(len(cps) for _, cps in self.beams)
+
+
+def _depth_key(cp):
+    """What the radiological depth of a control point depends on: isocentre,
+    gantry and couch angle (CudaBeam, CudaClasses.cu:17-51)."""
+    return (tuple(float(v) for v in np.asarray(cp.iso, np.float32)[:3]), float(cp.ga), float(cp.ta))
 
 
 def cone_steps(model, spacing):

@@ -20,6 +20,10 @@ as createCubePhantom, plan.py:226-232) and spots in ``spot_list`` rows
                      two lungs U(-850, -700), spine U(300, 1100)), gantries
                      0/90/180/270, 50 layers (RS0 ids 25..74) x spots on a
                      3 mm grid in a 30 mm disk (~63k spots), 30 mm target
+* imrt_head_plan     photons: the head phantom on an IMRTDoseGrid, a 7-field
+                     step-and-shoot plan (gantries 0, 51.4, ..., 308.6 deg),
+                     5 MLC segments per beam shrinking from 80x80 mm with
+                     random leaf offsets, 10-40 MU each (35 control points)
 """
 
 import numpy as np
@@ -202,3 +206,32 @@ def body_plan(shape=(300, 512, 512), spacing=1.0, seed=5, gantries=(0.0, 90.0, 1
         plan.addBeam(_beam(ga, sl))
     target = np.broadcast_to(X ** 2 + Y ** 2 + Z ** 2 <= 30.0 ** 2, shape)
     return g, plan, np.ascontiguousarray(target)
+
+
+def imrt_head_plan(seed=2, n_beams=7, segments=5, machine="VarianTrueBeamHF"):
+    """Photon step-and-shoot plan on the head phantom.  Returns
+    (IMRTDoseGrid, IMRTPlan)."""
+    from .plan_imrt import IMRTDoseGrid, IMRTPlan, IMRTBeam, IMRTControlPoint
+    hg, _ = head_phantom(seed=seed)
+    g = IMRTDoseGrid()
+    g.spacing, g.size, g.origin, g.HU = hg.spacing, hg.size, hg.origin, hg.HU
+    rng = np.random.default_rng(seed + 100)
+    plan = IMRTPlan(machine)
+    bm = plan.beam_models[0]
+    for b in range(n_beams):
+        beam = IMRTBeam()
+        beam.dicom_energy_label = str(plan.dicom_energy_label[0])
+        ga = 360.0 * b / n_beams
+        ca = float(rng.choice([0.0, 10.0, 350.0]))
+        for s in range(segments):
+            half = 40.0 - 6.0 * s
+            leaves = np.zeros((2, bm.n_mlc_pairs), np.float32)
+            open_ = (bm.mlc_offsets >= -half) & (bm.mlc_offsets <= half)
+            leaves[0, open_] = (-half + rng.uniform(-8, 8, open_.sum())).astype(np.float32)
+            leaves[1, open_] = (half + rng.uniform(-8, 8, open_.sum())).astype(np.float32)
+            leaves[1] = np.maximum(leaves[1], leaves[0])
+            mlc = plan._mlc_array(leaves)
+            beam.addControlPoint(IMRTControlPoint(np.zeros(3, np.float32), float(rng.uniform(10, 40)), mlc, ga, ca,
+                                                  0.0, None, None))
+        plan.addBeam(beam)
+    return g, plan

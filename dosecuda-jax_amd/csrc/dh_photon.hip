@@ -5,11 +5,13 @@
 //
 // In place of the reference's two border-addressed float textures (density,
 // TERMA) the kernels read a record volume over the grid padded by one voxel on
-// every side: per padded cell 32 bytes, the bilinear face forms of TERMA and of
-// density (include/dosehip.h).  A trilinear sample is two records' blend, four
-// 16-byte buffer loads, with the texture's border values (TERMA 0, density
-// AIR_DENSITY) built into the padding and returned outright for samples whose
-// every corner lies outside the grid.  Work mapping as in the proton kernels:
+// every side (include/dosehip.h): per padded cell 32 bytes, the bilinear face
+// forms of TERMA and of density side by side (the cone sum reads both from the
+// same cache lines), then a second copy of the density forms alone, 16 bytes
+// per cell (the ray-trace reads only density: half the lines).  A trilinear
+// sample is two cells' forms blended, 16-byte buffer loads, with the texture's
+// border values (TERMA 0, density AIR_DENSITY) built into the padding and
+// returned outright for samples whose every corner lies outside the grid.  Work mapping as in the proton kernels:
 // one wave per 4x4x4 brick of voxels, lane = voxel, four bricks per workgroup.
 #include "dh_common.h"
 
@@ -22,6 +24,7 @@ struct PGrid {
     int px, pxy;         // padded row and slice lengths (records)
     float s, inv_s;
     float hx, hy, hz;    // nx + 1, ny + 1, nz + 1: a sample at padded index c reads the grid iff c < h (and c >= 0)
+    unsigned rho_base;   // byte offset of the density-only quads (32 bytes x padded cells)
 };
 
 static PGrid make_pgrid(const dh_grid &g) {
@@ -32,6 +35,7 @@ static PGrid make_pgrid(const dh_grid &g) {
     p.s = g.spacing;
     p.inv_s = 1.0f / g.spacing;
     p.hx = (float)(g.nx + 1); p.hy = (float)(g.ny + 1); p.hz = (float)(g.nz + 1);
+    p.rho_base = (unsigned)(32LL * (g.nx + 2) * (g.ny + 2) * (g.nz + 2));
     return p;
 }
 
@@ -44,9 +48,10 @@ static bool photon_grid_ok(const dh_grid &g, const char *what) {
         set_error("%s: invalid grid %d x %d x %d, spacing %g", what, g.nx, g.ny, g.nz, (double)g.spacing);
         return false;
     }
-    // 32-bit byte offsets of the record loads
-    if (32.0 * (double)record_count(g) >= 4294967296.0) {
-        set_error("%s: grid %d x %d x %d too large (32 * (nx+2)(ny+2)(nz+2) must stay below 2^32)", what, g.nx, g.ny,
+    // 32-bit byte offsets of the record loads (vector offset + scalar offset of
+    // the density copy and the slice below 2^32)
+    if (48.0 * (double)record_count(g) >= 4294967296.0) {
+        set_error("%s: grid %d x %d x %d too large (48 * (nx+2)(ny+2)(nz+2) must stay below 2^32)", what, g.nx, g.ny,
                   g.nz);
         return false;
     }
@@ -91,16 +96,16 @@ __device__ __forceinline__ long long vox_index(const PGrid &g, int i, int j, int
     return i + (long long)g.nx * (j + (long long)g.ny * k);
 }
 
-// field 0: TERMA, 1: density
+// field 0: TERMA (float4 2t), 1: density (float4 2t + 1 and its copy 2n + t)
 __global__ __launch_bounds__(256) void k_ph_records_from(const float *__restrict__ src, float4 *__restrict__ rec,
                                                          PGrid g, long long n, int field, float border) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i, j, k;
     padded_coords(g, t, i, j, k);
-    gst(rec, 2 * t + field, padded_quad(g, i, j, k, border, [&](int a, int b, int c) {
-        return gld(src, vox_index(g, a, b, c));
-    }));
+    const float4 q = padded_quad(g, i, j, k, border, [&](int a, int b, int c) { return gld(src, vox_index(g, a, b, c)); });
+    gst(rec, 2 * t + field, q);
+    if (field) gst(rec, 2 * n + t, q);
 }
 
 __global__ __launch_bounds__(256) void k_ph_density_from_hu(const float *__restrict__ hu, float4 *__restrict__ rec,
@@ -109,9 +114,11 @@ __global__ __launch_bounds__(256) void k_ph_density_from_hu(const float *__restr
     if (t >= n) return;
     int i, j, k;
     padded_coords(g, t, i, j, k);
-    gst(rec, 2 * t + 1, padded_quad(g, i, j, k, kAirDensity, [&](int a, int b, int c) {
+    const float4 q = padded_quad(g, i, j, k, kAirDensity, [&](int a, int b, int c) {
         return (float)interp_knots(kn, (double)gld(hu, vox_index(g, a, b, c)));
-    }));
+    });
+    gst(rec, 2 * t + 1, q);
+    gst(rec, 2 * n + t, q);
 }
 
 // ---------------------------------------------------------------------------
@@ -125,18 +132,18 @@ __device__ __forceinline__ float bilerp(float4 q, float ax, float ay) {
     return fmaf(ay, fmaf(ax, q.w, q.z), fmaf(ax, q.y, q.x));
 }
 
-__device__ __forceinline__ unsigned record_offset(const PGrid &g, int ix, int iy, int iz) {
-    return ((unsigned)ix + __umul24((unsigned)iy, (unsigned)g.px) + (unsigned)iz * (unsigned)g.pxy) << 5;
+__device__ __forceinline__ unsigned cell_index(const PGrid &g, int ix, int iy, int iz) {
+    return (unsigned)ix + __umul24((unsigned)iy, (unsigned)g.px) + (unsigned)iz * (unsigned)g.pxy;
 }
 
 // Density at padded index c (every corner inside the padded grid).
 __device__ __forceinline__ float sample_rho(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy, float cz) {
     const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    const unsigned off = record_offset(g, (int)fx, (int)fy, (int)fz);
-    const int so_z = g.pxy << 5;
+    const unsigned off = cell_index(g, (int)fx, (int)fy, (int)fz) << 4;
+    const int so_z = g.pxy << 4;
     const float ax = cx - fx, ay = cy - fy, az = cz - fz;
-    const float r0 = bilerp(load4(rs, off + 16u, 0), ax, ay);
-    const float r1 = bilerp(load4(rs, off + 16u, so_z), ax, ay);
+    const float r0 = bilerp(load4(rs, off, (int)g.rho_base), ax, ay);
+    const float r1 = bilerp(load4(rs, off, (int)(g.rho_base + (unsigned)so_z)), ax, ay);
     return fmaf(az, r1 - r0, r0);
 }
 
@@ -146,7 +153,7 @@ __device__ __forceinline__ void sample_tr(__amdgpu_buffer_rsrc_t rs, const PGrid
                                           float &te, float &rho) {
     const bool in = cx >= 0.0f && cx < g.hx && cy >= 0.0f && cy < g.hy && cz >= 0.0f && cz < g.hz;
     const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    const unsigned off = in ? record_offset(g, (int)fx, (int)fy, (int)fz) : 0u;
+    const unsigned off = in ? cell_index(g, (int)fx, (int)fy, (int)fz) << 5 : 0u;
     const int so_z = g.pxy << 5;
     const float ax = cx - fx, ay = cy - fy, az = cz - fz;
     const float4 t0 = load4(rs, off, 0), d0 = load4(rs, off + 16u, 0);
@@ -157,16 +164,34 @@ __device__ __forceinline__ void sample_tr(__amdgpu_buffer_rsrc_t rs, const PGrid
     rho = in ? fmaf(az, dd1 - dd0, dd0) : kAirDensity;
 }
 
+// The same for a padded index known to lie in [0, n + 1) on every axis.
+__device__ __forceinline__ void sample_tr_inside(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy,
+                                                 float cz, float &te, float &rho) {
+    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    const unsigned off = cell_index(g, (int)fx, (int)fy, (int)fz) << 5;
+    const int so_z = g.pxy << 5;
+    const float ax = cx - fx, ay = cy - fy, az = cz - fz;
+    const float4 t0 = load4(rs, off, 0), d0 = load4(rs, off + 16u, 0);
+    const float4 t1 = load4(rs, off, so_z), d1 = load4(rs, off + 16u, so_z);
+    const float tt0 = bilerp(t0, ax, ay), tt1 = bilerp(t1, ax, ay);
+    const float dd0 = bilerp(d0, ax, ay), dd1 = bilerp(d1, ax, ay);
+    te = fmaf(az, tt1 - tt0, tt0);
+    rho = fmaf(az, dd1 - dd0, dd0);
+}
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t record_rsrc(const float4 *rec, const PGrid &g) {
-    const unsigned bytes = (unsigned)g.pxy * (unsigned)(g.nz + 2) * 32u;
+    const unsigned bytes = g.rho_base + g.rho_base / 2u;
     return __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)0, (int)bytes, 0x00020000);
 }
 
-// Brick mapping: workgroup q takes bricks 4q .. 4q + 3 (remapped per XCD), wave
-// w brick 4q + w, lane = x + 4 y + 16 z of the brick.
+// Brick mapping: workgroup q takes bricks 4q .. 4q + 3, wave w brick 4q + w,
+// lane = x + 4 y + 16 z of the brick.  Workgroups go to the XCDs in chunks of 16
+// in turn: the bricks a control point irradiates form one compact region of the
+// grid, which contiguous eighths of the brick range would hand to one or two
+// XCDs (5.98 -> 3.10 ms per control point of the head plan's cone sum).
 __device__ __forceinline__ bool brick_voxel(const BrickGrid &bg, const PGrid &g, unsigned nblocks, int &i, int &j,
                                             int &k) {
-    const unsigned q = xcd_remap(blockIdx.x, nblocks);
+    const unsigned q = xcd_chunk<16>(blockIdx.x, nblocks);
     const long long brick = 4LL * q + (threadIdx.x >> 6);
     if (brick >= bg.count()) return false;
     int bi, bj, bk;
@@ -303,6 +328,7 @@ struct PhotonCccArgs {
     float am_l2[N_CONES];        // am log2(e)
     float am[N_CONES], bm[N_CONES];
     float len0[N_CONES];         // initial ray length (cm, float32 counter)
+    int nsteps[N_CONES];         // values of the counter len0, len0 - sp, ... that are >= 0
     float ax[3];                 // unit vector from the isocentre to the source
     float sp;                    // spacing / 10 (cm)
     float mu_cal, mu, scale;
@@ -350,13 +376,33 @@ __global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, 
             const float ddy = (r10 * t0[0] + r11 * t0[1] + r12 * t0[2]) * sc10;
             const float ddz = (r20 * t0[0] + r21 * t0[1] + r22 * t0[2]) * sc10;
             float rp = 0.0f, rsum = 0.0f;
-            for (float rl = a.len0[cone]; rl >= 0.0f; rl = rl - sp) {
-                float te, rho;
-                sample_tr(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), te, rho);
+            const float len0 = a.len0[cone];
+            const int nsteps = a.nsteps[cone];
+            // samples at ray lengths len0, len0 - sp, ... (the float32 counter of the
+            // reference, nsteps values >= 0); the positions are monotone in the length,
+            // so when the farthest sample is inside the padded grid every sample is
+            auto step = [&](float te, float rho) {
                 const float di = fmaxf(rho, kAirDensity) * sp;
                 const float e = __builtin_amdgcn_exp2f(-am_l2 * di);
                 rp = fmaf(rp, e, te * k1 * (1.0f - e));
                 rsum = fmaf(rsum, fmaf(-bm, di, 1.0f), te * di * k2);
+            };
+            const float fx = fmaf(ddx, len0, c0x), fy = fmaf(ddy, len0, c0y), fz = fmaf(ddz, len0, c0z);
+            const bool far_in = fx >= 0.0f && fx < g.hx && fy >= 0.0f && fy < g.hy && fz >= 0.0f && fz < g.hz;
+            float rl = len0;
+            if (__all(far_in)) {
+#pragma unroll 2
+                for (int t = 0; t < nsteps; ++t, rl = rl - sp) {
+                    float te, rho;
+                    sample_tr_inside(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), te, rho);
+                    step(te, rho);
+                }
+            } else {
+                for (int t = 0; t < nsteps; ++t, rl = rl - sp) {
+                    float te, rho;
+                    sample_tr(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), te, rho);
+                    step(te, rho);
+                }
             }
             cone_sum_p += rp;
             cone_sum_s += rsum;
@@ -376,7 +422,7 @@ using namespace dh;
 
 extern "C" int64_t dh_photon_record_len(dh_grid g) {
     if (!photon_grid_ok(g, "dh_photon_record_len")) return -1;
-    return 8 * record_count(g);
+    return 12 * record_count(g);
 }
 
 static int records_ok(const void *rec, const char *what) {
@@ -508,7 +554,7 @@ extern "C" int dh_photon_terma(const float *depth, float *terma, float *electron
 // (sin th cos ph, sin th sin ph, cos th), ph = j pi / 6, taken to the image
 // frame by the inverse of the photon head transform (IMRTClasses.cu:81-118,
 // 286-309), in float32 as the reference evaluates them.
-static void cone_directions(const dh_photon_model &m, const dh_photon_beam &b, PhotonCccArgs &a) {
+static void cone_directions(const dh_photon_model &m, const dh_photon_beam &b, float sp, PhotonCccArgs &a) {
     const float pi = 3.14159265358979f;
     const dh_beam_geom &gm = b.geom;
     for (int c = 0; c < N_CONES; ++c) {
@@ -521,6 +567,9 @@ static void cone_directions(const dh_photon_model &m, const dh_photon_beam &b, P
         a.bm[c] = bm;
         a.am_l2[c] = am * 1.44269504088896341f;
         a.len0[c] = m.kernel[c][5];
+        int n = 0;
+        for (float rl = m.kernel[c][5]; rl >= 0.0f; rl = rl - sp) ++n;  // float32 as the reference's counter
+        a.nsteps[c] = n;
         for (int j = 0; j < N_AZ; ++j) {
             const float phi = (float)j * pi / 6.0f;
             const float hx = st * cosf(phi), hy = st * sinf(phi), hz = ct;
@@ -561,7 +610,7 @@ extern "C" int dh_photon_ccc(const float *records, const float *terma, const flo
         }
     }
     PhotonCccArgs a;
-    cone_directions(*model, *beam, a);
+    cone_directions(*model, *beam, sp, a);
     a.sp = sp;
     a.mu_cal = model->mu_cal;
     a.mu = beam->mu;

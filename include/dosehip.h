@@ -385,18 +385,20 @@ int dh_batch_adjoint(dh_batch *batch, int32_t flags, void *stream);
  *                                dh_photon_ccc
  *
  * Density and TERMA are read through a "record" volume in place of the
- * reference's two border-addressed 3-D textures (TextureClasses.cu:14-41):
- * one record per cell of the grid padded by one voxel on every side,
- * (nx+2)*(ny+2)*(nz+2) records, index i' + (nx+2)*(j' + (ny+2)*k'), each
- * 8 floats: the bilinear form (a, b-a, c-a, (d-c)-(b-a)) of TERMA on the
- * slice-k' face of the cell, then the same of density, where a = f(i',j',k'),
- * b = f(i'+1,j',k'), c = f(i',j'+1,k'), d = f(i'+1,j'+1,k') and f(i',j',k') is
- * the voxel value at (i'-1, j'-1, k'-1) inside the grid and the texture's
- * border value outside it (TERMA 0, density AIR_DENSITY = 0.0012,
- * IMRTClasses.cuh:6).  A sample at continuous voxel index c (texture
- * coordinate c + 1/2) is then two records' blend at c + 1, with the border
- * values wherever the texture would read only border texels.
- * dh_photon_record_len gives the floats of a record volume (32-byte aligned).
+ * reference's two border-addressed 3-D textures (TextureClasses.cu:14-41),
+ * over the grid padded by one voxel on every side: P = (nx+2)*(ny+2)*(nz+2)
+ * cells, index p = i' + (nx+2)*(j' + (ny+2)*k').  Floats 8p .. 8p+7 hold the
+ * bilinear form of TERMA on the slice-k' face of cell p, then that of density;
+ * floats 8P + 4p .. 8P + 4p + 3 hold the density form again (the ray-trace
+ * reads only these).  The form of field f is (a, b-a, c-a, (d-c)-(b-a)) with
+ * a = f(i',j',k'), b = f(i'+1,j',k'), c = f(i',j'+1,k'), d = f(i'+1,j'+1,k'),
+ * where f(i',j',k') is the voxel value at (i'-1, j'-1, k'-1) inside the grid
+ * and the texture's border value outside it (TERMA 0, density AIR_DENSITY =
+ * 0.0012, IMRTClasses.cuh:6).  A sample at continuous voxel index c (texture
+ * coordinate c + 1/2) is then the blend of two cells' forms at c + 1, with
+ * the border values wherever the texture would read only border texels.
+ * dh_photon_record_len gives the floats of a record volume (12 P; 32-byte
+ * aligned, below 2^32 bytes).
  * ---------------------------------------------------------------------- */
 
 /* MLC leaf pair (MLCPair, IMRTClasses.cuh:8-15), mm at the isocentre plane. */

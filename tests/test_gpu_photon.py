@@ -108,16 +108,20 @@ def test_density_records_from_hu(dev):
     with torch.cuda.device(dev):
         ws = PH.PhotonWorkspace(hu.shape, 2.0, dev)
         ws.set_density_from_hu(torch.from_numpy(hu).to(dev), tab[:, 0], tab[:, 1])
-        rec = ws.records.view(7, 8, 9, 8).cpu().numpy()   # (k', j', i', 8 floats)
+        allr = ws.records.cpu().numpy()
         torch.cuda.synchronize()
-    a = rec[..., 4]
+    npad = 7 * 8 * 9
+    rec = allr[8 * npad:].reshape(7, 8, 9, 4)            # density-only copy (k', j', i', 4 floats)
+    pairs = allr[:8 * npad].reshape(7, 8, 9, 8)         # (TERMA form, density form) per cell
+    np.testing.assert_array_equal(pairs[..., 4:], rec)
+    a = rec[..., 0]
     inner = a[1:-1, 1:-1, 1:-1]
     np.testing.assert_array_equal(inner, want)
     border = np.ones_like(a, bool)
     border[1:-1, 1:-1, 1:-1] = False
     assert np.all(a[border] == np.float32(0.0012))
     # the face form: a + (b - a) is the +x neighbour b up to the rounding of the float32 difference
-    np.testing.assert_allclose(rec[1:-1, 1:-1, 1:-2, 4] + rec[1:-1, 1:-1, 1:-2, 5], want[:, :, 1:], rtol=0,
+    np.testing.assert_allclose(rec[1:-1, 1:-1, 1:-2, 0] + rec[1:-1, 1:-1, 1:-2, 1], want[:, :, 1:], rtol=0,
                                atol=4e-7)
 
 
