@@ -132,8 +132,9 @@ __device__ __forceinline__ float bilerp(float4 q, float ax, float ay) {
     return fmaf(ay, fmaf(ax, q.w, q.z), fmaf(ax, q.y, q.x));
 }
 
+// (the padded slice length and index stay below 2^24: 24-bit products, low 32 bits)
 __device__ __forceinline__ unsigned cell_index(const PGrid &g, int ix, int iy, int iz) {
-    return (unsigned)ix + __umul24((unsigned)iy, (unsigned)g.px) + (unsigned)iz * (unsigned)g.pxy;
+    return (unsigned)ix + __umul24((unsigned)iy, (unsigned)g.px) + __umul24((unsigned)iz, (unsigned)g.pxy);
 }
 
 // Density at padded index c (every corner inside the padded grid).
@@ -167,10 +168,10 @@ __device__ __forceinline__ void sample_tr(__amdgpu_buffer_rsrc_t rs, const PGrid
 // The same for a padded index known to lie in [0, n + 1) on every axis.
 __device__ __forceinline__ void sample_tr_inside(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy,
                                                  float cz, float &te, float &rho) {
-    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    const unsigned off = cell_index(g, (int)fx, (int)fy, (int)fz) << 5;
+    // c >= 0: truncation is floor
+    const unsigned off = cell_index(g, (int)cx, (int)cy, (int)cz) << 5;
     const int so_z = g.pxy << 5;
-    const float ax = cx - fx, ay = cy - fy, az = cz - fz;
+    const float ax = __builtin_amdgcn_fractf(cx), ay = __builtin_amdgcn_fractf(cy), az = __builtin_amdgcn_fractf(cz);
     const float4 t0 = load4(rs, off, 0), d0 = load4(rs, off + 16u, 0);
     const float4 t1 = load4(rs, off, so_z), d1 = load4(rs, off + 16u, so_z);
     const float tt0 = bilerp(t0, ax, ay), tt1 = bilerp(t1, ax, ay);
@@ -366,7 +367,8 @@ __global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, 
     const float sp = a.sp;
     float sum = 0.0f;
     for (int cone = 0; cone < N_CONES; ++cone) {
-        const float k1 = a.k1[cone], k2 = a.k2[cone], am_l2 = a.am_l2[cone], bm = a.bm[cone];
+        const float k1 = a.k1[cone], bm = a.bm[cone];
+        const float am_l2sp = -a.am_l2[cone] * a.sp, k2_sp = a.k2[cone] * a.sp, bm_sp = -bm * a.sp;
         float cone_sum_p = 0.0f, cone_sum_s = 0.0f;
         for (int az = 0; az < N_AZ; ++az) {
             const float *t0 = a.t[cone * N_AZ + az];
@@ -381,11 +383,17 @@ __global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, 
             // samples at ray lengths len0, len0 - sp, ... (the float32 counter of the
             // reference, nsteps values >= 0); the positions are monotone in the length,
             // so when the farthest sample is inside the padded grid every sample is
+            // with D = max(rho, AIR) sp the reference's updates
+            //   R_p <- R_p e + T k1 (1 - e),  e = exp(-am D)
+            //   R_s <- R_s (1 - bm D) + T D k2
+            // are evaluated as R_p <- T k1 + e (R_p - T k1) and
+            // R_s <- R_s + rho' (T k2 sp - bm sp R_s), rho' = max(rho, AIR)
             auto step = [&](float te, float rho) {
-                const float di = fmaxf(rho, kAirDensity) * sp;
-                const float e = __builtin_amdgcn_exp2f(-am_l2 * di);
-                rp = fmaf(rp, e, te * k1 * (1.0f - e));
-                rsum = fmaf(rsum, fmaf(-bm, di, 1.0f), te * di * k2);
+                const float rc = fmaxf(rho, kAirDensity);
+                const float e = __builtin_amdgcn_exp2f(rc * am_l2sp);
+                const float tk = te * k1;
+                rp = fmaf(e, rp - tk, tk);
+                rsum = fmaf(rc, fmaf(bm_sp, rsum, te * k2_sp), rsum);
             };
             const float fx = fmaf(ddx, len0, c0x), fy = fmaf(ddy, len0, c0y), fz = fmaf(ddz, len0, c0z);
             const bool far_in = fx >= 0.0f && fx < g.hx && fy >= 0.0f && fy < g.hy && fz >= 0.0f && fz < g.hz;
