@@ -41,6 +41,7 @@ EXPLICIT_VR_LE = "1.2.840.10008.1.2.1"
 DEFLATED_EXPLICIT_VR_LE = "1.2.840.10008.1.2.1.99"
 EXPLICIT_VR_BE = "1.2.840.10008.1.2.2"
 RT_ION_PLAN_STORAGE = "1.2.840.10008.5.1.4.1.1.481.8"
+RT_PLAN_STORAGE = "1.2.840.10008.5.1.4.1.1.481.5"
 RT_STRUCTURE_SET_STORAGE = "1.2.840.10008.5.1.4.1.1.481.3"
 CT_IMAGE_STORAGE = "1.2.840.10008.5.1.4.1.1.2"
 # UUID-derived root (PS3.5 B.2): "2.25." + the decimal value of a UUID
@@ -61,9 +62,9 @@ _TEXT_SINGLE = frozenset(("LT", "ST", "UT", "UR"))
 _BINARY = {"US": "H", "SS": "h", "UL": "I", "SL": "i", "FL": "f", "FD": "d", "UV": "Q", "SV": "q"}
 _KNOWN_VRS = _LONG_VRS | _TEXT_MULTI | _TEXT_SINGLE | frozenset(_BINARY) | frozenset(("AT",))
 
-# (group, element, VR, keyword) of the attributes used by the plan reader /
-# writer, the structure-set lookup, the CT series reader and the RT Dose
-# reader / writer.
+# (group, element, VR, keyword) of the attributes used by the plan readers
+# (RT Ion Plan and RT Plan) / writer, the structure-set lookup, the CT series
+# reader and the RT Dose reader / writer.
 _DICTIONARY = """
 0002 0000 UL FileMetaInformationGroupLength
 0002 0001 OB FileMetaInformationVersion
@@ -140,8 +141,12 @@ _DICTIONARY = """
 300A 0080 IS NumberOfBeams
 300A 0086 DS BeamMeterset
 300A 00A0 IS NumberOfBrachyApplicationSetups
+300A 00B0 SQ BeamSequence
 300A 00B2 SH TreatmentMachineName
 300A 00B3 CS PrimaryDosimeterUnit
+300A 00B6 SQ BeamLimitingDeviceSequence
+300A 00B8 CS RTBeamLimitingDeviceType
+300A 00BC IS NumberOfLeafJawPairs
 300A 00C0 IS BeamNumber
 300A 00C2 LO BeamName
 300A 00C3 ST BeamDescription
@@ -156,8 +161,11 @@ _DICTIONARY = """
 300A 010C DS CumulativeDoseReferenceCoefficient
 300A 010E DS FinalCumulativeMetersetWeight
 300A 0110 IS NumberOfControlPoints
+300A 0111 SQ ControlPointSequence
 300A 0112 IS ControlPointIndex
 300A 0114 DS NominalBeamEnergy
+300A 011A SQ BeamLimitingDevicePositionSequence
+300A 011C DS LeafJawPositions
 300A 011E DS GantryAngle
 300A 011F CS GantryRotationDirection
 300A 0120 DS BeamLimitingDeviceAngle
