@@ -137,17 +137,6 @@ __device__ __forceinline__ unsigned cell_index(const PGrid &g, int ix, int iy, i
     return (unsigned)ix + __umul24((unsigned)iy, (unsigned)g.px) + __umul24((unsigned)iz, (unsigned)g.pxy);
 }
 
-// Density at padded index c (every corner inside the padded grid).
-__device__ __forceinline__ float sample_rho(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy, float cz) {
-    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    const unsigned off = cell_index(g, (int)fx, (int)fy, (int)fz) << 4;
-    const int so_z = g.pxy << 4;
-    const float ax = cx - fx, ay = cy - fy, az = cz - fz;
-    const float r0 = bilerp(load4(rs, off, (int)g.rho_base), ax, ay);
-    const float r1 = bilerp(load4(rs, off, (int)(g.rho_base + (unsigned)so_z)), ax, ay);
-    return fmaf(az, r1 - r0, r0);
-}
-
 // TERMA and density at padded index c, border values where the texture would
 // read only border texels (some axis outside [0, n + 1)).
 __device__ __forceinline__ void sample_tr(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy, float cz,
@@ -224,13 +213,28 @@ __global__ __launch_bounds__(256) void k_ph_raytrace(const float4 *__restrict__ 
     const __amdgpu_buffer_rsrc_t rs = record_rsrc(rec, g);
     const float nxf = (float)g.nx, nyf = (float)g.ny, nzf = (float)g.nz;
     float acc = -0.05f;
+    // the two density forms of the cell of the lane's last sample: at 1 mm steps
+    // a sample often lies in the cell of the one before and reads no memory
+    unsigned off_prev = 0xffffffffu;
+    float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q1 = q0;
+    const int so_z = g.pxy << 4;
     for (int n = 0;; ++n) {
         const float fn = (float)n;
         const float tx = fmaf(g.inv_s, fmaf(ux, fn, v.x) + b.iso[0], 0.5f);
         const float ty = fmaf(g.inv_s, fmaf(uy, fn, v.y) + b.iso[1], 0.5f);
         const float tz = fmaf(g.inv_s, fmaf(uz, fn, v.z) + b.iso[2], 0.5f);
         if (!(tx >= 0.0f && tx < nxf && ty >= 0.0f && ty < nyf && tz >= 0.0f && tz < nzf)) break;
-        const float rho = sample_rho(rs, g, tx + 0.5f, ty + 0.5f, tz + 0.5f);
+        // padded index tex + 1/2 >= 1/2: truncation is floor
+        const float cx = tx + 0.5f, cy = ty + 0.5f, cz = tz + 0.5f;
+        const unsigned off = cell_index(g, (int)cx, (int)cy, (int)cz) << 4;
+        if (off != off_prev) {
+            q0 = load4(rs, off, (int)g.rho_base);
+            q1 = load4(rs, off, (int)(g.rho_base + (unsigned)so_z));
+            off_prev = off;
+        }
+        const float ax = __builtin_amdgcn_fractf(cx), ay = __builtin_amdgcn_fractf(cy);
+        const float r0 = bilerp(q0, ax, ay), r1 = bilerp(q1, ax, ay);
+        const float rho = fmaf(__builtin_amdgcn_fractf(cz), r1 - r0, r0);
         acc = fmaf(fmaxf(rho, 0.0f), 0.1f, acc);
     }
     gst(depth, vox_index(g, i, j, k), acc);
