@@ -174,18 +174,27 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t record_rsrc(const float4 *rec,
     return __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)0, (int)bytes, 0x00020000);
 }
 
-// Brick mapping: workgroup q takes bricks 4q .. 4q + 3, wave w brick 4q + w,
-// lane = x + 4 y + 16 z of the brick.  Workgroups go to the XCDs in chunks of 16
-// in turn: the bricks a control point irradiates form one compact region of the
-// grid, which contiguous eighths of the brick range would hand to one or two
-// XCDs (5.98 -> 3.10 ms per control point of the head plan's cone sum).
+// Brick mapping.  Bricks are numbered tile by tile, 4x4x4 bricks (16^3 voxels)
+// per tile, x fastest inside a tile and across tiles; workgroup q takes virtual
+// bricks 4q .. 4q + 3 (wave w the brick 4q + w, lane = x + 4 y + 16 z of the
+// brick; virtual bricks past the grid's edge leave at once).  Workgroups go to
+// the XCDs in chunks of 64, four tiles, in turn: the bricks a control point
+// irradiates form one compact region of the grid, which contiguous eighths of
+// the brick range would hand to one or two XCDs, while tile-sized chunks keep
+// the rays of an XCD's concurrent waves in few cache lines of its L2.  Cone sum
+// of the head plan per control point: 5.98 ms with contiguous eighths, 2.70 with
+// chunks of 16 workgroups in raster brick order, 2.28 with tiles and chunks of 64.
 __device__ __forceinline__ bool brick_voxel(const BrickGrid &bg, const PGrid &g, unsigned nblocks, int &i, int &j,
                                             int &k) {
-    const unsigned q = xcd_chunk<16>(blockIdx.x, nblocks);
-    const long long brick = 4LL * q + (threadIdx.x >> 6);
-    if (brick >= bg.count()) return false;
-    int bi, bj, bk;
-    brick_coords(bg, brick, bi, bj, bk);
+    const unsigned q = xcd_chunk<64>(blockIdx.x, nblocks);
+    const unsigned vb = 4u * q + (threadIdx.x >> 6);
+    const unsigned tile = vb >> 6, w = vb & 63u;
+    const unsigned tx = (unsigned)(bg.bx + 3) >> 2, ty = (unsigned)(bg.by + 3) >> 2;
+    const unsigned t1 = tile / tx;
+    const int bi = (int)((tile - t1 * tx) * 4u + (w & 3u));
+    const int bj = (int)((t1 % ty) * 4u + ((w >> 2) & 3u));
+    const int bk = (int)((t1 / ty) * 4u + (w >> 4));
+    if (bi >= bg.bx || bj >= bg.by || bk >= bg.bz) return false;
     const int lane = threadIdx.x & 63;
     i = bi * BRICK + (lane & 3);
     j = bj * BRICK + ((lane >> 2) & 3);
@@ -426,7 +435,11 @@ __global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, 
     gst(dose, v, a.accumulate ? fmaf(a.scale, d, gld(dose, v)) : a.scale * d);
 }
 
-static unsigned brick_blocks(const BrickGrid &bg) { return (unsigned)((bg.count() + 3) / 4); }
+// Workgroups of a brick launch: 16 per tile of 4x4x4 bricks.
+static unsigned brick_blocks(const BrickGrid &bg) {
+    const long long tiles = (long long)((bg.bx + 3) / 4) * ((bg.by + 3) / 4) * ((bg.bz + 3) / 4);
+    return (unsigned)(tiles * 16);
+}
 
 }  // namespace dh
 
