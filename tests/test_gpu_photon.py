@@ -125,15 +125,23 @@ def test_density_records_from_hu(dev):
                                atol=4e-7)
 
 
-@pytest.mark.parametrize("ga,ta,ca,spacing", [(0.0, 0.0, 0.0, 2.0), (90.0, 0.0, 30.0, 2.5), (217.0, 35.0, 80.0, 3.0)])
-def test_stages_vs_oracle(dev, varian, ga, ta, ca, spacing):
+@pytest.mark.parametrize("machine,energy,ga,ta,ca,spacing", [
+    ("VarianTrueBeamHF", 0, 0.0, 0.0, 0.0, 2.0), ("VarianTrueBeamHF", 1, 90.0, 0.0, 30.0, 2.5),
+    ("VarianTrueBeamHF", 0, 217.0, 35.0, 80.0, 3.0), ("ElektaVersaSibAxesse", 1, 300.0, 10.0, 355.0, 2.0),
+    ("ElektaVersaSibAxesse", 3, 45.0, 0.0, 0.0, 1.5)])
+def test_stages_vs_oracle(dev, machine, energy, ga, ta, ca, spacing):
     """Depth vs the oracle; TERMA / electron vs the oracle on the GPU's depth;
-    dose vs the oracle on the GPU's TERMA, density and electron."""
-    _say("stages ga=%g ta=%g ca=%g s=%g" % (ga, ta, ca, spacing))
+    dose vs the oracle on the GPU's TERMA, density and electron.  Both
+    machines (60 leaf pairs of 5 / 10 mm; 80 of 5 mm with the MLC 643 mm from
+    the isocentre), 6 MV, 10 MV, 6 MV FFF and 15 MV."""
+    from DoseCUDA import IMRTPlan
+    _say("stages %s energy %d ga=%g ta=%g ca=%g s=%g" % (machine, energy, ga, ta, ca, spacing))
+    plan = IMRTPlan(machine)
     g = _hetero_grid((26, 30, 28), spacing, seed=int(ga))
-    model = varian.beam_models[0]
-    cp = _cp(varian, 0, ga, ta, ca, 20.0, 25.0, mu=80.0, irregular=True)
-    rho, depth, terma, el, dose = _stages(g, model, cp, dev)
+    model = plan.beam_models[energy]
+    cp = _cp(plan, energy, ga, ta, ca, 20.0, 25.0, mu=80.0, irregular=True)
+    rho = OR.density_from_hu(g.HU, os.path.join(PHOT, machine))
+    rho, depth, terma, el, dose = _stages(g, model, cp, dev, density=rho)
     geom = _geom(g, model, cp)
     s = float(np.float32(g.spacing[0]))
     d_ref = OR.raytrace(rho, s, geom)
