@@ -304,3 +304,48 @@ def test_dropin_argument_errors(dev, varian):
         dose_kernels.photon_dose_cuda(varian.beam_models[0], vol, bad, 0)
     out = dose_kernels.photon_dose_cuda(varian.beam_models[0], vol, cp, 0)
     assert out.shape == (4, 4, 4) and out.dtype == np.float32
+
+
+def test_body_grid_sampled(dev, varian):
+    """The body-size grid (300x512x512 @ 1 mm: lungs, spine; 3.8 GB of
+    records), one 10 x 12 cm 10 MV field at gantry 90, collimator 30, through
+    PhotonDoseEngine; 2000 sampled voxels (half above 10% of D_max): depth and
+    TERMA vs the oracle at them, the dose vs the oracle's cone sum on the
+    GPU's TERMA volume."""
+    from DoseCUDA import IMRTDoseGrid, IMRTPlan
+    from DoseCUDA import photon as PH
+    from DoseCUDA.workloads import body_plan
+    _say("body grid 300x512x512 @ 1 mm")
+    bg, _, _ = body_plan()
+    g = IMRTDoseGrid()
+    g.spacing, g.size, g.origin, g.HU = bg.spacing, bg.size, bg.origin, bg.HU
+    del bg
+    plan = IMRTPlan("VarianTrueBeamHF")
+    plan.addSquareField('10', 10.0, 12.0, mu=100, gantry_angle=90.0, collimator_angle=30.0)
+    eng = PH.PhotonDoseEngine(g, plan, gpu_id=0)
+    dose = eng.compute()[0].cpu().numpy().ravel()
+    dmodel, cps = eng.beams[0]
+    cpc, of, _ = cps[0]
+    depth = eng.ws.depth.cpu().numpy()
+    terma = eng.ws.terma.cpu().numpy()
+    el = eng.ws.electron.cpu().numpy().ravel()
+    rng = np.random.default_rng(5)
+    hi = np.flatnonzero(dose > 0.1 * dose.max())
+    idx = np.unique(np.concatenate([rng.choice(hi, 1000, replace=False), rng.choice(dose.size, 1000, replace=False),
+                                    [int(dose.argmax())]]))
+    model = plan.beam_models[list(np.asarray(plan.dicom_energy_label).astype(str)).index('10')]
+    cp = plan.beam_list[0].cp_list[0]
+    geom = _geom(g, model, cp)
+    rho = OR.density_from_hu(g.HU, os.path.join(PHOT, "VarianTrueBeamHF"))
+    d_ref = OR.raytrace(rho, 1.0, geom, idx=idx)
+    assert np.max(np.abs(depth.ravel()[idx] - d_ref)) < 2e-4, np.max(np.abs(depth.ravel()[idx] - d_ref))
+    t_ref, e_ref = OR.terma(depth.ravel()[idx].astype(np.float64), rho.shape, 1.0, geom, model, OR.mlc_rows(cp.mlc),
+                            idx=idx)
+    assert np.max(np.abs(terma.ravel()[idx] - t_ref)) < 1e-5 * terma.max()
+    ref = OR.ccc(terma, rho, el[idx].astype(np.float64), 1.0, geom, model.kernel, np.float32(model.mu_calibration),
+                 np.float32(cp.mu), idx=idx) * of
+    err = np.max(np.abs(dose[idx] - ref)) / dose.max()
+    _say("  %d voxels: depth %.2e, TERMA %.2e, dose %.2e of D_max" % (
+        len(idx), np.max(np.abs(depth.ravel()[idx] - d_ref)), np.max(np.abs(terma.ravel()[idx] - t_ref)) / terma.max(),
+        err))
+    assert err < 1e-4, err
