@@ -536,6 +536,10 @@ static bool model_ok(const dh_photon_model *m, const char *what) {
         set_error("%s: bad model (null tables or empty profile)", what);
         return false;
     }
+    if (!(m->primary_src_dist > 0.0f) || !(m->primary_src_size > 0.0f) || !(m->scatter_src_size > 0.0f)) {
+        set_error("%s: source distance and sizes must be positive", what);
+        return false;
+    }
     return true;
 }
 
@@ -621,16 +625,19 @@ extern "C" int dh_photon_ccc(const float *records, const float *terma, const flo
                              void *stream) {
     if (!photon_grid_ok(g, "dh_photon_ccc")) return DH_ERR_ARG;
     if (!records_ok(records, "dh_photon_ccc") || !beam_ok(beam, "dh_photon_ccc")) return DH_ERR_ARG;
-    if (!model || !terma || !electron || !dose) {
-        set_error("dh_photon_ccc: null model or volume");
+    if (!terma || !electron || !dose) {
+        set_error("dh_photon_ccc: null volume");
         return DH_ERR_ARG;
     }
+    if (!model_ok(model, "dh_photon_ccc")) return DH_ERR_ARG;
     const float sp = g.spacing / 10.0f;
     for (int c = 0; c < N_CONES; ++c) {
         // the ray-length counter must reach below zero in a bounded number of steps
-        if (!(model->kernel[c][5] >= 0.0f) || !(model->kernel[c][5] / sp < 4096.0f)) {
-            set_error("dh_photon_ccc: cone %d ray length %g cm is negative or beyond 4096 steps of %g cm", c,
-                      (double)model->kernel[c][5], (double)sp);
+        // a negative length takes no sample, as the reference's loop; NaN or more than 4096 steps is refused
+        const float L = model->kernel[c][5];
+        if (L != L || L / sp >= 4096.0f) {
+            set_error("dh_photon_ccc: cone %d ray length %g cm is not a number or beyond 4096 steps of %g cm", c,
+                      (double)L, (double)sp);
             return DH_ERR_ARG;
         }
     }
