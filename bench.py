@@ -444,17 +444,25 @@ def run_photon(args):
             PH.terma(eng.ws, dmodel, cpc)
             in_field.append(int((eng.ws.terma > 0.01).sum().item()))
     samples = per_voxel * sum(in_field)
-    for _ in range(max(1, args.warmup)):
-        eng.compute()
-    torch.cuda.synchronize()
     steps = max(1, args.steps)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        eng.compute()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) / steps * 1e3
-    log("photon plan: %.2f ms per plan (%d control points)" % (ms, n_cp))
-    # per-kernel HIP events on the launch stream (torch's current stream), one extra plan
+
+    def plan_ms(pipeline):
+        for _ in range(max(1, args.warmup)):
+            eng.compute(pipeline)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.compute(pipeline)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    # one stream (every kernel in order), then the timed form: depth and TERMA of the next
+    # control point on a side stream beside the current cone sum
+    ms_one = plan_ms(False)
+    ms = plan_ms(True)
+    log("photon plan: %.2f ms per plan (%d control points; %.2f ms on one stream)" % (ms, n_cp, ms_one))
+    # per-kernel HIP events on the launch stream (torch's current stream), one extra plan on one stream
+    # (each kernel alone on the GPU)
     st = torch.cuda.current_stream(dev)
     names = ("k_ph_raytrace", "k_ph_terma+k_ph_records_from", "k_ph_ccc")
     acc = {k: 0.0 for k in names}
@@ -517,8 +525,10 @@ def run_photon(args):
         "data": "synthetic",
         "config": {"workload": "head phantom 160x200x200 @ 2 mm on an IMRTDoseGrid, VarianTrueBeamHF 6 MV, "
                                "7 beams (gantry 0..308.6 deg) x 5 step-and-shoot MLC segments = %d control points; "
-                               "step = the plan's dose on the device (density from HU once, at set-up)" % n_cp,
+                               "step = the plan's dose on the device (density from HU once, at set-up; depth "
+                               "and TERMA of the next control point on a side stream beside the cone sum)" % n_cp,
                    "voxels": nvox, "setup_s": t_setup},
+        "one_stream_ms_per_plan": ms_one,
         "in_field_voxels_per_cp": float(np.mean(in_field)),
         "samples_per_in_field_voxel": per_voxel,
         "cone_samples_per_s": samples / (ms * 1e-3),

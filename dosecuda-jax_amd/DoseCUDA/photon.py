@@ -12,8 +12,9 @@
   every control point's dose times its output factor accumulated per beam
   (computeIMRTPlan, plan_imrt.py:126-158)
 
-Everything stays in device memory and is enqueued on torch's current stream;
-nothing here synchronises.
+Everything stays in device memory and is enqueued on torch's current stream
+(the engine's pipelined plan also on a side stream, ordered with the current
+stream by events); nothing here synchronises.
 """
 
 import numpy as np
@@ -95,16 +96,17 @@ class control_point:
 
 
 class PhotonWorkspace:
-    """Record volume and per-control-point volumes of one grid."""
+    """Record volume and per-control-point volumes of one grid (``depth``: a
+    depth volume to share with another workspace of the same grid)."""
 
-    def __init__(self, shape_zyx, spacing, device):
+    def __init__(self, shape_zyx, spacing, device, depth=None):
         self.grid = make_grid(shape_zyx, spacing)
         n = N.load().dh_photon_record_len(self.grid)
         N.check(0 if n > 0 else N.DH_ERR_ARG, "dh_photon_record_len")
         self.device = device
         # 32-byte aligned: torch's caching allocator hands out 512-byte aligned blocks
         self.records = torch.empty(int(n), dtype=torch.float32, device=device)
-        self.depth = torch.empty(tuple(shape_zyx), dtype=torch.float32, device=device)
+        self.depth = torch.empty(tuple(shape_zyx), dtype=torch.float32, device=device) if depth is None else depth
         self.terma = torch.empty(tuple(shape_zyx), dtype=torch.float32, device=device)
         self.electron = torch.empty(tuple(shape_zyx), dtype=torch.float32, device=device)
         self.shape = tuple(shape_zyx)
@@ -159,9 +161,18 @@ class PhotonDoseEngine:
     """A photon plan on one device: density from HU on the device (np.interp
     of HU_Density.csv in float64, rounded to float32), then per beam the sum
     over its control points of output factor x control-point dose, and the
-    plan dose = sum of beams x n_fractions (computeIMRTPlan, plan_imrt.py:126-158)."""
+    plan dose = sum of beams x n_fractions (computeIMRTPlan, plan_imrt.py:126-158).
 
-    def __init__(self, dose_grid, plan, gpu_id=0):
+    pipeline=True keeps two workspaces (two record volumes, one depth volume)
+    and computes the depth and TERMA of control point i + 1 on a side stream
+    while the cone sum of control point i runs on the caller's stream: the
+    TERMA kernel (VALU-bound) and the ray-trace fill issue slots the cone sum
+    (bound by its gathers) leaves idle.  The cone sums still run one after the
+    other in plan order on the caller's stream, so the doses are the same bit
+    for bit as with pipeline=False (one workspace, everything on the caller's
+    stream), which costs one record volume less (12 floats per padded cell)."""
+
+    def __init__(self, dose_grid, plan, gpu_id=0, pipeline=True):
         from .plan_imrt import _photon_dir
         import pandas as pd
         if not (dose_grid.spacing[0] == dose_grid.spacing[1] == dose_grid.spacing[2]):
@@ -175,7 +186,12 @@ class PhotonDoseEngine:
         with torch.cuda.device(self.device):
             self.hu = torch.from_numpy(_f32(dose_grid.HU)).to(self.device)
             self.ws = PhotonWorkspace(self.shape, spacing, self.device)
-            self.ws.set_density_from_hu(self.hu, df["HU"].to_numpy(), df["Density"].to_numpy())
+            self.wss = [self.ws]
+            if pipeline:
+                self.wss.append(PhotonWorkspace(self.shape, spacing, self.device, depth=self.ws.depth))
+                self.side = torch.cuda.Stream(self.device)
+            for ws in self.wss:
+                ws.set_density_from_hu(self.hu, df["HU"].to_numpy(), df["Density"].to_numpy())
             labels = list(np.asarray(plan.dicom_energy_label).astype(str))
             self.models = {}
             self.beams = []
@@ -191,25 +207,53 @@ class PhotonDoseEngine:
                                for _ in self.beams]
             self.dose = torch.zeros(self.shape, dtype=torch.float32, device=self.device)
 
-    def compute(self):
-        """Beam doses and the plan dose (device tensors), enqueued on the
-        current stream.  The radiological depth depends on the isocentre,
-        gantry and couch angles only (not on the leaves, collimator or
-        meterset), so consecutive control points with the same three reuse it:
-        a step-and-shoot beam traces its rays once."""
+    def compute(self, pipeline=None):
+        """Beam doses and the plan dose (device tensors), ordered on the
+        current stream (pipeline: None = as constructed, False = one workspace
+        on the current stream).  The radiological depth depends on the
+        isocentre, gantry and couch angles only (not on the leaves, collimator
+        or meterset), so consecutive control points with the same three reuse
+        it: a step-and-shoot beam traces its rays once."""
+        piped = len(self.wss) > 1 if pipeline is None else (bool(pipeline) and len(self.wss) > 1)
         with torch.cuda.device(self.device):
+            cur = torch.cuda.current_stream(self.device)
+            if piped:
+                # the side stream starts after the caller's earlier work (which
+                # may still read the workspaces)
+                self.side.wait_stream(cur)
+            done = [None, None]   # cone sum that last read workspace b
             self.dose.zero_()
             last = None
+            i = 0
             for (dmodel, cps), bd in zip(self.beams, self.beam_doses):
                 bd.zero_()
                 for cpc, of, key in cps:
-                    if key != last:
-                        raytrace(self.ws, cpc)
-                        last = key
-                    terma(self.ws, dmodel, cpc)
-                    ccc(self.ws, dmodel, cpc, bd, scale=of, accumulate=True)
+                    b = i & 1 if piped else 0
+                    ws = self.wss[b]
+                    if piped:
+                        with torch.cuda.stream(self.side):
+                            if done[b] is not None:
+                                self.side.wait_event(done[b])
+                            if key != last:
+                                raytrace(ws, cpc)
+                            terma(ws, dmodel, cpc)
+                            ready = torch.cuda.Event()
+                            ready.record(self.side)
+                        cur.wait_event(ready)
+                    else:
+                        if key != last:
+                            raytrace(ws, cpc)
+                        terma(ws, dmodel, cpc)
+                    last = key
+                    ccc(ws, dmodel, cpc, bd, scale=of, accumulate=True)
+                    if piped:
+                        done[b] = torch.cuda.Event()
+                        done[b].record(cur)
+                    i += 1
                 self.dose.add_(bd)
             self.dose.mul_(float(self.plan.n_fractions))
+            if piped:
+                cur.wait_stream(self.side)
         return self.dose, self.beam_doses
 
     def depth_traces(self):

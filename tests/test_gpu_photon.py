@@ -227,8 +227,9 @@ def test_reference_example_script_sampled(dev, varian, tmp_path):
 
 def test_engine_matches_dropin_and_is_deterministic(dev, varian):
     """PhotonDoseEngine (density from HU on the device, beams accumulated on
-    the device) equals the drop-in's plan dose to float rounding, and two runs
-    are bit-identical."""
+    the device) equals the drop-in's plan dose to float rounding; two runs
+    are bit-identical, and so are the pipelined (two workspaces, side stream)
+    and one-stream schedules."""
     from DoseCUDA import IMRTDoseGrid, IMRTPlan
     from DoseCUDA.photon import PhotonDoseEngine
     from DoseCUDA.plan_imrt import IMRTBeam
@@ -245,11 +246,18 @@ def test_engine_matches_dropin_and_is_deterministic(dev, varian):
     eng = PhotonDoseEngine(g, plan, gpu_id=0)
     d1, b1 = eng.compute()
     d1 = d1.cpu().numpy().copy()
-    d2 = eng.compute()[0].cpu().numpy()
+    b1 = [b.cpu().numpy().copy() for b in b1]
+    d2 = eng.compute()[0].cpu().numpy().copy()
     assert np.array_equal(d1, d2)
+    # one stream, one workspace: the same kernels in the same order, bit for bit
+    d3, b3 = eng.compute(pipeline=False)
+    assert np.array_equal(d1, d3.cpu().numpy())
+    assert all(np.array_equal(x, y.cpu().numpy()) for x, y in zip(b1, b3))
+    seq = PhotonDoseEngine(g, plan, gpu_id=0, pipeline=False)
+    assert np.array_equal(d1, seq.compute()[0].cpu().numpy())
     assert np.max(np.abs(d1 - g.dose)) < 1e-5 * g.dose.max()
     for b, hb in zip(b1, g.beam_doses):
-        assert np.max(np.abs(b.cpu().numpy() - hb)) < 1e-5 * hb.max()
+        assert np.max(np.abs(b - hb)) < 1e-5 * hb.max()
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (2, 3, 4), (5, 1, 7)])
