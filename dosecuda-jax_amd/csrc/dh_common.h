@@ -13,6 +13,13 @@ namespace dh {
 void set_error(const char *fmt, ...);
 int check_launch(const char *what);
 
+// Run radii of a block table (dh_wet.hip): float2 (value or NaN, radius) per
+// 4x4x4 block of cells, nbx x nby x nbz blocks, x fastest; s0, s1 two int
+// arrays of scratch of one int per block.  Radius r (0 .. 15): every block
+// within Chebyshev distance r of a uniform block that lies in the table holds
+// the same value.
+int block_radius_passes(float2 *table, int *s0, int *s1, int nbx, int nby, int nbz, hipStream_t st);
+
 // ---------------------------------------------------------------------------
 // kernel-argument copies of the ABI structs (passed by value)
 struct Grid {

@@ -11,8 +11,10 @@
 // per cell (the ray-trace reads only density: half the lines).  A trilinear
 // sample is two cells' forms blended, 16-byte buffer loads, with the texture's
 // border values (TERMA 0, density AIR_DENSITY) built into the padding and
-// returned outright for samples whose every corner lies outside the grid.  Work mapping as in the proton kernels:
-// one wave per 4x4x4 brick of voxels, lane = voxel, four bricks per workgroup.
+// returned outright for samples whose every corner lies outside the grid.  A
+// table of uniform 4x4x4 blocks of cells follows, for the ray-trace's runs.
+// Work mapping as in the proton kernels: one wave per 4x4x4 brick of voxels,
+// lane = voxel, four bricks per workgroup.
 #include "dh_common.h"
 
 namespace dh {
@@ -25,6 +27,8 @@ struct PGrid {
     float s, inv_s;
     float hx, hy, hz;    // nx + 1, ny + 1, nz + 1: a sample at padded index c reads the grid iff c < h (and c >= 0)
     unsigned rho_base;   // byte offset of the density-only quads (32 bytes x padded cells)
+    unsigned tab_off;    // byte offset of the block table (48 bytes x padded cells)
+    int nbx, nby, nbz;   // blocks of 4x4x4 sample cells per axis (cells 0 .. n: (n + 4) / 4)
 };
 
 static PGrid make_pgrid(const dh_grid &g) {
@@ -36,6 +40,8 @@ static PGrid make_pgrid(const dh_grid &g) {
     p.inv_s = 1.0f / g.spacing;
     p.hx = (float)(g.nx + 1); p.hy = (float)(g.ny + 1); p.hz = (float)(g.nz + 1);
     p.rho_base = (unsigned)(32LL * (g.nx + 2) * (g.ny + 2) * (g.nz + 2));
+    p.tab_off = (unsigned)(48LL * (g.nx + 2) * (g.ny + 2) * (g.nz + 2));
+    p.nbx = (g.nx + 4) / 4; p.nby = (g.ny + 4) / 4; p.nbz = (g.nz + 4) / 4;
     return p;
 }
 
@@ -43,16 +49,22 @@ static long long record_count(const dh_grid &g) {
     return (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
 }
 
+// Blocks of the density table: a sample's cell (padded index floor(c)) lies in
+// 0 .. n per axis, so (n + 1) cells, (n + 4) / 4 blocks.
+static long long photon_block_count(const dh_grid &g) {
+    return (long long)((g.nx + 4) / 4) * ((g.ny + 4) / 4) * ((g.nz + 4) / 4);
+}
+
 static bool photon_grid_ok(const dh_grid &g, const char *what) {
     if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0 || !(g.spacing > 0.0f)) {
         set_error("%s: invalid grid %d x %d x %d, spacing %g", what, g.nx, g.ny, g.nz, (double)g.spacing);
         return false;
     }
-    // 32-bit byte offsets of the record loads (vector offset + scalar offset of
-    // the density copy and the slice below 2^32)
-    if (48.0 * (double)record_count(g) >= 4294967296.0) {
-        set_error("%s: grid %d x %d x %d too large (48 * (nx+2)(ny+2)(nz+2) must stay below 2^32)", what, g.nx, g.ny,
-                  g.nz);
+    // 32-bit byte offsets of the record and table loads (vector offset + scalar
+    // offset of the density copy and the slice, the table after them, below 2^32)
+    if (48.0 * (double)record_count(g) + 8.0 * (double)photon_block_count(g) >= 4294967296.0) {
+        set_error("%s: grid %d x %d x %d too large (48 (nx+2)(ny+2)(nz+2) + 8 blocks must stay below 2^32)", what,
+                  g.nx, g.ny, g.nz);
         return false;
     }
     return true;
@@ -119,6 +131,32 @@ __global__ __launch_bounds__(256) void k_ph_density_from_hu(const float *__restr
     });
     gst(rec, 2 * t + 1, q);
     gst(rec, 2 * n + t, q);
+}
+
+// Block table after the records: one float2 per block of 4x4x4 sample cells
+// (padded cells 4b .. 4b + 3 per axis): the density every corner those cells
+// touch holds (padded indices 4b .. 4b + 4; AIR_DENSITY past the padded grid,
+// as the padding itself) or NaN where they differ, and the run radius, set by
+// block_radius_passes.  A sample in a cell of a uniform block is exactly that
+// density: every form it blends is (rho, 0, 0, 0).
+__global__ __launch_bounds__(256) void k_ph_density_blocks(const float4 *__restrict__ rho_q,
+                                                           float2 *__restrict__ table, PGrid g, long long nblk) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk) return;
+    const int bx = (int)(t % g.nbx);
+    const long long r = t / g.nbx;
+    const int by = (int)(r % g.nby), bz = (int)(r / g.nby);
+    const int py = g.pxy / g.px, pz = g.nz + 2;
+    auto val = [&](int i, int j, int k) -> float {
+        return (i < g.px && j < py && k < pz) ? gld(rho_q, (long long)i + (long long)g.px * (j + (long long)py * k)).x
+                                              : kAirDensity;
+    };
+    const float v = val(4 * bx, 4 * by, 4 * bz);
+    bool same = v == v;
+    for (int k = 4 * bz; k <= 4 * bz + 4 && same; ++k)
+        for (int j = 4 * by; j <= 4 * by + 4 && same; ++j)
+            for (int i = 4 * bx; i <= 4 * bx + 4 && same; ++i) same = val(i, j, k) == v;
+    table[t] = make_float2(same ? v : __int_as_float(0x7fc00000), 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -211,6 +249,14 @@ __device__ __forceinline__ float3 ph_voxel_xyz(const PGrid &g, const Geom &b, in
 // Radiological depth (rayTraceKernel, CudaClasses.cu:180-228) with the
 // reference's sample positions p_n = fma(u, n, v), texture coordinate
 // fma(1/s, p + iso, 1/2) and bounds test [0, n) per axis.
+//
+// Uniform runs: when sample n's block of cells is uniform with radius r, every
+// cell of the cube of blocks within r of it has that density, so the samples
+// that stay in the cube add it at once.  Per axis the run ends one step before
+// the ray reaches the cube's far face, and before padded index n (moving up) or
+// 1 (moving down): the run's samples lie inside the grid, and the margin of one
+// step covers the rounding of the positions.  In air and other uniform regions
+// one table read replaces the samples of up to 4 (2r + 1) cells.
 __global__ __launch_bounds__(256) void k_ph_raytrace(const float4 *__restrict__ rec, float *__restrict__ depth,
                                                      PGrid g, Geom b, BrickGrid bg, unsigned nblocks) {
     int i, j, k;
@@ -219,15 +265,26 @@ __global__ __launch_bounds__(256) void k_ph_raytrace(const float4 *__restrict__ 
     const float dx = b.src[0] - v.x, dy = b.src[1] - v.y, dz = b.src[2] - v.z;
     const float len = sqrtf(dx * dx + dy * dy + dz * dz);
     const float ux = dx / len, uy = dy / len, uz = dz / len;
-    const __amdgpu_buffer_rsrc_t rs = record_rsrc(rec, g);
+    const unsigned nblk = (unsigned)g.nbx * (unsigned)g.nby * (unsigned)g.nbz;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)rec, (short)0, (int)(g.tab_off + 8u * nblk), 0x00020000);
     const float nxf = (float)g.nx, nyf = (float)g.ny, nzf = (float)g.nz;
+    // per axis: cells per step, and the run's face: 4 b + off + 4 r sg, clamped at lim
+    const float dux = ux * g.inv_s, duy = uy * g.inv_s, duz = uz * g.inv_s;
+    const float rdx = dux != 0.0f ? 1.0f / dux : 0.0f, rdy = duy != 0.0f ? 1.0f / duy : 0.0f;
+    const float rdz = duz != 0.0f ? 1.0f / duz : 0.0f;
+    const float adx = dux != 0.0f ? 0.0f : 3.0e38f, ady = duy != 0.0f ? 0.0f : 3.0e38f;
+    const float adz = duz != 0.0f ? 0.0f : 3.0e38f;
+    const float sgx = dux > 0.0f ? 1.0f : -1.0f, sgy = duy > 0.0f ? 1.0f : -1.0f, sgz = duz > 0.0f ? 1.0f : -1.0f;
+    const int ofx = dux > 0.0f ? 4 : 0, ofy = duy > 0.0f ? 4 : 0, ofz = duz > 0.0f ? 4 : 0;
+    const unsigned nbxy = (unsigned)g.nbx * (unsigned)g.nby;
     float acc = -0.05f;
     // the two density forms of the cell of the lane's last sample: at 1 mm steps
     // a sample often lies in the cell of the one before and reads no memory
     unsigned off_prev = 0xffffffffu;
     float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), q1 = q0;
     const int so_z = g.pxy << 4;
-    for (int n = 0;; ++n) {
+    for (int n = 0;;) {
         const float fn = (float)n;
         const float tx = fmaf(g.inv_s, fmaf(ux, fn, v.x) + b.iso[0], 0.5f);
         const float ty = fmaf(g.inv_s, fmaf(uy, fn, v.y) + b.iso[1], 0.5f);
@@ -235,7 +292,28 @@ __global__ __launch_bounds__(256) void k_ph_raytrace(const float4 *__restrict__ 
         if (!(tx >= 0.0f && tx < nxf && ty >= 0.0f && ty < nyf && tz >= 0.0f && tz < nzf)) break;
         // padded index tex + 1/2 >= 1/2: truncation is floor
         const float cx = tx + 0.5f, cy = ty + 0.5f, cz = tz + 0.5f;
-        const unsigned off = cell_index(g, (int)cx, (int)cy, (int)cz) << 4;
+        const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
+        const int bx = ix >> 2, by = iy >> 2, bz = iz >> 2;
+        const unsigned bidx = (unsigned)bx + __umul24((unsigned)g.nbx, (unsigned)by) + __umul24(nbxy, (unsigned)bz);
+        const auto te = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(g.tab_off + (bidx << 3)), 0, 0);
+        const float u = __uint_as_float(te[0]);
+        if (u == u) {
+            const float r4 = 4.0f * __uint_as_float(te[1]);
+            const float fx = dux > 0.0f ? fminf(fmaf(r4, sgx, (float)((bx << 2) + ofx)), nxf)
+                                        : fmaxf(fmaf(r4, sgx, (float)((bx << 2) + ofx)), 1.0f);
+            const float fy = duy > 0.0f ? fminf(fmaf(r4, sgy, (float)((by << 2) + ofy)), nyf)
+                                        : fmaxf(fmaf(r4, sgy, (float)((by << 2) + ofy)), 1.0f);
+            const float fz = duz > 0.0f ? fminf(fmaf(r4, sgz, (float)((bz << 2) + ofz)), nzf)
+                                        : fmaxf(fmaf(r4, sgz, (float)((bz << 2) + ofz)), 1.0f);
+            const float sx = fmaf(fx - cx, rdx, adx), sy = fmaf(fy - cy, rdy, ady), sz = fmaf(fz - cz, rdz, adz);
+            const int T = (int)fminf(fminf(fminf(sx, sy), sz), 1.0e9f) - 1;
+            if (T >= 1) {
+                acc = fmaf(fmaxf(u, 0.0f) * (float)(T + 1), 0.1f, acc);
+                n += T + 1;
+                continue;
+            }
+        }
+        const unsigned off = cell_index(g, ix, iy, iz) << 4;
         if (off != off_prev) {
             q0 = load4(rs, off, (int)g.rho_base);
             q1 = load4(rs, off, (int)(g.rho_base + (unsigned)so_z));
@@ -245,6 +323,7 @@ __global__ __launch_bounds__(256) void k_ph_raytrace(const float4 *__restrict__ 
         const float r0 = bilerp(q0, ax, ay), r1 = bilerp(q1, ax, ay);
         const float rho = fmaf(__builtin_amdgcn_fractf(cz), r1 - r0, r0);
         acc = fmaf(fmaxf(rho, 0.0f), 0.1f, acc);
+        ++n;
     }
     gst(depth, vox_index(g, i, j, k), acc);
 }
@@ -447,7 +526,20 @@ using namespace dh;
 
 extern "C" int64_t dh_photon_record_len(dh_grid g) {
     if (!photon_grid_ok(g, "dh_photon_record_len")) return -1;
-    return 12 * record_count(g);
+    return 12 * record_count(g) + 4 * photon_block_count(g);
+}
+
+// The density's block table and run radii, after its records are written.
+static int density_blocks(float *records, const dh_grid &g, hipStream_t st) {
+    const long long P = record_count(g), nblk = photon_block_count(g);
+    const PGrid pg = make_pgrid(g);
+    float2 *table = reinterpret_cast<float2 *>(records + 12 * P);
+    int *s0 = reinterpret_cast<int *>(records + 12 * P + 2 * nblk);
+    hipLaunchKernelGGL(k_ph_density_blocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<const float4 *>(records) + 2 * P, table, pg, nblk);
+    const int rc = check_launch("k_ph_density_blocks");
+    if (rc) return rc;
+    return block_radius_passes(table, s0, s0 + nblk, pg.nbx, pg.nby, pg.nbz, st);
 }
 
 static int records_ok(const void *rec, const char *what) {
@@ -467,7 +559,8 @@ extern "C" int dh_photon_density(const float *density, float *records, dh_grid g
     const long long n = record_count(g);
     hipLaunchKernelGGL(k_ph_records_from, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        density, reinterpret_cast<float4 *>(records), make_pgrid(g), n, 1, kAirDensity);
-    return check_launch("k_ph_records_from");
+    const int rc = check_launch("k_ph_records_from");
+    return rc ? rc : density_blocks(records, g, (hipStream_t)stream);
 }
 
 extern "C" int dh_photon_density_from_hu(const float *hu, float *records, dh_grid g, const double *hu_knots,
@@ -492,7 +585,8 @@ extern "C" int dh_photon_density_from_hu(const float *hu, float *records, dh_gri
     const long long n = record_count(g);
     hipLaunchKernelGGL(k_ph_density_from_hu, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, hu,
                        reinterpret_cast<float4 *>(records), make_pgrid(g), kn, n);
-    return check_launch("k_ph_density_from_hu");
+    const int rc = check_launch("k_ph_density_from_hu");
+    return rc ? rc : density_blocks(records, g, (hipStream_t)stream);
 }
 
 extern "C" int dh_photon_terma_records(const float *terma, float *records, dh_grid g, void *stream) {

@@ -109,6 +109,21 @@ __global__ __launch_bounds__(256) void k_block_radius_set(float2 *__restrict__ t
     table[t].y = (float)max(f[t] - 1, 0);
 }
 
+int block_radius_passes(float2 *table, int *s0, int *s1, int nbx, int nby, int nbz, hipStream_t st) {
+    const long long nblk = (long long)nbx * nby * nbz;
+    const dim3 grid((unsigned)((nblk + 255) / 256));
+    int *const out[3] = {s0, s1, s0};
+    const int *const in[3] = {nullptr, s0, s1};
+    for (int axis = 0; axis < 3; ++axis) {
+        hipLaunchKernelGGL(k_block_radius, grid, dim3(256), 0, st, (const float2 *)table, in[axis], out[axis], nbx,
+                           nby, nbz, axis);
+        const int rc = check_launch("k_block_radius");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_block_radius_set, grid, dim3(256), 0, st, table, (const int *)s0, nblk);
+    return check_launch("k_block_radius_set");
+}
+
 // ---------------------------------------------------------------------------
 // HU -> RLSP (IMPTDoseGrid.RLSPFromHU, plan_impt.py:158-168)
 // (Knots, interp_knots: dh_common.h)
@@ -880,18 +895,9 @@ static int launch_blocks(float *rlsp_padded, const dh_grid &g, hipStream_t st) {
     const dim3 grid((unsigned)((nblk + 255) / 256));
     hipLaunchKernelGGL(k_rlsp_blocks, grid, dim3(256), 0, st, reinterpret_cast<const float4 *>(rlsp_padded), table,
                        make_grid(g), nblk);
-    int rc = check_launch("k_rlsp_blocks");
+    const int rc = check_launch("k_rlsp_blocks");
     if (rc) return rc;
-    const int nbx = (g.nx + 3) / 4, nby = (g.ny + 3) / 4, nbz = (g.nz + 3) / 4;
-    int *const out[3] = {s0, s1, s0};
-    const int *const in[3] = {nullptr, s0, s1};
-    for (int axis = 0; axis < 3; ++axis) {
-        hipLaunchKernelGGL(k_block_radius, grid, dim3(256), 0, st, (const float2 *)table, in[axis], out[axis], nbx,
-                           nby, nbz, axis);
-        if ((rc = check_launch("k_block_radius"))) return rc;
-    }
-    hipLaunchKernelGGL(k_block_radius_set, grid, dim3(256), 0, st, table, (const int *)s0, nblk);
-    return check_launch("k_block_radius_set");
+    return dh::block_radius_passes(table, s0, s1, (g.nx + 3) / 4, (g.ny + 3) / 4, (g.nz + 3) / 4, st);
 }
 
 extern "C" int dh_rlsp_from_hu(const float *hu, float *rlsp_padded, dh_grid g, const double *hu_knots,

@@ -397,8 +397,14 @@ int dh_batch_adjoint(dh_batch *batch, int32_t flags, void *stream);
  * 0.0012, IMRTClasses.cuh:6).  A sample at continuous voxel index c (texture
  * coordinate c + 1/2) is then the blend of two cells' forms at c + 1, with
  * the border values wherever the texture would read only border texels.
- * dh_photon_record_len gives the floats of a record volume (12 P; 32-byte
- * aligned, below 2^32 bytes).
+ * After them, from float 12 P on, a block table of the density: one float
+ * pair per block of 4x4x4 sample cells (padded cells 4b .. 4b+3 per axis,
+ * blocks (n+4)/4 per axis, x fastest), the density every corner of those
+ * cells holds (NaN where they differ) and the block's run radius, then two
+ * ints per block of scratch; the density entry points write it, and the
+ * ray-trace adds the samples that stay in a uniform cube of blocks at once.
+ * dh_photon_record_len gives the floats of a record volume (12 P + 4 blocks;
+ * 32-byte aligned, below 2^32 bytes).
  * ---------------------------------------------------------------------- */
 
 /* MLC leaf pair (MLCPair, IMRTClasses.cuh:8-15), mm at the isocentre plane. */
@@ -438,8 +444,8 @@ typedef struct dh_photon_beam {
 /* Floats of a record volume for grid g. */
 int64_t dh_photon_record_len(dh_grid g);
 
-/* Density half of the records from a density volume (nz*ny*nx floats; the
- * reference's DensityTexture, IMRTClasses.cu:394). */
+/* Density half of the records and its block table from a density volume
+ * (nz*ny*nx floats; the reference's DensityTexture, IMRTClasses.cu:394). */
 int dh_photon_density(const float *density, float *records, dh_grid g, void *stream);
 
 /* The same from HU, density = np.interp(HU, hu_knots, density_knots) rounded

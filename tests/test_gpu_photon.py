@@ -111,7 +111,7 @@ def test_density_records_from_hu(dev):
         allr = ws.records.cpu().numpy()
         torch.cuda.synchronize()
     npad = 7 * 8 * 9
-    rec = allr[8 * npad:].reshape(7, 8, 9, 4)            # density-only copy (k', j', i', 4 floats)
+    rec = allr[8 * npad:12 * npad].reshape(7, 8, 9, 4)   # density-only copy (k', j', i', 4 floats)
     pairs = allr[:8 * npad].reshape(7, 8, 9, 8)         # (TERMA form, density form) per cell
     np.testing.assert_array_equal(pairs[..., 4:], rec)
     a = rec[..., 0]
@@ -123,6 +123,101 @@ def test_density_records_from_hu(dev):
     # the face form: a + (b - a) is the +x neighbour b up to the rounding of the float32 difference
     np.testing.assert_allclose(rec[1:-1, 1:-1, 1:-2, 0] + rec[1:-1, 1:-1, 1:-2, 1], want[:, :, 1:], rtol=0,
                                atol=4e-7)
+
+
+def _block_table_bruteforce(a, shape):
+    """Block table of a padded density corner array a (k', j', i'): per block
+    of 4x4x4 sample cells (cells 0 .. n per axis) the common value of corners
+    4b .. 4b + 4 (AIR_DENSITY past the padded grid) or NaN, and the largest
+    r <= 15 with every block of the table within Chebyshev distance r equal."""
+    nb = [(n + 4) // 4 for n in shape]                  # (z, y, x)
+    ext = np.full([4 * m + 1 for m in nb], np.float32(0.0012), np.float32)
+    ext[:a.shape[0], :a.shape[1], :a.shape[2]] = a
+    val = np.full(nb, np.nan, np.float32)
+    for bz in range(nb[0]):
+        for by in range(nb[1]):
+            for bx in range(nb[2]):
+                c = ext[4 * bz:4 * bz + 5, 4 * by:4 * by + 5, 4 * bx:4 * bx + 5]
+                if np.all(c == c.flat[0]):
+                    val[bz, by, bx] = c.flat[0]
+    rad = np.zeros(nb, np.float32)
+    for idx in np.ndindex(*nb):
+        v = val[idx]
+        if v != v:
+            continue
+        r = 0
+        while r < 15:
+            lo = [max(i - r - 1, 0) for i in idx]
+            hi = [min(i + r + 2, m) for i, m in zip(idx, nb)]
+            if not np.all(val[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] == v):
+                break
+            r += 1
+        rad[idx] = r
+    return val, rad
+
+
+def _uniform_grid(shape, spacing):
+    """Air around a water ellipsoid of exactly 0 HU with a bone rod and a lung
+    sphere: large uniform regions (the ray-trace's runs) beside edges."""
+    from DoseCUDA import IMRTDoseGrid
+    g = IMRTDoseGrid()
+    g.spacing = np.array([spacing] * 3, dtype=np.float64)
+    g.size = np.array(shape)
+    g.origin = np.array([-shape[2] * spacing / 2.0, -shape[1] * spacing / 2.0, -shape[0] * spacing / 2.0])
+    z, y, x = np.meshgrid(*[np.arange(n) - (n - 1) / 2.0 for n in shape], indexing="ij")
+    hu = np.full(shape, -1000.0, np.float32)
+    hu[(x / (0.3 * shape[2])) ** 2 + (y / (0.3 * shape[1])) ** 2 + (z / (0.35 * shape[0])) ** 2 <= 1.0] = 0.0
+    hu[((x - 3) ** 2 + (y + 2) ** 2 <= 9) & (np.abs(z) < 0.3 * shape[0])] = 1000.0
+    hu[(x + 5) ** 2 + (y - 4) ** 2 + (z - 2) ** 2 <= 20] = -700.0
+    g.HU = hu
+    return g
+
+
+def test_density_block_table(dev):
+    """The block table after the records (value or NaN per 4x4x4 block of
+    sample cells, run radius) equals a brute force of its definition on a
+    phantom with uniform regions and edges."""
+    from DoseCUDA import photon as PH
+    _say("density block table")
+    g = _uniform_grid((60, 64, 72), 2.0)
+    tab = np.loadtxt(os.path.join(PHOT, "VarianTrueBeamHF", "HU_Density.csv"), delimiter=",", skiprows=1)
+    shape = g.HU.shape
+    with torch.cuda.device(dev):
+        ws = PH.PhotonWorkspace(shape, 2.0, dev)
+        ws.set_density_from_hu(torch.from_numpy(g.HU).to(dev), tab[:, 0], tab[:, 1])
+        allr = ws.records.cpu().numpy()
+    npad = int(np.prod([n + 2 for n in shape]))
+    a = allr[8 * npad:12 * npad].reshape(*[n + 2 for n in shape], 4)[..., 0]
+    val, rad = _block_table_bruteforce(a, shape)
+    nb = val.size
+    got = allr[12 * npad:12 * npad + 2 * nb].reshape(*val.shape, 2)
+    np.testing.assert_array_equal(got[..., 0], val)
+    np.testing.assert_array_equal(got[..., 1], rad)
+    assert np.sum(val == val) > nb // 4 and rad.max() >= 2     # the phantom has runs to take
+
+
+@pytest.mark.parametrize("ga,ta,spacing", [(0.0, 0.0, 2.0), (90.0, 0.0, 2.5), (33.0, 20.0, 2.0), (250.0, 335.0, 3.0)])
+def test_raytrace_uniform_regions_vs_oracle(dev, varian, ga, ta, spacing):
+    """Depth of every voxel vs the oracle on a phantom of large uniform regions
+    (runs of samples added at once), axis-aligned (gantry 0 / 90: samples on
+    cell boundaries, rays along one axis) and oblique."""
+    from DoseCUDA import photon as PH
+    _say("uniform-region ray-trace gantry %g couch %g @ %g mm" % (ga, ta, spacing))
+    g = _uniform_grid((48, 56, 64), spacing)
+    model = varian.beam_models[0]
+    cp = _cp(varian, 0, ga, ta, 0.0, 50.0, 50.0)
+    rho = OR.density_from_hu(g.HU, os.path.join(PHOT, "VarianTrueBeamHF"))
+    dmodel = PH.DevicePhotonModel(model, dev)
+    with torch.cuda.device(dev):
+        ws = PH.PhotonWorkspace(g.HU.shape, float(np.float32(spacing)), dev)
+        ws.set_density(torch.from_numpy(rho.astype(np.float32)).to(dev))
+        cpc = PH.control_point(cp, g.origin, dmodel, dev)
+        PH.raytrace(ws, cpc)
+        depth = ws.depth.cpu().numpy()
+    ref = OR.raytrace(rho, float(np.float32(spacing)), _geom(g, model, cp))
+    err = np.max(np.abs(depth - ref))
+    _say("  depth %.2e g/cm^2 (max %.1f)" % (err, ref.max()))
+    assert err < 2e-4, err
 
 
 @pytest.mark.parametrize("machine,energy,ga,ta,ca,spacing", [

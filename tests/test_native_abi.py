@@ -252,7 +252,8 @@ int main(void) {
                                                 N.dh_photon_beam.mu.offset]
     lib = N.load()
     g = N.dh_grid(10, 12, 14, 2.0)
-    assert lib.dh_photon_record_len(g) == 12 * 12 * 14 * 16
+    # 12 floats per padded cell, then 4 per block of 4x4x4 sample cells (11 x 13 x 15 cells)
+    assert lib.dh_photon_record_len(g) == 12 * 12 * 14 * 16 + 4 * 3 * 4 * 4
     assert lib.dh_photon_record_len(N.dh_grid(0, 12, 14, 2.0)) < 0
     assert lib.dh_photon_record_len(N.dh_grid(2000, 2000, 2000, 1.0)) < 0   # 32-bit record offsets
     assert "too large" in N.last_error()
