@@ -313,17 +313,28 @@ __global__ __launch_bounds__(256) void k_ph_raytrace(const float4 *__restrict__ 
                 continue;
             }
         }
-        const unsigned off = cell_index(g, ix, iy, iz) << 4;
-        if (off != off_prev) {
-            q0 = load4(rs, off, (int)g.rho_base);
-            q1 = load4(rs, off, (int)(g.rho_base + (unsigned)so_z));
-            off_prev = off;
-        }
-        const float ax = __builtin_amdgcn_fractf(cx), ay = __builtin_amdgcn_fractf(cy);
-        const float r0 = bilerp(q0, ax, ay), r1 = bilerp(q1, ax, ay);
-        const float rho = fmaf(__builtin_amdgcn_fractf(cz), r1 - r0, r0);
-        acc = fmaf(fmaxf(rho, 0.0f), 0.1f, acc);
-        ++n;
+        auto sample = [&](float sx, float sy, float sz) {
+            const unsigned off = cell_index(g, (int)sx, (int)sy, (int)sz) << 4;
+            if (off != off_prev) {
+                q0 = load4(rs, off, (int)g.rho_base);
+                q1 = load4(rs, off, (int)(g.rho_base + (unsigned)so_z));
+                off_prev = off;
+            }
+            const float ax = __builtin_amdgcn_fractf(sx), ay = __builtin_amdgcn_fractf(sy);
+            const float r0 = bilerp(q0, ax, ay), r1 = bilerp(q1, ax, ay);
+            const float rho = fmaf(__builtin_amdgcn_fractf(sz), r1 - r0, r0);
+            acc = fmaf(fmaxf(rho, 0.0f), 0.1f, acc);
+        };
+        sample(cx, cy, cz);
+        // the next sample too, without a table read (outside a uniform block it
+        // is most often in the same block or another non-uniform one)
+        const float f1 = (float)(n + 1);
+        const float ux1 = fmaf(g.inv_s, fmaf(ux, f1, v.x) + b.iso[0], 0.5f);
+        const float uy1 = fmaf(g.inv_s, fmaf(uy, f1, v.y) + b.iso[1], 0.5f);
+        const float uz1 = fmaf(g.inv_s, fmaf(uz, f1, v.z) + b.iso[2], 0.5f);
+        if (!(ux1 >= 0.0f && ux1 < nxf && uy1 >= 0.0f && uy1 < nyf && uz1 >= 0.0f && uz1 < nzf)) break;
+        sample(ux1 + 0.5f, uy1 + 0.5f, uz1 + 0.5f);
+        n += 2;
     }
     gst(depth, vox_index(g, i, j, k), acc);
 }
