@@ -176,31 +176,71 @@ __device__ __forceinline__ unsigned cell_index(const PGrid &g, int ix, int iy, i
 }
 
 // TERMA and density at padded index c, border values where the texture would
-// read only border texels (some axis outside [0, n + 1)).
+// read only border texels (some axis outside [0, n + 1)).  Lanes that are not
+// live read nothing (their te, rho are not used).
 __device__ __forceinline__ void sample_tr(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy, float cz,
-                                          float &te, float &rho) {
+                                          bool live, float &te, float &rho) {
     const bool in = cx >= 0.0f && cx < g.hx && cy >= 0.0f && cy < g.hy && cz >= 0.0f && cz < g.hz;
     const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
     const unsigned off = in ? cell_index(g, (int)fx, (int)fy, (int)fz) << 5 : 0u;
     const int so_z = g.pxy << 5;
     const float ax = cx - fx, ay = cy - fy, az = cz - fz;
-    const float4 t0 = load4(rs, off, 0), d0 = load4(rs, off + 16u, 0);
-    const float4 t1 = load4(rs, off, so_z), d1 = load4(rs, off + 16u, so_z);
+    float4 t0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d0 = t0, t1 = t0, d1 = t0;
+    if (live) {
+        t0 = load4(rs, off, 0);
+        d0 = load4(rs, off + 16u, 0);
+        t1 = load4(rs, off, so_z);
+        d1 = load4(rs, off + 16u, so_z);
+    }
     const float tt0 = bilerp(t0, ax, ay), tt1 = bilerp(t1, ax, ay);
     const float dd0 = bilerp(d0, ax, ay), dd1 = bilerp(d1, ax, ay);
     te = in ? fmaf(az, tt1 - tt0, tt0) : 0.0f;
     rho = in ? fmaf(az, dd1 - dd0, dd0) : kAirDensity;
 }
 
-// The same for a padded index known to lie in [0, n + 1) on every axis.
+// A byte offset past every record volume's end (48 bytes per padded cell plus
+// the block table stay below 2^32 - 32 for any grid that passes
+// photon_grid_ok): a buffer load there, or 16 bytes on, returns zeros without
+// a memory access.
+constexpr unsigned kNoRecord = 0xffffffe0u;
+
+__device__ __forceinline__ float4 bpermute4(int addr, float4 x) {
+    return make_float4(__int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x.x))),
+                       __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x.y))),
+                       __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x.z))),
+                       __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x.w))));
+}
+
+// The same for a padded index known to lie in [0, n + 1) on every axis, with
+// the upper face shared: lane l + 16 is the voxel one z-layer up in the brick
+// and its rays run parallel to lane l's to within a few hundredths of a cell,
+// so at the same step it mostly samples the cell one slice up, whose lower
+// face (the forms at byte offset off + slice) is lane l's upper face.  A lane
+// takes those forms from lane l + 16 through the lane crossbar when that
+// lane's cell offset is exactly off + slice, and reads them itself otherwise
+// (the top layer of the brick, cells that differ near a cell boundary, a
+// partner that is not live): the same bytes either way, so the same results,
+// with fewer cache-line lookups for the upper faces.  up = byte address of
+// lane l + 16 for ds_bpermute (the top layer wraps to the bottom one, whose
+// offsets never match); a lane that is not live has offset kNoRecord.
 __device__ __forceinline__ void sample_tr_inside(__amdgpu_buffer_rsrc_t rs, const PGrid &g, float cx, float cy,
-                                                 float cz, float &te, float &rho) {
-    // c >= 0: truncation is floor
-    const unsigned off = cell_index(g, (int)cx, (int)cy, (int)cz) << 5;
-    const int so_z = g.pxy << 5;
+                                                 float cz, bool live, int up, float &te, float &rho) {
+    // c >= 0: truncation is floor.  Loads at offset kNoRecord (past the end of
+    // the buffer) return zeros without a memory access: lanes that are not
+    // live, and upper faces taken from the partner, read nothing.
+    const unsigned so_z = (unsigned)g.pxy << 5;
+    const unsigned off = live ? cell_index(g, (int)cx, (int)cy, (int)cz) << 5 : kNoRecord;
+    const unsigned poff = (unsigned)__builtin_amdgcn_ds_bpermute(up, (int)off);
+    const bool share = live && poff == off + so_z;
+    const unsigned off1 = (share || !live) ? kNoRecord : off + so_z;
     const float ax = __builtin_amdgcn_fractf(cx), ay = __builtin_amdgcn_fractf(cy), az = __builtin_amdgcn_fractf(cz);
     const float4 t0 = load4(rs, off, 0), d0 = load4(rs, off + 16u, 0);
-    const float4 t1 = load4(rs, off, so_z), d1 = load4(rs, off + 16u, so_z);
+    float4 t1 = load4(rs, off1, 0), d1 = load4(rs, off1 + 16u, 0);
+    const float4 tu = bpermute4(up, t0), du = bpermute4(up, d0);
+    if (share) {
+        t1 = tu;
+        d1 = du;
+    }
     const float tt0 = bilerp(t0, ax, ay), tt1 = bilerp(t1, ax, ay);
     const float dd0 = bilerp(d0, ax, ay), dd1 = bilerp(d1, ax, ay);
     te = fmaf(az, tt1 - tt0, tt0);
@@ -442,13 +482,17 @@ struct PhotonCccArgs {
 __global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, const float *__restrict__ terma,
                                                 const float *__restrict__ electron, float *__restrict__ dose, PGrid g,
                                                 Geom b, PhotonCccArgs a, BrickGrid bg, unsigned nblocks) {
-    int i, j, k;
-    if (!brick_voxel(bg, g, nblocks, i, j, k)) return;
+    // Every lane of the wave stays to the end (the shared upper faces read
+    // other lanes' offsets); a lane is live when its voxel is in the grid and
+    // has TERMA > 0.01, and only live lanes read memory or write a dose.
+    int i = 0, j = 0, k = 0;
+    const bool in = brick_voxel(bg, g, nblocks, i, j, k);
+    if (!in) i = j = k = 0;   // a lane past the grid's edge computes a voxel it does not keep
     const long long v = vox_index(g, i, j, k);
-    if (!(gld(terma, v) > 0.01f)) {
-        if (!a.accumulate) gst(dose, v, 0.0f);
-        return;
-    }
+    const bool live = in && gld(terma, v) > 0.01f;
+    if (in && !live && !a.accumulate) gst(dose, v, 0.0f);
+    if (!__any(live)) return;   // wave-uniform
+    const int up = (int)(((threadIdx.x & 63u) + 16u) << 2);
     const float3 p = ph_voxel_xyz(g, b, i, j, k);
     // kernelTilt (IMRTClasses.cu:173-203): rotation about a x u taking the
     // isocentre's direction a to the voxel's direction u to the source, as the
@@ -499,19 +543,33 @@ __global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, 
                 rsum = fmaf(rc, fmaf(bm_sp, rsum, te * k2_sp), rsum);
             };
             const float fx = fmaf(ddx, len0, c0x), fy = fmaf(ddy, len0, c0y), fz = fmaf(ddz, len0, c0z);
-            const bool far_in = fx >= 0.0f && fx < g.hx && fy >= 0.0f && fy < g.hy && fz >= 0.0f && fz < g.hz;
+            const bool far_in =
+                !live || (fx >= 0.0f && fx < g.hx && fy >= 0.0f && fy < g.hy && fz >= 0.0f && fz < g.hz);
             float rl = len0;
             if (__all(far_in)) {
-#pragma unroll 2
-                for (int t = 0; t < nsteps; ++t, rl = rl - sp) {
+                // two samples per trip (their loads in flight together)
+                int t = 0;
+                for (; t + 1 < nsteps; t += 2) {
+                    const float rl1 = rl - sp;
+                    float te0, rho0, te1, rho1;
+                    sample_tr_inside(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), live, up, te0,
+                                     rho0);
+                    sample_tr_inside(rs, g, fmaf(ddx, rl1, c0x), fmaf(ddy, rl1, c0y), fmaf(ddz, rl1, c0z), live, up,
+                                     te1, rho1);
+                    step(te0, rho0);
+                    step(te1, rho1);
+                    rl = rl1 - sp;
+                }
+                if (t < nsteps) {
                     float te, rho;
-                    sample_tr_inside(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), te, rho);
+                    sample_tr_inside(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), live, up, te,
+                                     rho);
                     step(te, rho);
                 }
             } else {
                 for (int t = 0; t < nsteps; ++t, rl = rl - sp) {
                     float te, rho;
-                    sample_tr(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), te, rho);
+                    sample_tr(rs, g, fmaf(ddx, rl, c0x), fmaf(ddy, rl, c0y), fmaf(ddz, rl, c0z), live, te, rho);
                     step(te, rho);
                 }
             }
@@ -520,6 +578,7 @@ __global__ __launch_bounds__(256) void k_ph_ccc(const float4 *__restrict__ rec, 
         }
         sum = fmaf(a.am[cone], cone_sum_p, fmaf(bm, cone_sum_s, sum));
     }
+    if (!live) return;
     sum += gld(electron, v);
     const float d = (sum == sum && sum >= 0.0f) ? a.mu_cal * sum * a.mu : 0.0f;
     gst(dose, v, a.accumulate ? fmaf(a.scale, d, gld(dose, v)) : a.scale * d);
