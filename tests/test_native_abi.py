@@ -269,3 +269,20 @@ int main(void) {
     assert lib.dh_photon_raytrace(ctypes.c_void_p(16), ctypes.c_void_p(64), g, ctypes.byref(beam), None) == \
         N.DH_ERR_ARG
     assert "32-byte aligned" in N.last_error()
+    # model checks, all before any launch (the pointers are never dereferenced on the host)
+    fake = ctypes.c_void_p(4096)
+    model.profile = model.spectrum = 4096
+    model.n_profile, model.n_spectrum = 3, 2
+    model.primary_src_dist, model.scatter_src_dist = 1000.0, 1000.0
+    model.primary_src_size, model.scatter_src_size = 0.0, 30.0
+    assert lib.dh_photon_terma(fake, fake, fake, g, ctypes.byref(model), ctypes.byref(beam), None) == N.DH_ERR_ARG
+    assert "must be positive" in N.last_error()
+    model.primary_src_size = 1.0
+    model.kernel[2][5] = float("nan")
+    assert lib.dh_photon_ccc(fake, fake, fake, fake, g, ctypes.byref(model), ctypes.byref(beam), 1.0, 0,
+                             None) == N.DH_ERR_ARG
+    assert "cone 2" in N.last_error() and "not a number" in N.last_error()
+    model.kernel[2][5] = 1.0e6   # 5e6 steps of 0.2 cm
+    assert lib.dh_photon_ccc(fake, fake, fake, fake, g, ctypes.byref(model), ctypes.byref(beam), 1.0, 0,
+                             None) == N.DH_ERR_ARG
+    assert "beyond 4096 steps" in N.last_error()
