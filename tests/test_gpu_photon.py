@@ -262,6 +262,58 @@ def test_stages_vs_oracle(dev, machine, energy, ga, ta, ca, spacing):
     assert dose_ref.max() > 0 and err < 1e-4, err
 
 
+def _mixed_grid(shape, spacing, seed):
+    """The uniform-region phantom with its ellipsoid's lower half replaced by
+    noisy tissue: uniform runs and non-uniform samples on the same rays."""
+    g = _uniform_grid(shape, spacing)
+    rng = np.random.default_rng(seed)
+    hu = g.HU
+    nz = shape[0]
+    water = hu == 0.0
+    water[nz // 2:] = False
+    hu[water] = rng.normal(30.0, 40.0, size=int(water.sum())).astype(np.float32)
+    return g
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_geometries_vs_oracle(dev, seed):
+    """Randomised control points (machine, energy, gantry, couch, collimator,
+    spacing, isocentre offset, field size, irregular leaves) on a phantom with
+    uniform regions and noisy tissue: depth vs the oracle (2e-4 g/cm^2),
+    TERMA / electron vs the oracle on the GPU's depth (1e-5 of max), dose vs
+    the oracle on the GPU's TERMA (1e-4 of D_max)."""
+    from DoseCUDA import IMRTPlan
+    rng = np.random.default_rng(100 + seed)
+    machine = ("VarianTrueBeamHF", "ElektaVersaSibAxesse")[seed % 2]
+    plan = IMRTPlan(machine)
+    energy = int(rng.integers(len(plan.beam_models)))
+    ga, ta, ca = float(rng.uniform(0, 360)), float(rng.uniform(-40, 40)) % 360.0, float(rng.uniform(0, 360))
+    spacing = float(rng.choice([1.5, 2.0, 2.5, 3.0]))
+    shape = tuple(int(v) for v in rng.integers(24, 34, size=3))
+    iso = tuple(float(v) for v in rng.uniform(-8.0, 8.0, size=3))
+    hx, hy = float(rng.uniform(10, 30)), float(rng.uniform(10, 30))
+    _say("random %d: %s energy %d ga=%.1f ta=%.1f ca=%.1f s=%g shape %s" % (seed, machine, energy, ga, ta, ca,
+                                                                             spacing, shape))
+    g = _mixed_grid(shape, spacing, seed)
+    model = plan.beam_models[energy]
+    cp = _cp(plan, energy, ga, ta, ca, hx, hy, mu=60.0, iso=iso, irregular=True)
+    rho = OR.density_from_hu(g.HU, os.path.join(PHOT, machine))
+    rho, depth, terma, el, dose = _stages(g, model, cp, dev, density=rho)
+    geom = _geom(g, model, cp)
+    s = float(np.float32(g.spacing[0]))
+    d_ref = OR.raytrace(rho, s, geom)
+    assert np.max(np.abs(depth - d_ref)) < 2e-4, np.max(np.abs(depth - d_ref))
+    t_ref, e_ref = OR.terma(depth.astype(np.float64), rho.shape, s, geom, model, OR.mlc_rows(cp.mlc))
+    assert np.max(np.abs(terma - t_ref)) < 1e-5 * t_ref.max()
+    assert np.max(np.abs(el - e_ref)) < 1e-5 * max(e_ref.max(), 1e-30)
+    dose_ref = OR.ccc(terma, rho, el.astype(np.float64), s, geom, model.kernel, np.float32(model.mu_calibration),
+                      np.float32(cp.mu))
+    err = np.max(np.abs(dose - dose_ref)) / dose_ref.max()
+    _say("  depth %.2e, TERMA %.2e, dose %.2e of D_max" % (np.max(np.abs(depth - d_ref)),
+                                                           np.max(np.abs(terma - t_ref)) / t_ref.max(), err))
+    assert dose_ref.max() > 0 and err < 1e-4, err
+
+
 def test_dropin_two_field_plan_vs_oracle(dev, varian):
     """computeIMRTPlan (drop-in, host arrays) with two fields on a 60^3 @ 3 mm
     cube vs the full-volume oracle: 1e-3 of D_max, gamma 2%/2 mm >= 0.995."""
