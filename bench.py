@@ -494,36 +494,45 @@ def photon_record(args, steps, warmup, cpu_sample):
     ccc_ms = acc["k_ph_ccc"] / n_cp
     flop = 64.0 * samples / n_cp   # per launch: 64 FLOP per cone-ray sample (two trilinear fields, recursion)
     nvox = int(np.prod(g.HU.shape))
-    # CPU baseline: float32 oracle on a voxel sample of control point 0 (the GPU's TERMA volume as input)
+    # CPU baseline: float32 oracle on the in-field voxels of the first control points (each with the GPU's
+    # TERMA volume as input), up to cpu_sample voxels in all
     cpu = None
     if cpu_sample > 0:
         from oracle import imrt_oracle as OR
-        dmodel, cps = eng.beams[0]
-        cpc, _, _ = cps[0]
-        cp0 = plan.beam_list[0].cp_list[0]
-        PH.raytrace(eng.ws, cpc)
-        PH.terma(eng.ws, dmodel, cpc)
-        t32 = eng.ws.terma.cpu().numpy()
-        el = eng.ws.electron.cpu().numpy().astype(np.float64)
         rho = OR.density_from_hu(g.HU, os.path.join(PKG, "DoseCUDA", "lookuptables", "photons", plan.machine_name))
-        geom = OR.photon_geometry(np.asarray(cp0.iso, np.float32), np.asarray(g.origin, np.float32), cp0.ga, cp0.ta,
-                                  cp0.ca, np.float32(plan.beam_models[0].primary_source_distance))
         rng = np.random.default_rng(0)
-        hot = np.flatnonzero(t32.reshape(-1) > 0.01)
-        idx = np.sort(rng.choice(hot, size=min(cpu_sample, hot.size), replace=False)).astype(np.int64)
-        tc0 = time.perf_counter()
-        dep = OR.raytrace(rho, s, geom, idx=idx, precision="f32")
-        OR.terma(dep, rho.shape, s, geom, plan.beam_models[0], OR.mlc_rows(cp0.mlc), idx=idx, precision="f32")
-        OR.ccc(t32, rho, el.reshape(-1)[idx], s, geom, plan.beam_models[0].kernel,
-               np.float32(plan.beam_models[0].mu_calibration), np.float32(cp0.mu), idx=idx, precision="f32")
-        tc = time.perf_counter() - tc0
-        cpu_samples_per_s = idx.size * per_voxel / tc
-        cpu = {"value": cpu_samples_per_s / (samples / n_cp), "unit": "control points/s", "cores": os.cpu_count(),
+        labels = list(np.asarray(plan.dicom_energy_label).astype(str))
+        n_vox = n_samp = 0
+        tc = 0.0
+        used = 0
+        for (dmodel, cps), beam in zip(eng.beams, plan.beam_list):
+            for (cpc, _, _), cp in zip(cps, beam.cp_list):
+                if n_vox >= cpu_sample:
+                    break
+                PH.raytrace(eng.ws, cpc)
+                PH.terma(eng.ws, dmodel, cpc)
+                t32 = eng.ws.terma.cpu().numpy()
+                el = eng.ws.electron.cpu().numpy().astype(np.float64)
+                model = plan.beam_models[labels.index(str(beam.dicom_energy_label))]
+                geom = OR.photon_geometry(np.asarray(cp.iso, np.float32), np.asarray(g.origin, np.float32), cp.ga,
+                                          cp.ta, cp.ca, np.float32(model.primary_source_distance))
+                hot = np.flatnonzero(t32.reshape(-1) > 0.01)
+                idx = np.sort(rng.choice(hot, size=min(cpu_sample - n_vox, hot.size), replace=False)).astype(np.int64)
+                tc0 = time.perf_counter()
+                dep = OR.raytrace(rho, s, geom, idx=idx, precision="f32")
+                OR.terma(dep, rho.shape, s, geom, model, OR.mlc_rows(cp.mlc), idx=idx, precision="f32")
+                OR.ccc(t32, rho, el.reshape(-1)[idx], s, geom, model.kernel, np.float32(model.mu_calibration),
+                       np.float32(cp.mu), idx=idx, precision="f32")
+                tc += time.perf_counter() - tc0
+                n_vox += idx.size
+                n_samp += idx.size * per_voxel
+                used += 1
+        cpu = {"value": n_samp / tc / (samples / n_cp), "unit": "control points/s", "cores": os.cpu_count(),
                "kind": "port", "cpu_model": cpu_model(),
-               "sample": "%d in-field voxels of control point 0: depth, TERMA and the cone sum at "
-                         "them (float32 oracle build, OpenMP on %d threads), %.3g cone samples in "
-                         "%.1f s, scaled to the plan's %.3g samples per control point" %
-                         (idx.size, os.cpu_count(), idx.size * per_voxel, tc, samples / n_cp)}
+               "sample": "%d in-field voxels of the first %d control points: depth, TERMA and the cone sum at "
+                         "them (float32 oracle build, OpenMP on %d threads), %.3g cone samples in %.1f s, scaled "
+                         "to the plan's %.3g samples per control point" %
+                         (n_vox, used, os.cpu_count(), n_samp, tc, samples / n_cp)}
     rec = {
         "metric": "photon collapsed-cone plan dose: ms per plan and control points/s",
         "value": n_cp / (ms * 1e-3),
@@ -576,8 +585,9 @@ def main():
                          "optimisation loop (each prints its own JSON line)")
     ap.add_argument("--iters", type=int, default=50, help="optimisation iterations of the body workload")
     ap.add_argument("--uneven-plans", type=int, default=64, help="plans of the uneven workload")
-    ap.add_argument("--photon-cpu-sample", type=int, default=2000000,
-                    help="photon workload: in-field voxels of the CPU baseline's sample")
+    ap.add_argument("--photon-cpu-sample", type=int, default=6000000,
+                    help="photon workload: in-field voxels of the CPU baseline's sample (the first control "
+                         "points' in-field voxels, about 10 s of CPU work on 16 cores)")
     ap.add_argument("--no-photon", action="store_true",
                     help="head workload: leave out the photon plan measurement appended to its record")
     ap.add_argument("--eager-only", action="store_true",
