@@ -418,6 +418,110 @@ def run_body(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_small(args):
+    """The two small configurations on the 100^3 @ 2.5 mm water + slab grid: the 320-spot plan (1 beam,
+    5 layers; forward dose: WET + spot sum) and the 10k-spot target plan (forward + adjoint of
+    mean_target (D - 2)^2 + 0.1 mean_shell D^2). Per plan: the drop-in computeIMPTPlan (host arrays, every
+    spot), the plain device path (per-beam calls; dense, then with the 6-sigma cutoff) and the batched
+    HIP-graph step (reach-limited WET, cutoff 6); the float32 oracle on every voxel (OpenMP) beside them."""
+    import numpy as np
+    import torch
+    from DoseCUDA.batch import BatchedDose, GraphedStep
+    from DoseCUDA.engine import IMPTDoseEngine
+    from DoseCUDA.workloads import water_slab_plan, target_plan
+    from oracle import impt_oracle as O
+    reps = max(3, args.steps)
+
+    def timed(fn, n=reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    def cpu_oracle(g, plan, grad=None):
+        """float32 oracle on every voxel: ray-march, smoothing, dose (and the adjoint for grad)."""
+        O.lib("f32")
+        machine_dir = os.path.join(PKG, "DoseCUDA", "lookuptables", "protons", plan.machine_name)
+        rlsp = O.rlsp_from_hu(g.HU, machine_dir)
+        sp = float(np.float32(g.spacing[0]))
+        t0 = time.perf_counter()
+        for beam in plan.beam_list:
+            model = O.beam_model_for(plan, beam)
+            geom = O.beam_geometry(np.asarray(beam.iso, np.float32), np.asarray(g.origin, np.float32),
+                                   beam.gantry_angle, beam.couch_angle, model.VSADX, model.VSADY)
+            _, spots, layers, r80 = O.spots_and_layers(beam.spot_list, model.divergence_params)
+            marr = O.ModelArrays(model)
+            raw = O.raymarch(rlsp, sp, geom, precision="f32")
+            w = O.smooth(raw, sp, geom, precision="f32")
+            O.pb_dose(w, sp, geom, spots, layers, r80, marr, precision="f32")
+            if grad is not None:
+                O.pb_grad(w, grad, sp, geom, spots, layers, r80, marr, precision="f32")
+        return (time.perf_counter() - t0) * 1e3
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"metric": "small configurations: ms per plan, plain and batched paths and the CPU oracle",
+           "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+           "cpu": {"kind": "port", "threads": threads, "cpu_model": cpu_model()}}
+    # 320-spot water + slab plan: forward
+    log("water + slab plan (320 spots)")
+    g, plan = water_slab_plan()
+    g.computeIMPTPlan(plan)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.computeIMPTPlan(plan)
+    dropin = (time.perf_counter() - t0) / reps * 1e3
+    e0 = IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=0.0)
+    e6 = IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=6.0)
+    plain0 = timed(lambda: (e0.compute_wet(force=True), e0.dose()))
+    plain6 = timed(lambda: (e6.compute_wet(force=True), e6.dose()))
+    bd = BatchedDose([IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=6.0)], wet_mode="reach", tune=None)
+    gs = GraphedStep(lambda: bd.forward(), bd.device)
+    batched = timed(gs.replay)
+    out["water_slab_320_spots_forward"] = {
+        "dropin_host_arrays_ms": dropin, "plain_dense_ms": plain0, "plain_culled_ms": plain6,
+        "batched_graph_ms": batched, "cpu_oracle_ms": cpu_oracle(g, plan)}
+    # 10k-spot target plan: forward + adjoint
+    log("target plan (10k spots)")
+    g, plan, target, shell = target_plan()
+    dev = e0.device
+    tm = torch.from_numpy(target).to(dev)
+    sm = torch.from_numpy(shell).to(dev)
+    n_t, n_s = float(target.sum()), float(shell.sum())
+
+    def grad_of(d):
+        return torch.where(tm, (d - 2.0) * (2.0 / n_t), torch.zeros_like(d)) + \
+            torch.where(sm, d * (0.2 / n_s), torch.zeros_like(d))
+
+    res = {}
+    dense_dose = None
+    for name, cut in (("plain_dense_ms", 0.0), ("plain_culled_ms", 6.0)):
+        e = IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=cut)
+
+        def fa(e=e):
+            e.compute_wet(force=True)
+            e.adjoint(grad_of(e.dose()))
+        res[name] = timed(fa)
+        if dense_dose is None:
+            dense_dose = e.dose()
+    bd = BatchedDose([IMPTDoseEngine(g, plan, gpu_id=0, cutoff_sigma=6.0)], wet_mode="reach", tune=None)
+
+    def step():
+        d = bd.forward()[0]
+        bd.grad_doses[0].copy_(grad_of(d))
+        return bd.adjoint(None, reuse_prep=True, input_order=False)
+    gs = GraphedStep(step, bd.device)
+    res["batched_graph_ms"] = timed(gs.replay)
+    # the oracle's adjoint takes the dL/dD of the dense GPU dose (an input here, not a result)
+    gvol = grad_of(dense_dose).cpu().numpy().astype(np.float64)
+    res["cpu_oracle_ms"] = cpu_oracle(g, plan, gvol)
+    res["spots"] = int(sum(len(b.spot_list) for b in plan.beam_list))
+    out["target_10k_spots_forward_adjoint"] = res
+    print(json.dumps(out), flush=True)
+
+
 def run_photon(args):
     """Photon collapsed-cone plan on the head phantom: 7 beams x 5 MLC segments (35 control points); step =
     the whole plan's dose on the device (per control point: radiological depth, TERMA, TERMA records,
@@ -580,7 +684,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=800000,
                     help="voxels in the CPU-baseline sample (about 10-15 s of CPU work on 16 cores); 0 = skip")
     ap.add_argument("--no-tune", action="store_true", help="library default schedules instead of the autotuner's")
-    ap.add_argument("--workload", choices=("head", "uneven", "body", "photon"), default="head",
+    ap.add_argument("--workload", choices=("head", "uneven", "body", "photon", "small"), default="head",
                     help="head: the headline metric (default); uneven: 64 uneven plans batched; body: 512x512x300 "
                          "optimisation loop (each prints its own JSON line)")
     ap.add_argument("--iters", type=int, default=50, help="optimisation iterations of the body workload")
@@ -625,6 +729,8 @@ def main():
         return run_photon(args)
     if args.workload == "body":
         return run_body(args)
+    if args.workload == "small":
+        return run_small(args)
     K = args.plans
     log("building workload: %d head-phantom plans" % K)
     g, plans = plan_copies(K)
